@@ -1,0 +1,194 @@
+"""ctypes wrapper of the C restatement (liboracle.so) -- the checker.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg; the product (specfem3d_geotech_amd) never
+imports it.  Array layouts: see oracle/sgeo_oracle.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (make -C oracle lib)")
+        _LIB = C.CDLL(path)
+        d, i32, u8 = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_uint8)
+        _LIB.orc_gll_tables.argtypes = [d, d, d]
+        _LIB.orc_compute_cmat.argtypes = [C.c_double, C.c_double, d]
+        _LIB.orc_stress_invariant.argtypes = [d, d, d, d]
+        _LIB.orc_mohcouf.argtypes = [C.c_double] * 5
+        _LIB.orc_mohcouf.restype = C.c_double
+        _LIB.orc_mohcouq.argtypes = [C.c_double] * 3 + [d, d, d]
+        _LIB.orc_formm.argtypes = [d, d, d, d]
+        _LIB.orc_dt_viscoplas.argtypes = [C.c_int, d, d, d]
+        _LIB.orc_dt_viscoplas.restype = C.c_double
+        _LIB.orc_strength_reduction.argtypes = [C.c_double, C.c_int, C.c_int] + [d] * 8
+        _LIB.orc_strength_reduction.restype = C.c_int
+        _LIB.orc_stiffness_bodyload.argtypes = [C.c_int32, C.c_int32, d, i32, i32,
+                                                d, d, d, d, d, d, d]
+        _LIB.orc_ebe_matvec.argtypes = [C.c_int32, C.c_int32, d, i32, d, d]
+        _LIB.orc_pcg_solver.argtypes = [C.c_int32, C.c_int32, d, d, d, d, i32,
+                                        C.c_double, C.c_int32, i32]
+        _LIB.orc_pcg_solver.restype = C.c_int
+        _LIB.orc_plastic_body.argtypes = [C.c_int32, C.c_int32, d, i32, i32,
+                                          d, d, d, d, d, C.c_double, d, d, d, d,
+                                          C.c_int, u8]
+        _LIB.orc_plastic_body.restype = C.c_double
+        _LIB.orc_plastic_body_excav.argtypes = [C.c_int32, C.c_int32, i32, d, i32,
+                                                i32, d, d, d, d, d, C.c_double,
+                                                d, d, d, d, C.c_int, i32, d]
+        _LIB.orc_plastic_body_excav.restype = C.c_double
+        _LIB.orc_slope3d.argtypes = ([C.c_int32] * 3 + [d, i32, i32, i32, C.c_int]
+                                     + [d] * 6 + [C.c_int, C.c_double, C.c_int32,
+                                                  C.c_double, C.c_int32, C.c_int, d,
+                                                  i32, i32, d, d, d, d, i32,
+                                                  C.c_int32, i32])
+        _LIB.orc_slope3d.restype = C.c_int
+    return _LIB
+
+
+def _p(a, ct=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ct)) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def gll_tables():
+    w = np.zeros(27); dl = np.zeros(3 * 27 * 27); ds = np.zeros(3 * 8 * 27)
+    lib().orc_gll_tables(_p(w), _p(dl), _p(ds))
+    return w, dl.reshape(27, 27, 3), ds.reshape(27, 8, 3)
+
+
+def cmat(E, nu):
+    c = np.zeros(36)
+    lib().orc_compute_cmat(E, nu, _p(c))
+    return c.reshape(6, 6).T.copy()
+
+
+def stress_invariant(s):
+    s = _f64(s); a, b, t = C.c_double(), C.c_double(), C.c_double()
+    lib().orc_stress_invariant(_p(s), C.byref(a), C.byref(b), C.byref(t))
+    return a.value, b.value, t.value
+
+
+def mohcouf(phi, c, sigm, dsbar, theta):
+    return lib().orc_mohcouf(phi, c, sigm, dsbar, theta)
+
+
+def mohcouq(psi, dsbar, theta):
+    a, b, c = C.c_double(), C.c_double(), C.c_double()
+    lib().orc_mohcouq(psi, dsbar, theta, C.byref(a), C.byref(b), C.byref(c))
+    return a.value, b.value, c.value
+
+
+def formm(s):
+    s = _f64(s); m = [np.zeros(36) for _ in range(3)]
+    lib().orc_formm(_p(s), *(_p(x) for x in m))
+    return [x.reshape(6, 6).T.copy() for x in m]
+
+
+def dt_viscoplas(nuf, phif, ym):
+    nuf, phif, ym = _f64(nuf), _f64(phif), _f64(ym)
+    return lib().orc_dt_viscoplas(len(nuf), _p(nuf), _p(phif), _p(ym))
+
+
+def strength_reduction(srf, phinu, coh, nu, phi, psi):
+    coh, nu, phi, psi = map(_f64, (coh, nu, phi, psi))
+    out = [np.zeros_like(coh) for _ in range(4)]
+    st = lib().orc_strength_reduction(srf, int(phinu), len(coh), _p(coh), _p(nu),
+                                      _p(phi), _p(psi), *(_p(o) for o in out))
+    return (*out, st)   # cohf, nuf, phif, psif, istat
+
+
+def stiffness_bodyload(spec, gam, nu, ym, eqk=None, with_extload=True):
+    """Returns (storekm (E,81,81) with [e, c, r] = k(r,c,e), dprecon
+    (unassembled, not inverted), extload)."""
+    E, neq = spec.nelmt, spec.neq
+    km = np.zeros((E, 81, 81))
+    dp = np.zeros(neq + 1)
+    ext = np.zeros(neq + 1) if with_extload else None
+    xc = _f64(spec.corners)
+    g = _i32(spec.gdof_elmt)
+    eq = _f64(eqk) if eqk is not None else None
+    lib().orc_stiffness_bodyload(E, neq, _p(xc), _p(g, C.c_int32),
+                                 _p(_i32(spec.mat_id), C.c_int32), _p(_f64(gam)),
+                                 _p(_f64(nu)), _p(_f64(ym)), _p(km), _p(dp),
+                                 _p(ext), _p(eq))
+    return km, dp, ext
+
+
+def ebe_matvec(neq, storekm, gdof, p):
+    kp = np.zeros(neq + 1)
+    storekm = _f64(storekm); gdof = _i32(gdof); p = _f64(p)
+    lib().orc_ebe_matvec(neq, gdof.shape[0], _p(storekm), _p(gdof, C.c_int32),
+                         _p(p), _p(kp))
+    return kp
+
+
+def pcg_solver(neq, storekm, u, f, dinv, gdof, cg_tol, cg_maxiter):
+    """Returns (u_out, cg_iter, errcode)."""
+    u = _f64(u).copy(); storekm = _f64(storekm); gdof = _i32(gdof)
+    it = C.c_int32()
+    ec = lib().orc_pcg_solver(neq, gdof.shape[0], _p(storekm), _p(u), _p(_f64(f)),
+                              _p(_f64(dinv)), _p(gdof, C.c_int32), cg_tol,
+                              cg_maxiter, C.byref(it))
+    return u, it.value, ec
+
+
+def plastic_body(spec, ym, nuf, phif, cohf, psif, dt, x, evpt, stress,
+                 bodyload, conv_or_last, want_flags=False):
+    """In-place on evpt (E,27,6), stress (E,27,6), bodyload (neq+1,);
+    returns (fmax, flags or None)."""
+    flags = np.zeros((spec.nelmt, 27), dtype=np.uint8) if want_flags else None
+    fmax = lib().orc_plastic_body(
+        spec.neq, spec.nelmt, _p(_f64(spec.corners)),
+        _p(_i32(spec.gdof_elmt), C.c_int32), _p(_i32(spec.mat_id), C.c_int32),
+        _p(_f64(ym)), _p(_f64(nuf)), _p(_f64(phif)), _p(_f64(cohf)),
+        _p(_f64(psif)), dt, _p(_f64(x)), _p(evpt), _p(stress), _p(bodyload),
+        int(conv_or_last), _p(flags, C.c_uint8))
+    return fmax, flags
+
+
+def slope3d(spec, mesh, *, cg_tol, cg_maxiter, nl_tol, nl_maxiter, srf,
+            phinu=False, cg_log_cap=100000):
+    """Full SRF sweep; returns dict with per-SRF summary rows, the final x
+    and the cg_iter of every PCG call."""
+    gam, ym, nu, phi, coh, psi = mesh.material_arrays()
+    srf = _f64(srf)
+    n = len(srf)
+    scg = np.zeros(n, np.int32); snl = np.zeros(n, np.int32)
+    ux = np.zeros(n); um = np.zeros(n); fm = np.zeros(n)
+    x = np.zeros(spec.neq + 1)
+    log = np.zeros(cg_log_cap, np.int32); nlog = C.c_int32()
+    gnode = _i32(spec.gdof)
+    rc = lib().orc_slope3d(spec.nelmt, spec.nnode, spec.neq,
+                           _p(_f64(spec.corners)),
+                           _p(_i32(spec.gdof_elmt), C.c_int32),
+                           _p(gnode, C.c_int32), _p(_i32(spec.mat_id), C.c_int32),
+                           len(gam), _p(gam), _p(ym), _p(nu), _p(phi), _p(coh),
+                           _p(psi), int(phinu), cg_tol, cg_maxiter, nl_tol,
+                           nl_maxiter, n, _p(srf), _p(scg, C.c_int32),
+                           _p(snl, C.c_int32), _p(ux), _p(um), _p(fm), _p(x),
+                           _p(log, C.c_int32), cg_log_cap, C.byref(nlog))
+    if rc < 0:
+        raise RuntimeError("ERROR: PCG solver doesn't converge!")
+    rows = [dict(srf=float(srf[s]), cg_tot=int(scg[s]), nl_tot=int(snl[s]),
+                 uxmax=float(ux[s]), umax=float(um[s]), fmax=float(fm[s]))
+            for s in range(rc)]
+    return dict(rows=rows, x=x, cg_log=log[:min(nlog.value, cg_log_cap)].copy())

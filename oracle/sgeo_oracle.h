@@ -1,0 +1,127 @@
+/*
+ * sgeo_oracle -- CPU restatement of SPECFEM3D_GEOTECH's EBE-PCG solve and
+ * viscoplastic Mohr-Coulomb plastic-iteration body (NGLL = 3).
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker; the product path (specfem3d_geotech_amd) never links or calls it.
+ *
+ * Array conventions follow the reference (Fortran column-major, flattened):
+ *   storekm  k[r + 81*c + 6561*e]         k(r,c,e)      src/slope.F90:152
+ *   gdof     g[i + 81*e]                  gdof_elmt(i,e) src/slope.F90:139-143
+ *   vectors  v[0..neq], index 0 = sink for constrained dofs
+ *   corners  xc[d + 3*(g + 8*e)]          g_coord(d, g_num(hex8_gnode(g), e))
+ *   state    evpt[s + 6*(i + 27*e)]       evpt(s,i,e)   src/slope.F90:103
+ *   mat_id   1-based block ids, material arrays indexed [id-1]
+ */
+#ifndef SGEO_ORACLE_H
+#define SGEO_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* GLL tables for NGLL=3 (src/gll_library.f90:14-82, :257-299, :717-783;
+ * src/shape_library.f90:104-216).
+ *   w[27]; dlag[d + 3*(i + 27*n)] = dlagrange_gll(d,i,n);
+ *   dsh[d + 3*(g + 8*i)] = dshape_hex8(d,g,i). */
+void orc_gll_tables(double *w, double *dlag, double *dsh);
+
+/* src/elastic.f90:13-42 (c is 6x6 column-major) */
+void orc_compute_cmat(double E, double nu, double *c);
+
+/* src/math_library.f90:66-138 */
+double orc_determinant3(const double *j);
+void orc_invert3(double *j);
+
+/* src/math_library.f90:144-186 */
+void orc_stress_invariant(const double *s, double *sigm, double *dsbar,
+                          double *theta);
+/* src/plastic_library.f90:95-108 */
+double orc_mohcouf(double phi, double c, double sigm, double dsbar,
+                   double theta);
+/* src/plastic_library.f90:116-143 */
+void orc_mohcouq(double psi, double dsbar, double theta, double *dq1,
+                 double *dq2, double *dq3);
+/* src/plastic_library.f90:150-208 (6x6 column-major each) */
+void orc_formm(const double *s, double *m1, double *m2, double *m3);
+/* src/plastic_library.f90:17-39 */
+double orc_dt_viscoplas(int nmat, const double *nuf, const double *phif,
+                        const double *ymf);
+/* src/plastic_library.f90:42-88; returns istat (1 = nu changed) */
+int orc_strength_reduction(double srf, int phinu, int nmat, const double *coh,
+                           const double *nu, const double *phi,
+                           const double *psi, double *cohf, double *nuf,
+                           double *phif, double *psif);
+
+/* src/preprocess.f90:74-148.  dprecon is the *unassembled, un-inverted*
+ * diagonal as the reference leaves it; extload may be NULL (no body load).
+ * eqk (3 values) may be NULL (no pseudo-static load). */
+void orc_stiffness_bodyload(int32_t nelmt, int32_t neq, const double *xc,
+                            const int32_t *gdof, const int32_t *mat_id,
+                            const double *gam, const double *nu,
+                            const double *ym, double *storekm, double *dprecon,
+                            double *extload, const double *eqk);
+
+/* EBE product kp = sum_e K_e p_e in element order, kp[0] = 0
+ * (src/solver.f90:53-59). */
+void orc_ebe_matvec(int32_t neq, int32_t nelmt, const double *k,
+                    const int32_t *gdof, const double *p, double *kp);
+
+/* src/solver.f90:14-79.  dinv is the inverted diagonal (dprecon after
+ * src/slope.F90:199).  Returns 0 (converged) or -1.  *cg_iter as in the
+ * reference (the maxiter loop leaves it at maxiter+1 on failure). */
+int orc_pcg_solver(int32_t neq, int32_t nelmt, const double *k, double *u,
+                   const double *f, const double *dinv, const int32_t *gdof,
+                   double cg_tol, int32_t cg_maxiter, int32_t *cg_iter);
+
+/* One element x GLL pass of the plastic iteration (src/slope.F90:312-384).
+ * conv_or_last = nl_isconv .or. nl_iter==nl_maxiter.
+ * Updates evpt, stress_elmt, bodyload (bodyload(0)=0 at the end); returns
+ * fmax = max(0, max f).  flags (may be NULL) receives f>=0 per (i,e). */
+double orc_plastic_body(int32_t neq, int32_t nelmt, const double *xc,
+                        const int32_t *gdof, const int32_t *mat_id,
+                        const double *ym, const double *nuf,
+                        const double *phif, const double *cohf,
+                        const double *psif, double dt, const double *x,
+                        double *evpt, double *stress_elmt, double *bodyload,
+                        int conv_or_last, uint8_t *flags);
+
+/* Excavation variant of the body (src/excavation.F90:546-616): elements
+ * elmt[0..nel_act-1] (0-based ids into the full arrays), effective stress
+ * adds water (none here) then stress_elmt, and on conv_or_last also
+ * accumulates vmeps per node (g_num 0-based node ids, 27 per element). */
+double orc_plastic_body_excav(int32_t neq, int32_t nel_act,
+                              const int32_t *elmt, const double *xc,
+                              const int32_t *gdof_act, const int32_t *mat_id,
+                              const double *ym, const double *nuf,
+                              const double *phif, const double *cohf,
+                              const double *psif, double dt, const double *x,
+                              double *evpt, double *stress_elmt,
+                              double *bodyload, int conv_or_last,
+                              const int32_t *g_num, double *vmeps);
+
+/* Full strength-reduction sweep of slope3d (src/slope.F90:244-443) on a
+ * prepared mesh.  Per SRF s writes sum_cg[s], sum_nl[s], uxmax[s], umax[s],
+ * fmax[s]; returns the number of SRFs run (the loop stops after the first
+ * SRF with nl_iter == nl_maxiter).  x_out (may be NULL) receives the final
+ * displacement of the last SRF run; cg_log (may be NULL, capacity
+ * cg_log_cap) receives cg_iter of every PCG call, *n_cg_log the count.
+ * gdof_node[c + 3*n] is gdof(c+1, n+1) (0 = constrained).
+ * Returns a negative value if a PCG call fails (errcode -1). */
+int orc_slope3d(int32_t nelmt, int32_t nnode, int32_t neq, const double *xc,
+                const int32_t *gdof, const int32_t *gdof_node,
+                const int32_t *mat_id, int nmat, const double *gam,
+                const double *ym, const double *nu, const double *phi,
+                const double *coh, const double *psi, int phinu,
+                double cg_tol, int32_t cg_maxiter, double nl_tol,
+                int32_t nl_maxiter, int nsrf, const double *srf,
+                int32_t *sum_cg, int32_t *sum_nl, double *uxmax, double *umax,
+                double *fmax, double *x_out, int32_t *cg_log,
+                int32_t cg_log_cap, int32_t *n_cg_log);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
