@@ -1,0 +1,151 @@
+/*
+ * sgeo.h -- C ABI of the MI355X (gfx950) EBE-PCG + viscoplastic hot path of
+ * SPECFEM3D_GEOTECH.
+ *
+ * Conventions (same as the reference, src/slope.F90:139-199):
+ *   * equations are numbered 1..neq; every global vector has neq+1 entries
+ *     and index 0 is the sink of constrained dofs (always zero on output);
+ *   * gdof_elmt[e*81 + i] is the reference's gdof_elmt(i+1, e+1): element dof
+ *     order is node-major (27 GLL nodes in local k,j,i order, i fastest),
+ *     x/y/z fastest; 0 = constrained;
+ *   * storekm[e*6561 + r + 81*c] is the reference's k(r+1, c+1, e+1)
+ *     (column-major 81x81 per element);
+ *   * corners[e*24 + 3*g + d] is coordinate d of hex8 corner g (EXODUS order,
+ *     src/element.f90:32-40) of element e;
+ *   * mat_id[e] is 1-based; per-material arrays are indexed mat_id-1;
+ *     angles in degrees.
+ * Every function returns 0 on success and a negative SGEO_E* code on a
+ * device or argument error; sgeo_last_error() then describes it.  PCG
+ * non-convergence is not an error of the call: it is reported through
+ * errcode = -1 exactly like the reference's pcg_solver.
+ * A context is bound to one HIP device and one stream; calls on one context
+ * are not thread-safe, separate contexts may be used from separate threads.
+ */
+#ifndef SGEO_H
+#define SGEO_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGEO_OK 0
+#define SGEO_EARG (-1)     /* invalid argument / size */
+#define SGEO_EHIP (-2)     /* HIP runtime error */
+#define SGEO_ENOMEM (-3)   /* device allocation failed */
+#define SGEO_ESTATE (-4)   /* call out of order */
+
+/* precision of the stored element stiffness */
+#define SGEO_K_F64 0
+#define SGEO_K_F32 1
+
+typedef struct sgeo_ctx sgeo_ctx;
+typedef struct sgeo_mesh sgeo_mesh;
+typedef struct sgeo_slope sgeo_slope;
+typedef struct sgeo_batch sgeo_batch;
+
+/* ---- context ---------------------------------------------------------- */
+int sgeo_init(int device, sgeo_ctx **ctx);
+int sgeo_finalize(sgeo_ctx *ctx);
+const char *sgeo_last_error(void);
+/* Device name, CU count and HBM bytes of the context's device. */
+int sgeo_device_info(sgeo_ctx *ctx, char *name, size_t name_len, int *n_cu,
+                     size_t *hbm_bytes);
+int sgeo_synchronize(sgeo_ctx *ctx);
+
+/* Summation order of the PCG dot products (no counterpart in the reference,
+ * whose dot_product is sequential, src/solver.f90:61-62,72).
+ *   SGEO_REDUCE_TREE        blocked tree reductions (default, fast,
+ *                           deterministic run to run)
+ *   SGEO_REDUCE_SEQUENTIAL  index order, one rounding per term: with it the
+ *                           whole PCG reproduces the reference bit for bit;
+ *                           its dot products are latency-bound (O(neq) on one
+ *                           lane), so it is meant for parity runs. */
+#define SGEO_REDUCE_TREE 0
+#define SGEO_REDUCE_SEQUENTIAL 1
+int sgeo_set_reduction(sgeo_ctx *ctx, int mode);
+
+/* ---- Seam 1: drop-in for pcg_solver ------------------------------------
+ * Replaces subroutine pcg_solver(neq,nelmt,k,u,f,dprecon,gdof_elmt,cg_iter,
+ * errcode,errtag) (src/solver.f90:14-79; called from src/slope.F90:293,
+ * src/excavation.F90:231,525).  Host arrays in the reference layout:
+ * k (81*81*nelmt), u (neq+1, in/out, warm start), f and dprecon (neq+1,
+ * dprecon already inverted with dprecon[0]=0 as src/slope.F90:199 leaves
+ * it), gdof_elmt (81*nelmt).  cg_tol / cg_maxiter are the reference's
+ * module globals (src/global.f90:96-97).  On return cg_iter, errcode
+ * (0 converged, -1 not converged) and errtag (250 chars, blank padded,
+ * "ERROR: PCG solver doesn't converge!" on -1) follow the reference.
+ * Device copies of k, dprecon and gdof_elmt are cached on the context and
+ * reused while the host pointers, sizes and a strided content fingerprint
+ * are unchanged. */
+int sgeo_pcg_solver(sgeo_ctx *ctx, int32_t neq, int32_t nelmt,
+                    const double *k, double *u, const double *f,
+                    const double *dprecon, const int32_t *gdof_elmt,
+                    double cg_tol, int32_t cg_maxiter, int32_t *cg_iter,
+                    int32_t *errcode, char *errtag);
+
+/* ---- resident mesh ------------------------------------------------------
+ * A mesh keeps gdof_elmt, corner coordinates, material ids and the element
+ * stiffness resident in HBM.  nelmt elements, neq equations. */
+int sgeo_mesh_create(sgeo_ctx *ctx, int32_t nelmt, int32_t neq,
+                     const double *corners, const int32_t *gdof_elmt,
+                     const int32_t *mat_id, int32_t nmat, sgeo_mesh **mesh);
+int sgeo_mesh_destroy(sgeo_mesh *mesh);
+/* Builds storekm, the assembled diagonal and (if extload != NULL) the
+ * gravity load on the device: src/preprocess.f90:74-148 with
+ * E = ym[m], nu = nu[m], gamma = gam[m].  kprec selects FP64 or FP32
+ * storage (FP32 is rounded from the FP64 element matrices).  extload
+ * (host, neq+1, may be NULL) receives the gravity load; dprecon_out (host,
+ * neq+1, may be NULL) receives the *inverted* diagonal (dprecon[0]=0). */
+int sgeo_mesh_build_stiffness(sgeo_mesh *mesh, const double *gam,
+                              const double *ym, const double *nu, int kprec,
+                              double *extload, double *dprecon_out);
+/* Uploads a host storekm (reference layout, FP64) and inverted dprecon. */
+int sgeo_mesh_upload_stiffness(sgeo_mesh *mesh, const double *storekm,
+                               const double *dprecon, int kprec);
+/* Copies the device storekm back (FP64 meshes only). */
+int sgeo_mesh_download_stiffness(sgeo_mesh *mesh, double *storekm);
+/* kp = K p on the resident operator (host vectors, neq+1). */
+int sgeo_mesh_matvec(sgeo_mesh *mesh, const double *p, double *kp);
+/* HBM bytes held by the mesh. */
+int sgeo_mesh_bytes(sgeo_mesh *mesh, size_t *bytes);
+
+/* ---- Seam 2: slope plastic iteration on a resident mesh -----------------
+ * State of one strength-reduction trial (src/slope.F90:244-443): x, oldx,
+ * bodyload, evpt(6,27,E), stress_elmt(6,27,E) live on the device. */
+int sgeo_slope_create(sgeo_mesh *mesh, const double *extload,
+                      sgeo_slope **slope);
+int sgeo_slope_destroy(sgeo_slope *slope);
+/* Resets the trial (src/slope.F90:248-249, 281-282) and sets the factored
+ * material properties of this SRF (from strength_reduction, per material)
+ * and the pseudo-time step dt (dt_viscoplas). */
+int sgeo_slope_begin_srf(sgeo_slope *slope, const double *ym,
+                         const double *nuf, const double *phif,
+                         const double *cohf, const double *psif, double dt);
+/* One plastic iteration (src/slope.F90:286-391): load = extload+bodyload,
+ * PCG (warm start), x(0)=0, uerr, nl_isconv, then the element x GLL body.
+ * is_last = (nl_iter == nl_maxiter).  Outputs: cg_iter, errcode of the
+ * PCG (as sgeo_pcg_solver), uerr, nl_isconv, fmax. */
+int sgeo_slope_plastic_iter(sgeo_slope *slope, double cg_tol,
+                            int32_t cg_maxiter, double nl_tol, int is_last,
+                            int32_t *cg_iter, int32_t *errcode, double *uerr,
+                            int32_t *nl_isconv, double *fmax);
+/* Full plastic loop of one SRF (the loop above until nl_isconv or
+ * nl_maxiter).  Outputs the summary row of src/slope.F90:415 (cg_tot,
+ * nl_tot, uxmax, umax, fmax); errcode as sgeo_pcg_solver. */
+int sgeo_slope_run_srf(sgeo_slope *slope, const int32_t *gdof_node,
+                       int32_t nnode, double cg_tol, int32_t cg_maxiter,
+                       double nl_tol, int32_t nl_maxiter, int32_t *cg_tot,
+                       int32_t *nl_tot, double *uxmax, double *umax,
+                       double *fmax, int32_t *errcode);
+/* Host copies of the trial state: x (neq+1), bodyload (neq+1),
+ * evpt / stress_elmt (6*27*nelmt, reference layout), plastic flags
+ * (27*nelmt bytes: f>=0 at the last body pass).  Any pointer may be NULL. */
+int sgeo_slope_download(sgeo_slope *slope, double *x, double *bodyload,
+                        double *evpt, double *stress_elmt, uint8_t *flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SGEO_H */

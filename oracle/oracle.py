@@ -24,6 +24,7 @@ def lib():
         _LIB = C.CDLL(path)
         d, i32, u8 = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_uint8)
         _LIB.orc_gll_tables.argtypes = [d, d, d]
+        _LIB.orc_set_dot_mode.argtypes = [C.c_int]
         _LIB.orc_compute_cmat.argtypes = [C.c_double, C.c_double, d]
         _LIB.orc_stress_invariant.argtypes = [d, d, d, d]
         _LIB.orc_mohcouf.argtypes = [C.c_double] * 5
@@ -67,6 +68,11 @@ def _f64(a):
 
 def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def set_dot_mode(mode: int):
+    """0 reference order; 1 descending; 2 blocked pairwise (sensitivity)."""
+    lib().orc_set_dot_mode(int(mode))
 
 
 def gll_tables():

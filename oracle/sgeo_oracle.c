@@ -328,8 +328,32 @@ void orc_ebe_matvec(int32_t neq, int32_t nelmt, const double *k,
   free(y);
 }
 
+/* Summation order of the PCG dot products.  0 (default) is the reference's
+ * sequential dot_product (src/solver.f90:61-62,72); 1 sums in descending
+ * index order and 2 in blocks of 1024 combined pairwise -- both equally
+ * valid orders, used only to measure how sensitive the reference's results
+ * are to the order of its own sums. */
+static int g_dot_mode = 0;
+void orc_set_dot_mode(int mode) { g_dot_mode = mode; }
+
+static double dot_tree(const double *a, const double *b, int64_t lo,
+                       int64_t hi) {
+  if (hi - lo <= 1024) {
+    double s = 0.0;
+    for (int64_t i = lo; i < hi; i++) s = s + a[i] * b[i];
+    return s;
+  }
+  int64_t mid = lo + (((hi - lo) / 2 + 1023) / 1024) * 1024;
+  return dot_tree(a, b, lo, mid) + dot_tree(a, b, mid, hi);
+}
+
 static double dot(const double *a, const double *b, int64_t n1) {
   double s = 0.0;
+  if (g_dot_mode == 1) {
+    for (int64_t i = n1 - 1; i >= 0; i--) s = s + a[i] * b[i];
+    return s;
+  }
+  if (g_dot_mode == 2) return dot_tree(a, b, 0, n1);
   for (int64_t i = 0; i < n1; i++) s = s + a[i] * b[i];
   return s;
 }

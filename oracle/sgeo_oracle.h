@@ -64,6 +64,10 @@ void orc_stiffness_bodyload(int32_t nelmt, int32_t neq, const double *xc,
                             const double *ym, double *storekm, double *dprecon,
                             double *extload, const double *eqk);
 
+/* Order of the PCG dot products: 0 reference (sequential ascending),
+ * 1 descending, 2 blocked pairwise.  For sensitivity measurements only. */
+void orc_set_dot_mode(int mode);
+
 /* EBE product kp = sum_e K_e p_e in element order, kp[0] = 0
  * (src/solver.f90:53-59). */
 void orc_ebe_matvec(int32_t neq, int32_t nelmt, const double *k,

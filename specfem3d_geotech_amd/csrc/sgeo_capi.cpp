@@ -1,0 +1,701 @@
+// sgeo_capi.cpp -- host orchestration and C ABI (include/sgeo.h) of the
+// MI355X EBE-PCG + viscoplastic hot path.  One context = one HIP device +
+// one stream; all device work of a context is ordered on that stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sgeo.h"
+#include "sgeo_internal.h"
+
+using namespace sgeo;
+
+// --------------------------------------------------------------------------
+// errors
+// --------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(call)                                                        \
+  do {                                                                      \
+    hipError_t _e = (call);                                                 \
+    if (_e != hipSuccess)                                                   \
+      return fail(SGEO_EHIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,     \
+                  hipGetErrorString(_e));                                   \
+  } while (0)
+
+extern "C" const char* sgeo_last_error(void) { return g_err.c_str(); }
+
+// --------------------------------------------------------------------------
+// objects
+// --------------------------------------------------------------------------
+struct sgeo_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  PcgScalars* d_sc = nullptr;
+  PcgScalars* h_sc = nullptr;  // pinned
+  PlasticScalars* d_ps = nullptr;
+  PlasticScalars* h_ps = nullptr;  // pinned
+  double* d_part = nullptr;
+  size_t part_cap = 0;  // doubles
+  // Seam-1 cache
+  sgeo_mesh* cache = nullptr;
+  const void* cache_k = nullptr;
+  const void* cache_g = nullptr;
+  const void* cache_d = nullptr;
+  uint64_t cache_sig = 0;
+  double* d_u = nullptr;
+  double* d_f = nullptr;
+  size_t uf_cap = 0;
+  int reduction = SGEO_REDUCE_TREE;
+};
+
+struct sgeo_mesh {
+  sgeo_ctx* ctx = nullptr;
+  int32_t nelmt = 0, neq = 0, nmat = 0;
+  int32_t* d_gdof = nullptr;
+  int32_t* d_rowptr = nullptr;
+  int32_t* d_slots = nullptr;
+  double* d_corners = nullptr;
+  int32_t* d_mat = nullptr;
+  int kprec = -1;  // -1: no stiffness yet
+  double* d_K64 = nullptr;
+  float* d_K32 = nullptr;
+  double* d_dinv = nullptr;
+  // work vectors
+  double* d_Y = nullptr;
+  double *d_r = nullptr, *d_z = nullptr, *d_p = nullptr, *d_kp = nullptr;
+  size_t bytes = 0;
+};
+
+struct sgeo_slope {
+  sgeo_mesh* mesh = nullptr;
+  double *d_x = nullptr, *d_oldx = nullptr, *d_load = nullptr;
+  double *d_body = nullptr, *d_ext = nullptr;
+  double *d_evpt = nullptr, *d_stress = nullptr, *d_BL = nullptr;
+  uint8_t* d_flags = nullptr;
+  MatParams* d_mp = nullptr;
+  std::vector<MatParams> h_mp;
+  bool begun = false;
+};
+
+template <typename T>
+static int dalloc(T** p, size_t n, size_t* acc = nullptr) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess)
+    return fail(SGEO_ENOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T),
+                hipGetErrorString(e));
+  if (acc) *acc += n * sizeof(T);
+  return 0;
+}
+
+#define TRY(x)               \
+  do {                       \
+    int _r = (x);            \
+    if (_r != 0) return _r;  \
+  } while (0)
+
+static int ensure_part(sgeo_ctx* c, size_t n) {
+  if (n <= c->part_cap) return 0;
+  if (c->d_part) hipFree(c->d_part);
+  c->d_part = nullptr;
+  c->part_cap = 0;
+  TRY(dalloc(&c->d_part, n));
+  c->part_cap = n;
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// context
+// --------------------------------------------------------------------------
+extern "C" int sgeo_init(int device, sgeo_ctx** out) {
+  if (!out) return fail(SGEO_EARG, "sgeo_init: null output");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev)
+    return fail(SGEO_EARG, "sgeo_init: device %d of %d", device, ndev);
+  HIPCHK(hipSetDevice(device));
+  sgeo_ctx* c = new sgeo_ctx();
+  c->device = device;
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  TRY(dalloc(&c->d_sc, 1));
+  TRY(dalloc(&c->d_ps, 1));
+  HIPCHK(hipHostMalloc((void**)&c->h_sc, sizeof(PcgScalars)));
+  HIPCHK(hipHostMalloc((void**)&c->h_ps, sizeof(PlasticScalars)));
+  HIPCHK(hipMemsetAsync(c->d_sc, 0, sizeof(PcgScalars), c->stream));
+  HIPCHK(hipMemsetAsync(c->d_ps, 0, sizeof(PlasticScalars), c->stream));
+  // GLL tables: l_m'(xi_q) at {-1,0,1} (src/gll_library.f90:257-299) and
+  // weights wx*wy*wz with wx = {1/3, 4/3, 1/3} (zwgljd, np=3)
+  const double d1[9] = {-1.5, 2.0, -0.5, -0.5, 0.0, 0.5, 0.5, -2.0, 1.5};
+  const double wx[3] = {1.0 / 3.0, 4.0 / 3.0, 1.0 / 3.0};
+  double w[27];
+  for (int k = 0, n = 0; k < 3; k++)
+    for (int j = 0; j < 3; j++)
+      for (int i = 0; i < 3; i++) w[n++] = wx[i] * wx[j] * wx[k];
+  upload_tables(c->stream, d1, w);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *out = c;
+  return 0;
+}
+
+extern "C" int sgeo_finalize(sgeo_ctx* c) {
+  if (!c) return 0;
+  hipSetDevice(c->device);
+  if (c->cache) sgeo_mesh_destroy(c->cache);
+  if (c->d_u) hipFree(c->d_u);
+  if (c->d_f) hipFree(c->d_f);
+  if (c->d_part) hipFree(c->d_part);
+  if (c->d_sc) hipFree(c->d_sc);
+  if (c->d_ps) hipFree(c->d_ps);
+  if (c->h_sc) hipHostFree(c->h_sc);
+  if (c->h_ps) hipHostFree(c->h_ps);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+extern "C" int sgeo_device_info(sgeo_ctx* c, char* name, size_t name_len,
+                                int* n_cu, size_t* hbm) {
+  if (!c) return fail(SGEO_EARG, "null ctx");
+  hipDeviceProp_t p;
+  HIPCHK(hipGetDeviceProperties(&p, c->device));
+  if (name && name_len) {
+    snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+  }
+  if (n_cu) *n_cu = p.multiProcessorCount;
+  if (hbm) *hbm = p.totalGlobalMem;
+  return 0;
+}
+
+extern "C" int sgeo_set_reduction(sgeo_ctx* c, int mode) {
+  if (!c || (mode != SGEO_REDUCE_TREE && mode != SGEO_REDUCE_SEQUENTIAL))
+    return fail(SGEO_EARG, "sgeo_set_reduction: mode %d", mode);
+  c->reduction = mode;
+  return 0;
+}
+
+extern "C" int sgeo_synchronize(sgeo_ctx* c) {
+  if (!c) return fail(SGEO_EARG, "null ctx");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// mesh
+// --------------------------------------------------------------------------
+extern "C" int sgeo_mesh_create(sgeo_ctx* c, int32_t nelmt, int32_t neq,
+                                const double* corners, const int32_t* gdof,
+                                const int32_t* mat_id, int32_t nmat,
+                                sgeo_mesh** out) {
+  if (!c || !out || !gdof || nelmt <= 0 || neq <= 0)
+    return fail(SGEO_EARG, "sgeo_mesh_create: bad arguments");
+  if ((int64_t)nelmt * 81 > INT32_MAX)
+    return fail(SGEO_EARG, "sgeo_mesh_create: nelmt*81 exceeds int32");
+  const int64_t ns = (int64_t)nelmt * 81;
+  // validate equations and build the inverse map (counting sort, stable in
+  // element order)
+  std::vector<int32_t> rowptr((size_t)neq + 2, 0);
+  for (int64_t q = 0; q < ns; q++) {
+    const int32_t g = gdof[q];
+    if (g < 0 || g > neq)
+      return fail(SGEO_EARG, "gdof_elmt[%lld]=%d outside 0..%d", (long long)q, g,
+                  neq);
+    rowptr[(size_t)g + 1]++;
+  }
+  for (size_t g = 1; g < rowptr.size(); g++) rowptr[g] += rowptr[g - 1];
+  std::vector<int32_t> slots((size_t)ns), fill(rowptr.begin(), rowptr.end() - 1);
+  for (int64_t q = 0; q < ns; q++) slots[(size_t)fill[(size_t)gdof[q]]++] = (int32_t)q;
+  if (mat_id) {
+    for (int32_t e = 0; e < nelmt; e++)
+      if (mat_id[e] < 1 || mat_id[e] > nmat)
+        return fail(SGEO_EARG, "mat_id[%d]=%d outside 1..%d", e, mat_id[e], nmat);
+  }
+  HIPCHK(hipSetDevice(c->device));
+  sgeo_mesh* m = new sgeo_mesh();
+  m->ctx = c;
+  m->nelmt = nelmt;
+  m->neq = neq;
+  m->nmat = nmat;
+  const int64_t n1 = (int64_t)neq + 1;
+  TRY(dalloc(&m->d_gdof, (size_t)ns, &m->bytes));
+  TRY(dalloc(&m->d_rowptr, (size_t)neq + 2, &m->bytes));
+  TRY(dalloc(&m->d_slots, (size_t)ns, &m->bytes));
+  TRY(dalloc(&m->d_Y, (size_t)ns, &m->bytes));
+  TRY(dalloc(&m->d_r, (size_t)n1, &m->bytes));
+  TRY(dalloc(&m->d_z, (size_t)n1, &m->bytes));
+  TRY(dalloc(&m->d_p, (size_t)n1, &m->bytes));
+  TRY(dalloc(&m->d_kp, (size_t)n1, &m->bytes));
+  TRY(dalloc(&m->d_dinv, (size_t)n1, &m->bytes));
+  hipStream_t s = c->stream;
+  HIPCHK(hipMemcpyAsync(m->d_gdof, gdof, ns * 4, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(m->d_rowptr, rowptr.data(), rowptr.size() * 4,
+                        hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(m->d_slots, slots.data(), ns * 4, hipMemcpyHostToDevice, s));
+  if (corners) {
+    TRY(dalloc(&m->d_corners, (size_t)nelmt * 24, &m->bytes));
+    HIPCHK(hipMemcpyAsync(m->d_corners, corners, (size_t)nelmt * 24 * 8,
+                          hipMemcpyHostToDevice, s));
+  }
+  if (mat_id) {
+    TRY(dalloc(&m->d_mat, (size_t)nelmt, &m->bytes));
+    HIPCHK(hipMemcpyAsync(m->d_mat, mat_id, (size_t)nelmt * 4,
+                          hipMemcpyHostToDevice, s));
+  }
+  TRY(ensure_part(c, 4 * (size_t)vec_reduce_blocks(n1) + 64));
+  HIPCHK(hipStreamSynchronize(s));  // host vectors die at return
+  *out = m;
+  return 0;
+}
+
+extern "C" int sgeo_mesh_destroy(sgeo_mesh* m) {
+  if (!m) return 0;
+  hipSetDevice(m->ctx->device);
+  hipStreamSynchronize(m->ctx->stream);
+  void* ptrs[] = {m->d_gdof, m->d_rowptr, m->d_slots, m->d_corners, m->d_mat,
+                  m->d_K64,  m->d_K32,    m->d_dinv,  m->d_Y,       m->d_r,
+                  m->d_z,    m->d_p,      m->d_kp};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  delete m;
+  return 0;
+}
+
+extern "C" int sgeo_mesh_bytes(sgeo_mesh* m, size_t* bytes) {
+  if (!m || !bytes) return fail(SGEO_EARG, "null argument");
+  *bytes = m->bytes;
+  return 0;
+}
+
+static int set_k_storage(sgeo_mesh* m, int kprec) {
+  if (kprec != SGEO_K_F64 && kprec != SGEO_K_F32)
+    return fail(SGEO_EARG, "kprec %d", kprec);
+  const size_t nk = (size_t)m->nelmt * 6561;
+  if (!m->d_K64) TRY(dalloc(&m->d_K64, nk, &m->bytes));
+  if (kprec == SGEO_K_F32 && !m->d_K32) TRY(dalloc(&m->d_K32, nk, &m->bytes));
+  m->kprec = kprec;
+  return 0;
+}
+
+extern "C" int sgeo_mesh_build_stiffness(sgeo_mesh* m, const double* gam,
+                                         const double* ym, const double* nu,
+                                         int kprec, double* extload,
+                                         double* dprecon_out) {
+  if (!m || !ym || !nu || !m->d_corners || !m->d_mat)
+    return fail(SGEO_EARG, "sgeo_mesh_build_stiffness: mesh lacks corners/mat_id");
+  if (extload && !gam) return fail(SGEO_EARG, "extload requested without gam");
+  sgeo_ctx* c = m->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  TRY(set_k_storage(m, kprec));
+  const int64_t n1 = (int64_t)m->neq + 1;
+  double *d_ym = nullptr, *d_nu = nullptr, *d_gam = nullptr, *d_kdiag = nullptr,
+         *d_eld = nullptr, *d_ext = nullptr;
+  TRY(dalloc(&d_ym, m->nmat));
+  TRY(dalloc(&d_nu, m->nmat));
+  TRY(dalloc(&d_gam, m->nmat));
+  TRY(dalloc(&d_kdiag, (size_t)m->nelmt * 81));
+  TRY(dalloc(&d_eld, (size_t)m->nelmt * 81));
+  TRY(dalloc(&d_ext, (size_t)n1));
+  HIPCHK(hipMemcpyAsync(d_ym, ym, m->nmat * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d_nu, nu, m->nmat * 8, hipMemcpyHostToDevice, s));
+  if (gam) HIPCHK(hipMemcpyAsync(d_gam, gam, m->nmat * 8, hipMemcpyHostToDevice, s));
+  launch_stiffness(s, m->nelmt, m->d_corners, m->d_mat, d_ym, d_nu, m->d_K64,
+                   d_kdiag, d_eld);
+  // dprecon(egdof(k)) += km(k,k) in element order, then 1/d
+  launch_assemble(s, n1, m->d_rowptr, m->d_slots, d_kdiag, m->d_dinv, c->d_sc, 0);
+  launch_invert_diag(s, n1, m->d_dinv);
+  if (extload) {
+    // extload(egdof) = extload(egdof) - eld*gam, element order
+    launch_scale_by_mat(s, m->nelmt, d_eld, m->d_mat, d_gam, d_kdiag);
+    HIPCHK(hipMemsetAsync(d_ext, 0, n1 * 8, s));
+    launch_assemble_acc(s, n1, m->d_rowptr, m->d_slots, d_kdiag, d_ext, 1);
+    HIPCHK(hipMemcpyAsync(extload, d_ext, n1 * 8, hipMemcpyDeviceToHost, s));
+  }
+  if (kprec == SGEO_K_F32)
+    launch_round_f32(s, (int64_t)m->nelmt * 6561, m->d_K64, m->d_K32);
+  if (dprecon_out)
+    HIPCHK(hipMemcpyAsync(dprecon_out, m->d_dinv, n1 * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  hipFree(d_ym); hipFree(d_nu); hipFree(d_gam); hipFree(d_kdiag); hipFree(d_eld);
+  hipFree(d_ext);
+  return 0;
+}
+
+extern "C" int sgeo_mesh_upload_stiffness(sgeo_mesh* m, const double* k,
+                                          const double* dprecon, int kprec) {
+  if (!m || !k || !dprecon) return fail(SGEO_EARG, "null argument");
+  sgeo_ctx* c = m->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  TRY(set_k_storage(m, kprec));
+  const size_t nk = (size_t)m->nelmt * 6561;
+  HIPCHK(hipMemcpyAsync(m->d_K64, k, nk * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(m->d_dinv, dprecon, ((size_t)m->neq + 1) * 8,
+                        hipMemcpyHostToDevice, s));
+  if (kprec == SGEO_K_F32) launch_round_f32(s, (int64_t)nk, m->d_K64, m->d_K32);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int sgeo_mesh_download_stiffness(sgeo_mesh* m, double* k) {
+  if (!m || !k || !m->d_K64) return fail(SGEO_EARG, "no FP64 stiffness");
+  HIPCHK(hipSetDevice(m->ctx->device));
+  HIPCHK(hipMemcpyAsync(k, m->d_K64, (size_t)m->nelmt * 6561 * 8,
+                        hipMemcpyDeviceToHost, m->ctx->stream));
+  HIPCHK(hipStreamSynchronize(m->ctx->stream));
+  return 0;
+}
+
+static void ebe_elem(sgeo_mesh* m, const double* p, double* Y, int skip_mode) {
+  hipStream_t s = m->ctx->stream;
+  if (m->kprec == SGEO_K_F32)
+    launch_ebe_elem_f32k(s, m->nelmt, m->d_K32, m->d_gdof, p, Y, m->ctx->d_sc,
+                         skip_mode);
+  else
+    launch_ebe_elem_f64(s, m->nelmt, m->d_K64, m->d_gdof, p, Y, m->ctx->d_sc,
+                        skip_mode);
+}
+
+extern "C" int sgeo_mesh_matvec(sgeo_mesh* m, const double* p, double* kp) {
+  if (!m || !p || !kp || m->kprec < 0) return fail(SGEO_EARG, "no stiffness");
+  sgeo_ctx* c = m->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const int64_t n1 = (int64_t)m->neq + 1;
+  HIPCHK(hipMemcpyAsync(m->d_p, p, n1 * 8, hipMemcpyHostToDevice, s));
+  ebe_elem(m, m->d_p, m->d_Y, 0);
+  launch_assemble(s, n1, m->d_rowptr, m->d_slots, m->d_Y, m->d_kp, c->d_sc, 0);
+  HIPCHK(hipMemcpyAsync(kp, m->d_kp, n1 * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// PCG on device vectors (src/solver.f90:14-79)
+// --------------------------------------------------------------------------
+// Iterations are enqueued in chunks; each kernel of an iteration returns at
+// once when the device-side stop flag is set, so over-enqueued iterations
+// cost only their launches.  The host reads the flag once per chunk.
+static int pcg_device(sgeo_mesh* m, double* d_u, const double* d_f, double tol,
+                      int32_t maxiter, int32_t* cg_iter, int32_t* errcode) {
+  sgeo_ctx* c = m->ctx;
+  hipStream_t s = c->stream;
+  const int64_t n1 = (int64_t)m->neq + 1;
+  if (maxiter < 1) {
+    // the reference's DO loop body never runs: cg_iter = 1, errcode = -1
+    *cg_iter = 1;
+    *errcode = -1;
+    return 0;
+  }
+  launch_pcg_begin(s, n1, d_u, c->d_part, c->d_sc, tol, maxiter);
+  ebe_elem(m, d_u, m->d_Y, 2);
+  launch_assemble(s, n1, m->d_rowptr, m->d_slots, m->d_Y, m->d_kp, c->d_sc, 2);
+  launch_pcg_init(s, n1, d_f, m->d_kp, m->d_dinv, m->d_r, m->d_z, m->d_p,
+                  c->d_part, c->d_sc);
+  const int seq = c->reduction == SGEO_REDUCE_SEQUENTIAL;
+  if (seq) launch_pcg_seqdot(s, n1, m->d_r, m->d_z, c->d_sc, 0);
+  int32_t enq = 0;
+  int chunk = 8;
+  for (;;) {
+    const int n = std::min<int32_t>(chunk, maxiter - enq);
+    for (int it = 0; it < n; it++) {
+      ebe_elem(m, m->d_p, m->d_Y, 1);
+      launch_pcg_assemble_dot(s, n1, m->d_rowptr, m->d_slots, m->d_Y, m->d_p,
+                              m->d_kp, c->d_part, c->d_sc, seq);
+      if (seq) launch_pcg_seqdot(s, n1, m->d_p, m->d_kp, c->d_sc, 1);
+      launch_pcg_update(s, n1, m->d_p, m->d_kp, m->d_dinv, d_u, m->d_r,
+                        m->d_z, c->d_part, c->d_sc, seq);
+      if (seq) launch_pcg_seqdot(s, n1, m->d_r, m->d_z, c->d_sc, 2);
+      launch_pcg_pupdate(s, n1, m->d_z, m->d_p, c->d_sc);
+    }
+    enq += n;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->h_sc, c->d_sc, sizeof(PcgScalars),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (c->h_sc->done) break;
+    if (enq >= maxiter)
+      return fail(SGEO_ESTATE, "PCG: %d iterations enqueued without a verdict",
+                  enq);
+    chunk = std::min(chunk * 2, 64);
+  }
+  *cg_iter = c->h_sc->conv_iter;
+  *errcode = c->h_sc->done == 1 ? 0 : -1;
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// Seam 1: sgeo_pcg_solver
+// --------------------------------------------------------------------------
+static uint64_t fp_mix(uint64_t h, const void* p, size_t nbytes, size_t nsamp) {
+  const unsigned char* b = (const unsigned char*)p;
+  const size_t step = std::max<size_t>(1, nbytes / 8 / nsamp) * 8;
+  for (size_t o = 0; o + 8 <= nbytes; o += step) {
+    uint64_t v;
+    memcpy(&v, b + o, 8);
+    h ^= v + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2);
+  }
+  if (nbytes >= 8) {
+    uint64_t v;
+    memcpy(&v, b + nbytes - 8, 8);
+    h ^= v + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2);
+  }
+  return h;
+}
+
+static void set_errtag(char* errtag, const char* msg) {
+  if (!errtag) return;
+  size_t n = strlen(msg);
+  if (n > 250) n = 250;
+  memcpy(errtag, msg, n);
+  memset(errtag + n, ' ', 250 - n);
+}
+
+extern "C" int sgeo_pcg_solver(sgeo_ctx* c, int32_t neq, int32_t nelmt,
+                               const double* k, double* u, const double* f,
+                               const double* dprecon, const int32_t* gdof,
+                               double cg_tol, int32_t cg_maxiter,
+                               int32_t* cg_iter, int32_t* errcode,
+                               char* errtag) {
+  if (!c || !k || !u || !f || !dprecon || !gdof || !cg_iter || !errcode ||
+      neq <= 0 || nelmt <= 0)
+    return fail(SGEO_EARG, "sgeo_pcg_solver: bad arguments");
+  set_errtag(errtag, "ERROR: unknown!");
+  *errcode = -1;
+  HIPCHK(hipSetDevice(c->device));
+  const size_t nk = (size_t)nelmt * 6561;
+  const int64_t n1 = (int64_t)neq + 1;
+  uint64_t sig = 1469598103934665603ULL ^ ((uint64_t)nelmt << 32) ^ (uint64_t)neq;
+  sig = fp_mix(sig, k, nk * 8, 8192);
+  sig = fp_mix(sig, dprecon, n1 * 8, 4096);
+  sig = fp_mix(sig, gdof, (size_t)nelmt * 81 * 4, 4096);
+  if (!c->cache || c->cache_k != k || c->cache_g != gdof ||
+      c->cache_d != dprecon || c->cache_sig != sig ||
+      c->cache->nelmt != nelmt || c->cache->neq != neq) {
+    if (c->cache) sgeo_mesh_destroy(c->cache);
+    c->cache = nullptr;
+    TRY(sgeo_mesh_create(c, nelmt, neq, nullptr, gdof, nullptr, 0, &c->cache));
+    TRY(sgeo_mesh_upload_stiffness(c->cache, k, dprecon, SGEO_K_F64));
+    c->cache_k = k;
+    c->cache_g = gdof;
+    c->cache_d = dprecon;
+    c->cache_sig = sig;
+  }
+  if ((size_t)n1 > c->uf_cap) {
+    if (c->d_u) hipFree(c->d_u);
+    if (c->d_f) hipFree(c->d_f);
+    c->d_u = c->d_f = nullptr;
+    c->uf_cap = 0;
+    TRY(dalloc(&c->d_u, (size_t)n1));
+    TRY(dalloc(&c->d_f, (size_t)n1));
+    c->uf_cap = (size_t)n1;
+  }
+  hipStream_t s = c->stream;
+  HIPCHK(hipMemcpyAsync(c->d_u, u, n1 * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->d_f, f, n1 * 8, hipMemcpyHostToDevice, s));
+  TRY(pcg_device(c->cache, c->d_u, c->d_f, cg_tol, cg_maxiter, cg_iter, errcode));
+  HIPCHK(hipMemcpyAsync(u, c->d_u, n1 * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (*errcode != 0) set_errtag(errtag, "ERROR: PCG solver doesn't converge!");
+  else set_errtag(errtag, "");
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// Seam 2: slope plastic iteration
+// --------------------------------------------------------------------------
+extern "C" int sgeo_slope_create(sgeo_mesh* m, const double* extload,
+                                 sgeo_slope** out) {
+  if (!m || !extload || !out || m->kprec < 0 || !m->d_corners || !m->d_mat)
+    return fail(SGEO_EARG, "sgeo_slope_create: mesh needs stiffness, corners, mat_id");
+  sgeo_ctx* c = m->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  sgeo_slope* t = new sgeo_slope();
+  t->mesh = m;
+  const size_t n1 = (size_t)m->neq + 1;
+  const size_t ns = (size_t)m->nelmt * 27 * 6;
+  TRY(dalloc(&t->d_x, n1));
+  TRY(dalloc(&t->d_oldx, n1));
+  TRY(dalloc(&t->d_load, n1));
+  TRY(dalloc(&t->d_body, n1));
+  TRY(dalloc(&t->d_ext, n1));
+  TRY(dalloc(&t->d_evpt, ns));
+  TRY(dalloc(&t->d_stress, ns));
+  TRY(dalloc(&t->d_BL, (size_t)m->nelmt * 81));
+  TRY(dalloc(&t->d_flags, (size_t)m->nelmt * 27));
+  TRY(dalloc(&t->d_mp, (size_t)std::max(1, m->nmat)));
+  HIPCHK(hipMemcpyAsync(t->d_ext, extload, n1 * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *out = t;
+  return 0;
+}
+
+extern "C" int sgeo_slope_destroy(sgeo_slope* t) {
+  if (!t) return 0;
+  hipSetDevice(t->mesh->ctx->device);
+  hipStreamSynchronize(t->mesh->ctx->stream);
+  void* ptrs[] = {t->d_x, t->d_oldx, t->d_load, t->d_body, t->d_ext,
+                  t->d_evpt, t->d_stress, t->d_BL, t->d_flags, t->d_mp};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  delete t;
+  return 0;
+}
+
+extern "C" int sgeo_slope_begin_srf(sgeo_slope* t, const double* ym,
+                                    const double* nuf, const double* phif,
+                                    const double* cohf, const double* psif,
+                                    double dt) {
+  if (!t || !ym || !nuf || !phif || !cohf || !psif)
+    return fail(SGEO_EARG, "sgeo_slope_begin_srf: null argument");
+  sgeo_mesh* m = t->mesh;
+  sgeo_ctx* c = m->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  t->h_mp.resize((size_t)m->nmat);
+  for (int i = 0; i < m->nmat; i++)
+    t->h_mp[i] = MatParams{ym[i], nuf[i], phif[i], cohf[i], psif[i], dt};
+  const size_t n1 = (size_t)m->neq + 1;
+  const size_t ns = (size_t)m->nelmt * 27 * 6;
+  HIPCHK(hipMemcpyAsync(t->d_mp, t->h_mp.data(), m->nmat * sizeof(MatParams),
+                        hipMemcpyHostToDevice, s));
+  // stress_elmt=0 (:249); bodyload=0, evpt=0, x=0, oldx=0 (:281-282)
+  HIPCHK(hipMemsetAsync(t->d_stress, 0, ns * 8, s));
+  HIPCHK(hipMemsetAsync(t->d_evpt, 0, ns * 8, s));
+  HIPCHK(hipMemsetAsync(t->d_body, 0, n1 * 8, s));
+  HIPCHK(hipMemsetAsync(t->d_x, 0, n1 * 8, s));
+  HIPCHK(hipMemsetAsync(t->d_oldx, 0, n1 * 8, s));
+  HIPCHK(hipMemsetAsync(t->d_flags, 0, (size_t)m->nelmt * 27, s));
+  t->begun = true;
+  return 0;
+}
+
+extern "C" int sgeo_slope_plastic_iter(sgeo_slope* t, double cg_tol,
+                                       int32_t cg_maxiter, double nl_tol,
+                                       int is_last, int32_t* cg_iter,
+                                       int32_t* errcode, double* uerr,
+                                       int32_t* nl_isconv, double* fmax) {
+  if (!t || !t->begun) return fail(SGEO_ESTATE, "plastic_iter before begin_srf");
+  sgeo_mesh* m = t->mesh;
+  sgeo_ctx* c = m->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const int64_t n1 = (int64_t)m->neq + 1;
+  int32_t it = 0, ec = -1;
+  launch_load(s, n1, t->d_ext, t->d_body, t->d_load);
+  TRY(pcg_device(m, t->d_x, t->d_load, cg_tol, cg_maxiter, &it, &ec));
+  if (cg_iter) *cg_iter = it;
+  if (errcode) *errcode = ec;
+  if (ec != 0) return 0;  // the caller stops (src/slope.F90:295)
+  launch_uerr(s, n1, t->d_x, t->d_oldx, c->d_part, c->d_ps, nl_tol);
+  launch_plastic_body(s, m->nelmt, nullptr, m->d_corners, m->d_gdof, m->d_mat,
+                      t->d_mp, t->d_x, t->d_evpt, t->d_stress, t->d_BL,
+                      t->d_flags, c->d_ps, is_last, 0, nullptr, nullptr);
+  // bodyload(egdof) += bload unless converged/last (src/slope.F90:380-384)
+  HIPCHK(hipMemcpyAsync(c->h_ps, c->d_ps, sizeof(PlasticScalars),
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const int conv = c->h_ps->nl_isconv;
+  if (!(conv || is_last))
+    launch_assemble_acc(s, n1, m->d_rowptr, m->d_slots, t->d_BL, t->d_body, 0);
+  HIPCHK(hipGetLastError());
+  if (uerr) *uerr = c->h_ps->uerr;
+  if (nl_isconv) *nl_isconv = conv;
+  if (fmax) {
+    unsigned long long b = c->h_ps->fmax_bits;
+    double v;
+    memcpy(&v, &b, 8);
+    *fmax = v;
+  }
+  return 0;
+}
+
+extern "C" int sgeo_slope_run_srf(sgeo_slope* t, const int32_t* gdof_node,
+                                  int32_t nnode, double cg_tol,
+                                  int32_t cg_maxiter, double nl_tol,
+                                  int32_t nl_maxiter, int32_t* cg_tot,
+                                  int32_t* nl_tot, double* uxmax, double* umax,
+                                  double* fmax, int32_t* errcode) {
+  if (!t || !gdof_node || nl_maxiter < 1)
+    return fail(SGEO_EARG, "sgeo_slope_run_srf: bad arguments");
+  sgeo_mesh* m = t->mesh;
+  int32_t tot = 0, nl = 0, ec = 0;
+  double fm = 0.0;
+  for (nl = 1; nl <= nl_maxiter; nl++) {
+    int32_t it = 0, conv = 0;
+    double ue = 0.0;
+    const int last = nl == nl_maxiter;
+    TRY(sgeo_slope_plastic_iter(t, cg_tol, cg_maxiter, nl_tol, last, &it, &ec,
+                                &ue, &conv, &fm));
+    if (ec != 0) break;
+    tot += it;
+    if (conv || last) break;
+  }
+  if (nl > nl_maxiter) nl = nl_maxiter;
+  *errcode = ec;
+  *cg_tot = tot;
+  *nl_tot = nl;
+  *fmax = fm;
+  // uxmax/umax over nodal displacements (src/slope.F90:401-413)
+  std::vector<double> x((size_t)m->neq + 1);
+  HIPCHK(hipMemcpyAsync(x.data(), t->d_x, x.size() * 8, hipMemcpyDeviceToHost,
+                        m->ctx->stream));
+  HIPCHK(hipStreamSynchronize(m->ctx->stream));
+  double ux = 0.0, um = 0.0;
+  for (int32_t n = 0; n < nnode; n++) {
+    double u3[3];
+    for (int cc = 0; cc < 3; cc++) {
+      const int32_t q = gdof_node[3 * (size_t)n + cc];
+      u3[cc] = q ? x[(size_t)q] : 0.0;
+      ux = std::max(ux, std::fabs(u3[cc]));
+    }
+    um = std::max(um, std::sqrt(u3[0] * u3[0] + u3[1] * u3[1] + u3[2] * u3[2]));
+  }
+  *uxmax = ux;
+  *umax = um;
+  return 0;
+}
+
+extern "C" int sgeo_slope_download(sgeo_slope* t, double* x, double* bodyload,
+                                   double* evpt, double* stress,
+                                   uint8_t* flags) {
+  if (!t) return fail(SGEO_EARG, "null slope");
+  sgeo_mesh* m = t->mesh;
+  hipStream_t s = m->ctx->stream;
+  HIPCHK(hipSetDevice(m->ctx->device));
+  const size_t n1 = (size_t)m->neq + 1;
+  const size_t ns = (size_t)m->nelmt * 27 * 6;
+  if (x) HIPCHK(hipMemcpyAsync(x, t->d_x, n1 * 8, hipMemcpyDeviceToHost, s));
+  if (bodyload)
+    HIPCHK(hipMemcpyAsync(bodyload, t->d_body, n1 * 8, hipMemcpyDeviceToHost, s));
+  if (evpt) HIPCHK(hipMemcpyAsync(evpt, t->d_evpt, ns * 8, hipMemcpyDeviceToHost, s));
+  if (stress)
+    HIPCHK(hipMemcpyAsync(stress, t->d_stress, ns * 8, hipMemcpyDeviceToHost, s));
+  if (flags)
+    HIPCHK(hipMemcpyAsync(flags, t->d_flags, (size_t)m->nelmt * 27,
+                          hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}

@@ -1,0 +1,153 @@
+"""Host mirror of the reference's `solver` module and the resident-mesh API.
+
+`pcg_solver` keeps the reference's interface and error behaviour
+(src/solver.f90:14-79): arrays in the reference layout, warm-started `u`,
+`cg_iter`, `errcode` (0 / -1) and `errtag`.  The solve itself runs in
+libsgeo.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, f64, i32, ptr
+from .spec import SpecMesh
+
+K_F64 = 0
+K_F32 = 1
+REDUCE_TREE = 0
+REDUCE_SEQUENTIAL = 1
+
+
+class Context:
+    """One HIP device + stream (sgeo_init)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(L.lib().sgeo_init(device, C.byref(h)), "sgeo_init")
+        self.h = h
+        self.device = device
+
+    def info(self):
+        name = C.create_string_buffer(256)
+        ncu = C.c_int()
+        hbm = C.c_size_t()
+        check(L.lib().sgeo_device_info(self.h, name, 256, C.byref(ncu), C.byref(hbm)),
+              "sgeo_device_info")
+        return dict(name=name.value.decode(), n_cu=ncu.value, hbm_bytes=hbm.value)
+
+    def synchronize(self):
+        check(L.lib().sgeo_synchronize(self.h), "sgeo_synchronize")
+
+    def set_reduction(self, mode: int):
+        """REDUCE_TREE (fast) or REDUCE_SEQUENTIAL (reference order)."""
+        check(L.lib().sgeo_set_reduction(self.h, mode), "sgeo_set_reduction")
+
+    def close(self):
+        if self.h:
+            L.lib().sgeo_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def pcg_solver(ctx: Context, neq: int, nelmt: int, k: np.ndarray, u: np.ndarray,
+               f: np.ndarray, dprecon: np.ndarray, gdof_elmt: np.ndarray,
+               cg_tol: float, cg_maxiter: int) -> Tuple[np.ndarray, int, int, str]:
+    """Drop-in for pcg_solver: returns (u, cg_iter, errcode, errtag).
+
+    `k` is storekm in the reference layout (nelmt x 81 x 81 with
+    k[e, c, r] = k(r+1, c+1, e+1), i.e. column-major per element),
+    `dprecon` the inverted diagonal, `gdof_elmt` (nelmt x 81)."""
+    k = f64(k)
+    if k.size != nelmt * 6561:
+        raise ValueError("k must hold 81*81*nelmt entries")
+    u = f64(u).copy()
+    f = f64(f)
+    dprecon = f64(dprecon)
+    g = i32(gdof_elmt)
+    if u.size != neq + 1 or f.size != neq + 1 or dprecon.size != neq + 1:
+        raise ValueError("vectors must have neq+1 entries")
+    if g.size != nelmt * 81:
+        raise ValueError("gdof_elmt must hold 81*nelmt entries")
+    it = C.c_int32()
+    ec = C.c_int32()
+    tag = C.create_string_buffer(251)
+    check(L.lib().sgeo_pcg_solver(ctx.h, neq, nelmt, ptr(k), ptr(u), ptr(f),
+                                  ptr(dprecon), ptr(g, C.c_int32), cg_tol,
+                                  cg_maxiter, C.byref(it), C.byref(ec), tag),
+          "sgeo_pcg_solver")
+    return u, it.value, ec.value, tag.raw[:250].decode().rstrip()
+
+
+class DeviceMesh:
+    """A mesh resident in HBM (sgeo_mesh): gdof_elmt, corners, materials and
+    the element stiffness."""
+
+    def __init__(self, ctx: Context, spec: SpecMesh, nmat: int):
+        self.ctx = ctx
+        self.spec = spec
+        self.nmat = nmat
+        h = C.c_void_p()
+        self._corners = f64(spec.corners)
+        self._gdof = i32(spec.gdof_elmt)
+        self._mat = i32(spec.mat_id)
+        check(L.lib().sgeo_mesh_create(ctx.h, spec.nelmt, spec.neq, ptr(self._corners),
+                                       ptr(self._gdof, C.c_int32),
+                                       ptr(self._mat, C.c_int32), nmat, C.byref(h)),
+              "sgeo_mesh_create")
+        self.h = h
+        self.kprec: Optional[int] = None
+
+    def build_stiffness(self, gam, ym, nu, kprec: int = K_F64,
+                        want_extload: bool = True):
+        """Device stiffness_bodyload; returns (extload, dinv)."""
+        n1 = self.spec.neq + 1
+        ext = np.zeros(n1) if want_extload else None
+        dinv = np.zeros(n1)
+        check(L.lib().sgeo_mesh_build_stiffness(self.h, ptr(f64(gam)), ptr(f64(ym)),
+                                                ptr(f64(nu)), kprec, ptr(ext),
+                                                ptr(dinv)),
+              "sgeo_mesh_build_stiffness")
+        self.kprec = kprec
+        return ext, dinv
+
+    def upload_stiffness(self, storekm, dinv, kprec: int = K_F64):
+        check(L.lib().sgeo_mesh_upload_stiffness(self.h, ptr(f64(storekm)),
+                                                 ptr(f64(dinv)), kprec),
+              "sgeo_mesh_upload_stiffness")
+        self.kprec = kprec
+
+    def download_stiffness(self) -> np.ndarray:
+        k = np.zeros((self.spec.nelmt, 81, 81))
+        check(L.lib().sgeo_mesh_download_stiffness(self.h, ptr(k)),
+              "sgeo_mesh_download_stiffness")
+        return k
+
+    def matvec(self, p) -> np.ndarray:
+        kp = np.zeros(self.spec.neq + 1)
+        check(L.lib().sgeo_mesh_matvec(self.h, ptr(f64(p)), ptr(kp)), "sgeo_mesh_matvec")
+        return kp
+
+    def nbytes(self) -> int:
+        b = C.c_size_t()
+        check(L.lib().sgeo_mesh_bytes(self.h, C.byref(b)), "sgeo_mesh_bytes")
+        return b.value
+
+    def close(self):
+        if self.h:
+            L.lib().sgeo_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,215 @@
+"""Writes the golden fixtures of tests/golden/ from the reference itself.
+
+Runs the unmodified serial reference built by oracle/Makefile
+(oracle/_ref/semgeotech_tap, with the pcg_solver / mohcouf taps of
+oracle/ref_tap.c) on small seeded meshes written in the reference's own
+input format, and on the reference's example `input/validation1`, then
+stores inputs and outputs as .npz (no pickles).  Needs /root/reference, so
+it runs in the build container only; the fixtures it writes are committed.
+
+    make -C oracle && python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from specfem3d_geotech_amd.mesh import Material, read_reference_input, slope_mesh, \
+    write_reference_input  # noqa: E402
+
+REF_TAP = os.path.join(ROOT, "oracle", "_ref", "semgeotech_tap")
+GLL_PROBE = os.path.join(ROOT, "oracle", "_ref", "gll_probe")
+FN_PROBE = os.path.join(ROOT, "oracle", "_ref", "fn_probe")
+REF_INPUT = "/root/reference/input"
+
+
+def run_tap(sem: str, workdir: str, mode: str = "full") -> None:
+    for f in ("pcg_static.bin", "pcg_calls.bin", "pcg_times.txt", "mohcouf.bin"):
+        p = os.path.join(workdir, f)
+        if os.path.exists(p):
+            os.remove(p)
+    env = dict(os.environ, SGEO_TAP_DIR=workdir, SGEO_TAP_MODE=mode)
+    # flang places the reference's automatic arrays on the stack
+    r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {REF_TAP} {sem}"],
+                       cwd=workdir, env=env, capture_output=True, text=True)
+    if "Result summary" not in open(find_summary(workdir)).read():
+        raise RuntimeError(f"reference run failed: {r.stdout[-2000:]} {r.stderr}")
+
+
+def find_summary(workdir: str) -> str:
+    for root, _, files in os.walk(workdir):
+        for f in files:
+            if f.endswith("_summary"):
+                return os.path.join(root, f)
+    raise FileNotFoundError("no summary file")
+
+
+def parse_summary(path: str) -> np.ndarray:
+    """Rows (srf, cg_tot, nl_tot, uxmax, umax, fmax) as float64; the
+    reference writes list-directed full precision (src/slope.F90:415)."""
+    rows = []
+    lines = open(path).read().splitlines()
+    start = [i for i, ln in enumerate(lines) if "SRF, CGITER" in ln][0] + 1
+    for ln in lines[start:]:
+        if "ELAPSED" in ln or not ln.strip():
+            break
+        rows.append([float(t) for t in ln.split()])
+    return np.array(rows, dtype=np.float64)
+
+
+def read_static(workdir: str):
+    raw = np.fromfile(os.path.join(workdir, "pcg_static.bin"), dtype=np.uint8)
+    hdr = raw[:16].view(np.int32)
+    neq, nel, nedof = int(hdr[0]), int(hdr[1]), int(hdr[2])
+    o = 16
+    k = raw[o:o + 8 * nedof * nedof * nel].view(np.float64).reshape(nel, nedof, nedof)
+    o += 8 * nedof * nedof * nel
+    dinv = raw[o:o + 8 * (neq + 1)].view(np.float64).copy()
+    o += 8 * (neq + 1)
+    g = raw[o:o + 4 * nedof * nel].view(np.int32).reshape(nel, nedof).copy()
+    return neq, k, dinv, g
+
+
+def read_calls(workdir: str, neq: int):
+    raw = np.fromfile(os.path.join(workdir, "pcg_calls.bin"), dtype=np.uint8)
+    rec = 16 + 3 * 8 * (neq + 1)
+    n = raw.size // rec
+    cg = np.zeros(n, np.int32)
+    uin = np.zeros((n, neq + 1)); f = np.zeros((n, neq + 1)); uout = np.zeros((n, neq + 1))
+    for c in range(n):
+        r = raw[c * rec:(c + 1) * rec]
+        cg[c] = r[:16].view(np.int32)[2]
+        uin[c] = r[16:16 + 8 * (neq + 1)].view(np.float64)
+        f[c] = r[16 + 8 * (neq + 1):16 + 16 * (neq + 1)].view(np.float64)
+        uout[c] = r[16 + 16 * (neq + 1):].view(np.float64)
+    return cg, uin, f, uout
+
+
+def mesh_arrays(m):
+    gam, ym, nu, phi, coh, psi = m.material_arrays()
+    return dict(coord=m.coord, conn=m.conn, mat_id=m.mat_id,
+                materials=np.stack([gam, ym, nu, phi, coh, psi], 1),
+                bc_ux=m.bc["ux"], bc_uy=m.bc["uy"], bc_uz=m.bc["uz"])
+
+
+def slope_case(name, mesh, control, keep_calls):
+    work = tempfile.mkdtemp(prefix=f"golden_{name}_")
+    sem = write_reference_input(mesh, work, name, **control)
+    run_tap(sem, work)
+    neq, k, dinv, g = read_static(work)
+    cg, uin, f, uout = read_calls(work, neq)
+    moh = np.fromfile(os.path.join(work, "mohcouf.bin"), dtype=np.float64)
+    summ = parse_summary(find_summary(work))
+    out = dict(mesh_arrays(mesh))
+    out.update(
+        neq=np.int32(neq), gdof_elmt=g, dinv=dinv,
+        k_sha256=np.frombuffer(hashlib.sha256(np.ascontiguousarray(k).tobytes())
+                               .digest(), dtype=np.uint8),
+        k_elem0=k[0].copy(), extload=f[0].copy(), cg_log=cg, summary=summ,
+        call_idx=np.array(keep_calls, np.int32),
+        call_uin=uin[keep_calls], call_f=f[keep_calls], call_uout=uout[keep_calls],
+        # f of every GLL point at the final body pass of the last SRF
+        last_pass_f=moh[-27 * mesh.nelmt:].copy(),
+        # ... and at the final body pass of every SRF (row s)
+        final_pass_f=np.stack([moh[(p - 1) * 27 * mesh.nelmt:p * 27 * mesh.nelmt]
+                               for p in np.cumsum(summ[:, 2].astype(int))]),
+        control=np.array([control["cg_tol"], control["cg_maxiter"], control["nl_tol"],
+                          control["nl_maxiter"]]),
+        srf=np.array(control["srf"]))
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+    shutil.rmtree(work)
+    print(name, "neq", neq, "calls", len(cg), "summary", summ.tolist())
+
+
+def validation1_case():
+    """The reference's own example, SRF 1.0 only (the first of its list)."""
+    work = tempfile.mkdtemp(prefix="golden_v1_")
+    os.makedirs(os.path.join(work, "input"))
+    os.makedirs(os.path.join(work, "output"))
+    for f in os.listdir(REF_INPUT):
+        if f.startswith("validation1") and not f.endswith(".sem") and \
+                "phinu" not in f:
+            shutil.copy(os.path.join(REF_INPUT, f), os.path.join(work, "input"))
+    src = open(os.path.join(REF_INPUT, "validation1.sem")).read()
+    src = src.replace("nsrf=9, srf=1.0 1.5 2.0 2.15 2.16 2.17 2.18 2.19 2.20",
+                      "nsrf=1, srf=1.0")
+    sem = os.path.join(work, "input", "validation1.sem")
+    open(sem, "w").write(src)
+    run_tap(sem, work, mode="time")
+    summ = parse_summary(find_summary(work))
+    times = np.loadtxt(os.path.join(work, "pcg_times.txt"), ndmin=2)
+    mesh, ctl = read_reference_input(sem)
+    out = dict(mesh_arrays(mesh))
+    out.update(cg_log=times[:, 1].astype(np.int32), summary=summ,
+               control=np.array([ctl["cg_tol"], ctl["cg_maxiter"], ctl["nl_tol"],
+                                 ctl["nl_maxiter"]]), srf=np.array(ctl["srf"]))
+    np.savez_compressed(os.path.join(HERE, "validation1_srf1.npz"), **out)
+    shutil.rmtree(work)
+    print("validation1", summ.tolist())
+
+
+def gll_case():
+    txt = subprocess.run([GLL_PROBE], capture_output=True, text=True, check=True).stdout
+    vals = [float(t) for t in txt.split() if not t.startswith("#") and
+            t[0] in "-0123456789" and ("E" in t or "e" in t)]
+    v = np.array(vals)
+    np.savez_compressed(os.path.join(HERE, "gll_tables.npz"),
+                        zwgljd=v[:6].reshape(3, 2), gll_weights=v[6:33],
+                        dlagrange_gll=v[33:33 + 2187], dshape_hex8=v[2220:2220 + 648])
+    print("gll tables", v.size)
+
+
+def fn_case():
+    rng = np.random.default_rng(5)
+    n = 600
+    S = rng.normal(0, 100, (n, 6))
+    S[:, :3] -= 150
+    S[0] = [-100, -100, -100, 0, 0, 0]      # hydrostatic
+    S[1] = [-100, -200, -200, 0, 0, 0]      # triaxial extension
+    S[2] = [-200, -100, -100, 0, 0, 0]      # triaxial compression
+    P = np.stack([rng.uniform(0, 45, n), rng.uniform(0, 50, n),
+                  rng.uniform(0, 20, n)], 1)
+    rec = np.concatenate([S, P], 1)
+    work = tempfile.mkdtemp(prefix="golden_fn_")
+    fin, fout = os.path.join(work, "in.bin"), os.path.join(work, "out.bin")
+    with open(fin, "wb") as fh:
+        np.array([n], np.int32).tofile(fh)
+        rec.astype(np.float64).tofile(fh)
+    subprocess.run([FN_PROBE, fin, fout], check=True)
+    out = np.fromfile(fout, dtype=np.float64)
+    A = out[:n * 115].reshape(n, 115)
+    B = out[n * 115:].reshape(-1, 6)
+    np.savez_compressed(os.path.join(HERE, "constitutive.npz"), inputs=rec,
+                        outputs=A, srf_outputs=B)
+    shutil.rmtree(work)
+    print("constitutive", A.shape, B.shape)
+
+
+def main():
+    base = dict(cg_tol=1e-8, cg_maxiter=5000, nl_tol=5e-4, nl_maxiter=300)
+    slope_case("slope_4x2x3", slope_mesh(4, 2, 3),
+               dict(base, srf=[1.0, 1.5]), keep_calls=list(range(12)) + [20, 100, 310])
+    layered = slope_mesh(8, 4, 6, materials=[
+        Material(gam=20.0, ym=1e5, nu=0.3, phi=25.0, coh=8.0),
+        Material(gam=20.0, ym=1e5, nu=0.3, phi=30.0, coh=20.0)],
+        layer_z=[10.0], jitter=0.02, seed=1)
+    slope_case("slope_8x4x6_layered", layered,
+               dict(base, nl_maxiter=100, srf=[1.0, 1.2, 1.4, 1.6, 1.8, 2.0]),
+               keep_calls=[0, 1, 2])
+    validation1_case()
+    gll_case()
+    fn_case()
+
+
+if __name__ == "__main__":
+    main()

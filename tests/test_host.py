@@ -1,0 +1,99 @@
+"""Host-side pieces (no GPU): numbering, inverse map, colouring, input
+format round trip, per-trial scalars, and the C-ABI library's exports."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import ROOT, golden, mesh_from_golden
+from specfem3d_geotech_amd import plastic
+from specfem3d_geotech_amd.mesh import (Material, read_reference_input, slope_mesh,
+                                        write_reference_input)
+from specfem3d_geotech_amd.spec import hex2spec, structured_colours
+
+
+def test_inverse_map_reproduces_element_order():
+    z = golden("slope_8x4x6_layered")
+    spec = hex2spec(mesh_from_golden(z))
+    rowptr, slots = spec.inverse_map()
+    flat = spec.gdof_elmt.reshape(-1)
+    for g in (1, 2, 17, spec.neq // 2, spec.neq):
+        s = slots[rowptr[g]:rowptr[g + 1]]
+        assert (flat[s] == g).all()
+        assert (np.diff(s // 81) > 0).all()  # ascending element order
+    assert rowptr[-1] == flat.size
+
+
+def test_colourings_are_conflict_free():
+    m = slope_mesh(5, 3, 4)
+    spec = hex2spec(m)
+    for col in (spec.colours(), structured_colours(m)):
+        for c in np.unique(col):
+            nodes = spec.g_num[col == c].ravel()
+            assert np.unique(nodes).size == nodes.size
+    assert structured_colours(m).max() == 7
+
+
+def test_reference_input_round_trip(tmp_path):
+    m = slope_mesh(4, 2, 3, materials=[Material(20, 1e5, 0.3, 25, 8),
+                                       Material(20, 1e5, 0.3, 30, 20)],
+                   layer_z=[10.0], jitter=0.02, seed=7)
+    sem = write_reference_input(m, str(tmp_path), "rt", cg_tol=1e-8,
+                                cg_maxiter=5000, nl_tol=5e-4, nl_maxiter=300,
+                                srf=[1.0, 1.25])
+    m2, ctl = read_reference_input(sem)
+    assert np.array_equal(m2.coord, m.coord)
+    assert np.array_equal(m2.conn, m.conn)
+    assert np.array_equal(m2.mat_id, m.mat_id)
+    for k in ("ux", "uy", "uz"):
+        assert np.array_equal(m2.bc[k], m.bc[k])
+    assert ctl["srf"] == [1.0, 1.25] and ctl["nl_maxiter"] == 300
+    assert np.array_equal(hex2spec(m2).gdof_elmt, hex2spec(m).gdof_elmt)
+
+
+def test_trial_scalars_match_oracle():
+    """The product's host restatement of strength_reduction / dt_viscoplas
+    against the oracle (itself pinned bit for bit to the reference)."""
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        n = 3
+        coh, nu = rng.uniform(1, 40, n), rng.uniform(0.1, 0.45, n)
+        phi, psi = rng.uniform(0, 45, n), rng.uniform(0, 15, n)
+        ym = rng.uniform(1e4, 1e6, n)
+        srf = float(rng.choice([1.0, rng.uniform(0.8, 3.0)]))
+        for phinu in (False, True):
+            a = plastic.strength_reduction(srf, phinu, coh, nu, phi, psi)
+            b = O.strength_reduction(srf, phinu, coh, nu, phi, psi)
+            for x, y in zip(a[:4], b[:4]):
+                assert np.array_equal(x, y)
+            assert a[4] == b[4]
+            assert plastic.dt_viscoplas(a[1], a[2], ym) == O.dt_viscoplas(b[1], b[2], ym)
+
+
+def _declared_symbols():
+    hdr = open(os.path.join(ROOT, "include", "sgeo.h")).read()
+    return sorted(set(re.findall(r"\b(sgeo_[a-z0-9_]+)\s*\(", hdr)))
+
+
+def test_capi_library_exports_every_declared_symbol():
+    lib = os.path.join(ROOT, "specfem3d_geotech_amd", "libsgeo.so")
+    if not os.path.exists(lib):
+        pytest.fail("libsgeo.so not built (python -m specfem3d_geotech_amd.build)")
+    out = subprocess.run(["nm", "-D", "--defined-only", lib], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (sgeo_[a-z0-9_]+)", out))
+    missing = [s for s in _declared_symbols() if s not in exported]
+    assert not missing, missing
+    # the ctypes table covers the header too
+    from specfem3d_geotech_amd._lib import SIGNATURES
+    assert set(_declared_symbols()) <= set(SIGNATURES), \
+        set(_declared_symbols()) - set(SIGNATURES)
+
+
+def test_capi_library_loads_without_gpu():
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(ROOT, "specfem3d_geotech_amd", "libsgeo.so"))
+    assert hasattr(lib, "sgeo_pcg_solver")
