@@ -66,6 +66,14 @@ int sgeo_synchronize(sgeo_ctx *ctx);
 #define SGEO_REDUCE_SEQUENTIAL 1
 int sgeo_set_reduction(sgeo_ctx *ctx, int mode);
 
+/* HIP-event timing of the element-product (EBE matvec) kernel inside PCG
+ * loops, on the context's stream (no counterpart in the reference).  Reads
+ * the launches timed and their summed duration since the last reset; then
+ * enable >= 0 resets the counters and turns timing on (1) or off (0);
+ * enable < 0 only reads. */
+int sgeo_profile(sgeo_ctx *ctx, int enable, int64_t *launches,
+                 double *total_ms);
+
 /* ---- Seam 1: drop-in for pcg_solver ------------------------------------
  * Replaces subroutine pcg_solver(neq,nelmt,k,u,f,dprecon,gdof_elmt,cg_iter,
  * errcode,errtag) (src/solver.f90:14-79; called from src/slope.F90:293,

@@ -29,6 +29,8 @@ SIGNATURES = {
                                    C.POINTER(C.c_size_t)]),
     "sgeo_synchronize": (C.c_int, [vp]),
     "sgeo_set_reduction": (C.c_int, [vp, C.c_int]),
+    "sgeo_profile": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int64),
+                               C.POINTER(C.c_double)]),
     "sgeo_pcg_solver": (C.c_int, [vp, C.c_int32, C.c_int32, d_p, d_p, d_p, d_p, i_p,
                                   C.c_double, C.c_int32, i_p, i_p, C.c_char_p]),
     "sgeo_mesh_create": (C.c_int, [vp, C.c_int32, C.c_int32, d_p, i_p, i_p,

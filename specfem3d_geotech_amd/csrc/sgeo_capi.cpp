@@ -64,7 +64,36 @@ struct sgeo_ctx {
   double* d_f = nullptr;
   size_t uf_cap = 0;
   int reduction = SGEO_REDUCE_TREE;
+  // HIP-event timing of the element-product kernel launches in PCG loops
+  bool prof = false;
+  std::vector<hipEvent_t> ev;  // pairs, reused per chunk
+  int ev_used = 0;
+  int64_t prof_count = 0;
+  double prof_ms = 0.0;
 };
+
+static int prof_flush(sgeo_ctx* c) {
+  // events of completed launches (caller has synchronized the stream)
+  for (int i = 0; i + 1 < c->ev_used; i += 2) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+    c->prof_ms += ms;
+    c->prof_count++;
+  }
+  c->ev_used = 0;
+  return 0;
+}
+
+static int prof_mark(sgeo_ctx* c) {
+  if (!c->prof) return 0;
+  if ((size_t)c->ev_used >= c->ev.size()) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    c->ev.push_back(e);
+  }
+  HIPCHK(hipEventRecord(c->ev[c->ev_used++], c->stream));
+  return 0;
+}
 
 struct sgeo_mesh {
   sgeo_ctx* ctx = nullptr;
@@ -167,6 +196,7 @@ extern "C" int sgeo_finalize(sgeo_ctx* c) {
   if (c->d_ps) hipFree(c->d_ps);
   if (c->h_sc) hipHostFree(c->h_sc);
   if (c->h_ps) hipHostFree(c->h_ps);
+  for (hipEvent_t e : c->ev) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -189,6 +219,20 @@ extern "C" int sgeo_set_reduction(sgeo_ctx* c, int mode) {
   if (!c || (mode != SGEO_REDUCE_TREE && mode != SGEO_REDUCE_SEQUENTIAL))
     return fail(SGEO_EARG, "sgeo_set_reduction: mode %d", mode);
   c->reduction = mode;
+  return 0;
+}
+
+extern "C" int sgeo_profile(sgeo_ctx* c, int enable, int64_t* launches,
+                            double* total_ms) {
+  if (!c) return fail(SGEO_EARG, "null ctx");
+  if (launches) *launches = c->prof_count;
+  if (total_ms) *total_ms = c->prof_ms;
+  if (enable >= 0) {
+    c->prof = enable != 0;
+    c->prof_count = 0;
+    c->prof_ms = 0.0;
+    c->ev_used = 0;
+  }
   return 0;
 }
 
@@ -327,7 +371,7 @@ extern "C" int sgeo_mesh_build_stiffness(sgeo_mesh* m, const double* gam,
     // extload(egdof) = extload(egdof) - eld*gam, element order
     launch_scale_by_mat(s, m->nelmt, d_eld, m->d_mat, d_gam, d_kdiag);
     HIPCHK(hipMemsetAsync(d_ext, 0, n1 * 8, s));
-    launch_assemble_acc(s, n1, m->d_rowptr, m->d_slots, d_kdiag, d_ext, 1);
+    launch_assemble_acc(s, n1, m->d_rowptr, m->d_slots, d_kdiag, d_ext, 1, nullptr);
     HIPCHK(hipMemcpyAsync(extload, d_ext, n1 * 8, hipMemcpyDeviceToHost, s));
   }
   if (kprec == SGEO_K_F32)
@@ -412,22 +456,22 @@ static int pcg_device(sgeo_mesh* m, double* d_u, const double* d_f, double tol,
   launch_pcg_begin(s, n1, d_u, c->d_part, c->d_sc, tol, maxiter);
   ebe_elem(m, d_u, m->d_Y, 2);
   launch_assemble(s, n1, m->d_rowptr, m->d_slots, m->d_Y, m->d_kp, c->d_sc, 2);
-  launch_pcg_init(s, n1, d_f, m->d_kp, m->d_dinv, m->d_r, m->d_z, m->d_p,
-                  c->d_part, c->d_sc);
   const int seq = c->reduction == SGEO_REDUCE_SEQUENTIAL;
-  if (seq) launch_pcg_seqdot(s, n1, m->d_r, m->d_z, c->d_sc, 0);
-  int32_t enq = 0;
+  launch_pcg_init(s, n1, d_f, m->d_kp, m->d_dinv, m->d_r, m->d_z, m->d_p,
+                  c->d_part, c->d_sc, seq);
+  int32_t enq = 0, iter_prev = 0;
   int chunk = 8;
+  c->ev_used = 0;
   for (;;) {
     const int n = std::min<int32_t>(chunk, maxiter - enq);
     for (int it = 0; it < n; it++) {
+      TRY(prof_mark(c));
       ebe_elem(m, m->d_p, m->d_Y, 1);
+      TRY(prof_mark(c));
       launch_pcg_assemble_dot(s, n1, m->d_rowptr, m->d_slots, m->d_Y, m->d_p,
                               m->d_kp, c->d_part, c->d_sc, seq);
-      if (seq) launch_pcg_seqdot(s, n1, m->d_p, m->d_kp, c->d_sc, 1);
       launch_pcg_update(s, n1, m->d_p, m->d_kp, m->d_dinv, d_u, m->d_r,
                         m->d_z, c->d_part, c->d_sc, seq);
-      if (seq) launch_pcg_seqdot(s, n1, m->d_r, m->d_z, c->d_sc, 2);
       launch_pcg_pupdate(s, n1, m->d_z, m->d_p, c->d_sc);
     }
     enq += n;
@@ -435,6 +479,13 @@ static int pcg_device(sgeo_mesh* m, double* d_u, const double* d_f, double tol,
     HIPCHK(hipMemcpyAsync(c->h_sc, c->d_sc, sizeof(PcgScalars),
                           hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    if (c->prof) {
+      // only the launches of iterations that ran (later ones return at once)
+      const int ran = c->h_sc->iter - iter_prev;
+      c->ev_used = std::min(c->ev_used, 2 * std::max(ran, 0));
+      TRY(prof_flush(c));
+      iter_prev = c->h_sc->iter;
+    }
     if (c->h_sc->done) break;
     if (enq >= maxiter)
       return fail(SGEO_ESTATE, "PCG: %d iterations enqueued without a verdict",
@@ -613,14 +664,16 @@ extern "C" int sgeo_slope_plastic_iter(sgeo_slope* t, double cg_tol,
   launch_plastic_body(s, m->nelmt, nullptr, m->d_corners, m->d_gdof, m->d_mat,
                       t->d_mp, t->d_x, t->d_evpt, t->d_stress, t->d_BL,
                       t->d_flags, c->d_ps, is_last, 0, nullptr, nullptr);
-  // bodyload(egdof) += bload unless converged/last (src/slope.F90:380-384)
+  // bodyload(egdof) += bload unless converged/last (src/slope.F90:380-384);
+  // the kernel itself returns when nl_isconv is set on the device
+  if (!is_last)
+    launch_assemble_acc(s, n1, m->d_rowptr, m->d_slots, t->d_BL, t->d_body, 0,
+                        c->d_ps);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(c->h_ps, c->d_ps, sizeof(PlasticScalars),
                         hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   const int conv = c->h_ps->nl_isconv;
-  if (!(conv || is_last))
-    launch_assemble_acc(s, n1, m->d_rowptr, m->d_slots, t->d_BL, t->d_body, 0);
-  HIPCHK(hipGetLastError());
   if (uerr) *uerr = c->h_ps->uerr;
   if (nl_isconv) *nl_isconv = conv;
   if (fmax) {

@@ -29,7 +29,6 @@ struct PcgScalars {
   int32_t done;        // 1 = converged (stop test passed), 2 = failed
   int32_t conv_iter;   // cg_iter reported to the caller
   int32_t u_nonzero;   // max|u| > 0 at entry (src/solver.f90:39)
-  uint32_t ticket[4];  // last-block-done counters of the reductions
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -79,24 +78,4 @@ __device__ __forceinline__ double block_max(double v, double* lds) {
   }
   __syncthreads();
   return s;  // valid in thread 0
-}
-
-// Last-block-done pattern: every block publishes its partial, then takes a
-// ticket; the block holding the last ticket sees all partials and reduces
-// them in index order.  The ticket is reset by that block, so the pattern is
-// replayable (graph capture) without a host-side memset.
-__device__ __forceinline__ bool last_block(uint32_t* ticket) {
-  __shared__ bool is_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t = atomicAdd(ticket, 1u);
-    is_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (is_last) {
-    __threadfence();
-    if (threadIdx.x == 0) *ticket = 0u;
-  }
-  return is_last;
 }

@@ -21,7 +21,6 @@ struct PlasticScalars {
   unsigned long long fmax_bits;  // max(0, f) as ordered bits (f >= 0)
   int32_t nl_isconv;
   int32_t n_plastic;
-  uint32_t ticket[2];
 };
 
 unsigned vec_reduce_blocks(int64_t n1);
@@ -42,7 +41,7 @@ void launch_assemble(hipStream_t s, int64_t n1, const int32_t* rowptr,
                      const PcgScalars* sc, int skip_mode);
 void launch_assemble_acc(hipStream_t s, int64_t n1, const int32_t* rowptr,
                          const int32_t* slots, const double* Y, double* out,
-                         int subtract);
+                         int subtract, const PlasticScalars* ps);
 void launch_invert_diag(hipStream_t s, int64_t n1, double* d);
 void launch_scale_by_mat(hipStream_t s, int32_t nelmt, const double* eld,
                          const int32_t* mat_id, const double* gam, double* Y);
@@ -50,13 +49,11 @@ void launch_pcg_begin(hipStream_t s, int64_t n1, const double* u, double* part,
                       PcgScalars* sc, double tol, int32_t maxiter);
 void launch_pcg_init(hipStream_t s, int64_t n1, const double* f,
                      const double* kp, const double* dinv, double* r, double* z,
-                     double* p, double* part, PcgScalars* sc);
+                     double* p, double* part, PcgScalars* sc, int seq);
 void launch_pcg_assemble_dot(hipStream_t s, int64_t n1, const int32_t* rowptr,
                              const int32_t* slots, const double* Y,
                              const double* p, double* kp, double* part,
                              PcgScalars* sc, int seq);
-void launch_pcg_seqdot(hipStream_t s, int64_t n1, const double* a,
-                       const double* b, PcgScalars* sc, int which);
 void launch_pcg_update(hipStream_t s, int64_t n1, const double* p,
                        const double* kp, const double* dinv, double* u,
                        double* r, double* z, double* part, PcgScalars* sc,

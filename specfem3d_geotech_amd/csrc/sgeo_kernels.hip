@@ -237,393 +237,6 @@ __global__ void k_round_f32(int64_t n, const double* __restrict__ a,
 }
 
 // --------------------------------------------------------------------------
-// EBE matrix-vector product, element part: Y[e] = K_e p_e
-// --------------------------------------------------------------------------
-// One wave per element, 4 waves per block.  Lane l owns rows l and 64+l
-// (l < 17).  p_e is gathered into LDS; column c of K_e is read as one
-// 512-byte row block plus a 136-byte tail.  Each row sums c = 0..80 in
-// order, as matmul(km, p(egdof)) (src/solver.f90:57).
-// skip_mode: 0 none, 1 skip when sc->done, 2 skip when !sc->u_nonzero.
-template <typename KT>
-__global__ __launch_bounds__(256) void k_ebe_elem(
-    int32_t nelmt, const KT* __restrict__ K, const int32_t* __restrict__ gdof,
-    const double* __restrict__ p, double* __restrict__ Y,
-    const PcgScalars* __restrict__ sc, int skip_mode) {
-  if (skip_mode == 1 && sc->done) return;
-  if (skip_mode == 2 && !sc->u_nonzero) return;
-  __shared__ double pe[4][81];
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int64_t e = (int64_t)blockIdx.x * 4 + w;
-  const bool valid = e < nelmt;
-  if (valid) {
-    const int32_t* g = gdof + e * 81;
-    pe[w][l] = p[g[l]];
-    if (l < 17) pe[w][64 + l] = p[g[64 + l]];
-  }
-  __syncthreads();
-  if (!valid) return;
-  const KT* ke = K + e * 6561;
-  double a0 = 0.0, a1 = 0.0;
-  const bool tail = l < 17;
-#pragma unroll 9
-  for (int c = 0; c < 81; c++) {
-    const double pc = pe[w][c];
-    a0 = a0 + (double)ke[c * 81 + l] * pc;
-    if (tail) a1 = a1 + (double)ke[c * 81 + 64 + l] * pc;
-  }
-  double* ye = Y + e * 81;
-  ye[l] = a0;
-  if (tail) ye[64 + l] = a1;
-}
-
-// --------------------------------------------------------------------------
-// assembly: kp_g = sum over element slots of Y, ascending element order
-// (the order of kp(egdof)=kp(egdof)+... in src/solver.f90:54-58)
-// --------------------------------------------------------------------------
-__device__ __forceinline__ double gather_row(const int32_t* __restrict__ rowptr,
-                                             const int32_t* __restrict__ slots,
-                                             const double* __restrict__ Y,
-                                             int64_t g) {
-  double acc = 0.0;
-  const int32_t b = rowptr[g], en = rowptr[g + 1];
-  for (int32_t s = b; s < en; s++) acc = acc + Y[slots[s]];
-  return acc;
-}
-
-// plain assembly into out (out[0] = 0); used for the initial K u
-__global__ __launch_bounds__(SGEO_VEC_THREADS) void k_assemble(
-    int64_t n1, const int32_t* __restrict__ rowptr,
-    const int32_t* __restrict__ slots, const double* __restrict__ Y,
-    double* __restrict__ out, const PcgScalars* __restrict__ sc,
-    int skip_mode) {
-  if (skip_mode == 2 && !sc->u_nonzero) return;
-  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n1;
-       g += (int64_t)gridDim.x * blockDim.x)
-    out[g] = (g == 0) ? 0.0 : gather_row(rowptr, slots, Y, g);
-}
-
-// accumulate into an existing vector: out_g = out_g (+|-) Y terms in element
-// order, out[0] = 0.  (bodyload(egdof)=bodyload(egdof)+bload,
-// src/slope.F90:382; extload(egdof)=extload(egdof)-eld*gam,
-// src/preprocess.f90:136)
-__global__ __launch_bounds__(SGEO_VEC_THREADS) void k_assemble_acc(
-    int64_t n1, const int32_t* __restrict__ rowptr,
-    const int32_t* __restrict__ slots, const double* __restrict__ Y,
-    double* __restrict__ out, int subtract) {
-  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n1;
-       g += (int64_t)gridDim.x * blockDim.x) {
-    if (g == 0) { out[0] = 0.0; continue; }
-    double acc = out[g];
-    const int32_t b = rowptr[g], en = rowptr[g + 1];
-    if (subtract) {
-      for (int32_t s = b; s < en; s++) acc = acc - Y[slots[s]];
-    } else {
-      for (int32_t s = b; s < en; s++) acc = acc + Y[slots[s]];
-    }
-    out[g] = acc;
-  }
-}
-
-// --------------------------------------------------------------------------
-// PCG vector kernels (src/solver.f90:38-73)
-// --------------------------------------------------------------------------
-__device__ __forceinline__ double ld_coherent(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// start of a solve: max|u| (src/solver.f90:39) and scalar reset
-__global__ __launch_bounds__(SGEO_VEC_THREADS) void k_pcg_begin(
-    int64_t n1, const double* __restrict__ u, double* __restrict__ part,
-    PcgScalars* __restrict__ sc, double tol, int32_t maxiter) {
-  __shared__ double red[SGEO_VEC_THREADS / 64];
-  const int64_t base = (int64_t)blockIdx.x * SGEO_VEC_THREADS * SGEO_VEC_ITEMS;
-  double m = 0.0;
-#pragma unroll
-  for (int k = 0; k < SGEO_VEC_ITEMS; k++) {
-    const int64_t g = base + threadIdx.x + (int64_t)k * SGEO_VEC_THREADS;
-    if (g < n1) m = fmax(m, fabs(u[g]));
-  }
-  m = block_max<SGEO_VEC_THREADS / 64>(m, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = m;
-  if (last_block(&sc->ticket[0])) {
-    double v = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += SGEO_VEC_THREADS)
-      v = fmax(v, ld_coherent(part + b));
-    v = block_max<SGEO_VEC_THREADS / 64>(v, red);
-    if (threadIdx.x == 0) {
-      sc->u_nonzero = v > 0.0 ? 1 : 0;
-      sc->iter = 0;
-      sc->done = 0;
-      sc->conv_iter = 0;
-      sc->tol = tol;
-      sc->maxiter = maxiter;
-    }
-  }
-}
-
-// r = f - kp (kp = K u if u != 0, else 0); z = dinv*r; p = z; rz = r.z
-__global__ __launch_bounds__(SGEO_VEC_THREADS) void k_pcg_init(
-    int64_t n1, const double* __restrict__ f, const double* __restrict__ kp,
-    const double* __restrict__ dinv, double* __restrict__ r,
-    double* __restrict__ z, double* __restrict__ p, double* __restrict__ part,
-    PcgScalars* __restrict__ sc) {
-  __shared__ double red[SGEO_VEC_THREADS / 64];
-  const bool unz = sc->u_nonzero != 0;
-  const int64_t base = (int64_t)blockIdx.x * SGEO_VEC_THREADS * SGEO_VEC_ITEMS;
-  double s = 0.0;
-#pragma unroll
-  for (int k = 0; k < SGEO_VEC_ITEMS; k++) {
-    const int64_t g = base + threadIdx.x + (int64_t)k * SGEO_VEC_THREADS;
-    if (g < n1) {
-      const double rg = f[g] - (unz ? kp[g] : 0.0);
-      const double zg = dinv[g] * rg;
-      r[g] = rg;
-      z[g] = zg;
-      p[g] = zg;
-      s = s + rg * zg;
-    }
-  }
-  s = block_sum<SGEO_VEC_THREADS / 64>(s, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
-  if (last_block(&sc->ticket[1])) {
-    double v = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += SGEO_VEC_THREADS)
-      v = v + ld_coherent(part + b);
-    v = block_sum<SGEO_VEC_THREADS / 64>(v, red);
-    if (threadIdx.x == 0) sc->rz = v;
-  }
-}
-
-// kp = assemble(Y); p.kp and max|p| -> alpha = rz / p.kp
-__global__ __launch_bounds__(SGEO_VEC_THREADS) void k_pcg_assemble_dot(
-    int64_t n1, const int32_t* __restrict__ rowptr,
-    const int32_t* __restrict__ slots, const double* __restrict__ Y,
-    const double* __restrict__ p, double* __restrict__ kp,
-    double* __restrict__ part, PcgScalars* __restrict__ sc, int seq) {
-  if (sc->done) return;
-  __shared__ double red[SGEO_VEC_THREADS / 64];
-  const int64_t base = (int64_t)blockIdx.x * SGEO_VEC_THREADS * SGEO_VEC_ITEMS;
-  double s = 0.0, m = 0.0;
-#pragma unroll
-  for (int k = 0; k < SGEO_VEC_ITEMS; k++) {
-    const int64_t g = base + threadIdx.x + (int64_t)k * SGEO_VEC_THREADS;
-    if (g < n1) {
-      const double v = (g == 0) ? 0.0 : gather_row(rowptr, slots, Y, g);
-      kp[g] = v;
-      const double pg = p[g];
-      s = s + pg * v;
-      m = fmax(m, fabs(pg));
-    }
-  }
-  const double bs = block_sum<SGEO_VEC_THREADS / 64>(s, red);
-  const double bm = block_max<SGEO_VEC_THREADS / 64>(m, red);
-  const int nb = gridDim.x;
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = bs;
-    part[nb + blockIdx.x] = bm;
-  }
-  if (last_block(&sc->ticket[2])) {
-    double vs = 0.0, vm = 0.0;
-    for (int b = threadIdx.x; b < nb; b += SGEO_VEC_THREADS) {
-      vs = vs + ld_coherent(part + b);
-      vm = fmax(vm, ld_coherent(part + nb + b));
-    }
-    vs = block_sum<SGEO_VEC_THREADS / 64>(vs, red);
-    vm = block_max<SGEO_VEC_THREADS / 64>(vm, red);
-    if (threadIdx.x == 0) {
-      sc->maxp = vm;
-      if (!seq) {  // seq: k_pcg_seqdot computes p.kp in reference order
-        sc->pkp = vs;
-        sc->alpha = sc->rz / vs;
-      }
-    }
-  }
-}
-
-// u += alpha p; max|u|; stop test; r -= alpha kp; z = dinv r; r.z -> beta
-__global__ __launch_bounds__(SGEO_VEC_THREADS) void k_pcg_update(
-    int64_t n1, const double* __restrict__ p, const double* __restrict__ kp,
-    const double* __restrict__ dinv, double* __restrict__ u,
-    double* __restrict__ r, double* __restrict__ z, double* __restrict__ part,
-    PcgScalars* __restrict__ sc, int seq) {
-  if (sc->done) return;
-  __shared__ double red[SGEO_VEC_THREADS / 64];
-  const double alpha = sc->alpha;
-  const int64_t base = (int64_t)blockIdx.x * SGEO_VEC_THREADS * SGEO_VEC_ITEMS;
-  double m = 0.0, s = 0.0;
-#pragma unroll
-  for (int k = 0; k < SGEO_VEC_ITEMS; k++) {
-    const int64_t g = base + threadIdx.x + (int64_t)k * SGEO_VEC_THREADS;
-    if (g < n1) {
-      const double ug = u[g] + alpha * p[g];
-      u[g] = ug;
-      m = fmax(m, fabs(ug));
-      const double rg = r[g] - alpha * kp[g];
-      const double zg = dinv[g] * rg;
-      r[g] = rg;
-      z[g] = zg;
-      s = s + rg * zg;
-    }
-  }
-  const double bm = block_max<SGEO_VEC_THREADS / 64>(m, red);
-  const double bs = block_sum<SGEO_VEC_THREADS / 64>(s, red);
-  const int nb = gridDim.x;
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = bm;
-    part[nb + blockIdx.x] = bs;
-  }
-  if (last_block(&sc->ticket[3])) {
-    double vm = 0.0, vs = 0.0;
-    for (int b = threadIdx.x; b < nb; b += SGEO_VEC_THREADS) {
-      vm = fmax(vm, ld_coherent(part + b));
-      vs = vs + ld_coherent(part + nb + b);
-    }
-    vm = block_max<SGEO_VEC_THREADS / 64>(vm, red);
-    vs = block_sum<SGEO_VEC_THREADS / 64>(vs, red);
-    if (threadIdx.x == 0) {
-      const int it = sc->iter + 1;
-      sc->iter = it;
-      sc->maxu = vm;
-      // src/solver.f90:65: stop after the u update, before the r update
-      if (fabs(sc->alpha) * sc->maxp / vm <= sc->tol) {
-        sc->done = 1;
-        sc->conv_iter = it;
-      } else if (it >= sc->maxiter) {
-        sc->done = 2;
-        sc->conv_iter = sc->maxiter + 1;
-      } else if (!seq) {  // seq: k_pcg_seqdot computes r.z in reference order
-        sc->beta = vs / sc->rz;
-        sc->rz = vs;
-      }
-    }
-  }
-}
-
-// --------------------------------------------------------------------------
-// reference-order dot products: s = ((0 + a0 b0) + a1 b1) + ..., exactly the
-// sequential dot_product of src/solver.f90:61-62,72.  One block: waves 1-3
-// stage rounded products of the next chunk in LDS while lane 0 of wave 0
-// adds the current chunk in index order.  O(n) latency-bound; meant for
-// bit-for-bit parity runs, the tree reductions above are the fast path.
-// --------------------------------------------------------------------------
-#define SEQ_CHUNK 1536
-__device__ double seq_dot(int64_t n1, const double* __restrict__ a,
-                          const double* __restrict__ b, double* buf /*2*CHUNK*/) {
-  const int t = threadIdx.x;
-  const int64_t nch = (n1 + SEQ_CHUNK - 1) / SEQ_CHUNK;
-  double s = 0.0;
-  // stage chunk 0 with all threads
-  for (int j = t; j < SEQ_CHUNK; j += blockDim.x)
-    buf[j] = (j < n1) ? a[j] * b[j] : 0.0;
-  __syncthreads();
-  for (int64_t c = 0; c < nch; c++) {
-    const double* cur = buf + (c & 1) * SEQ_CHUNK;
-    if (t < 64) {
-      if (t == 0) {
-        const int64_t cnt = (c == nch - 1) ? n1 - c * SEQ_CHUNK : SEQ_CHUNK;
-        for (int64_t j = 0; j < cnt; j++) s = s + cur[j];
-      }
-    } else if (c + 1 < nch) {
-      double* nxt = buf + ((c + 1) & 1) * SEQ_CHUNK;
-      const int64_t base = (c + 1) * SEQ_CHUNK;
-      for (int j = t - 64; j < SEQ_CHUNK; j += blockDim.x - 64) {
-        const int64_t g = base + j;
-        nxt[j] = (g < n1) ? a[g] * b[g] : 0.0;
-      }
-    }
-    __syncthreads();
-  }
-  return s;  // valid in thread 0
-}
-
-// which: 0 initial rz (rz = r.z); 1 alpha = rz / p.kp; 2 beta from r.z
-__global__ __launch_bounds__(256) void k_pcg_seqdot(
-    int64_t n1, const double* __restrict__ a, const double* __restrict__ b,
-    PcgScalars* __restrict__ sc, int which) {
-  __shared__ double buf[2 * SEQ_CHUNK];
-  if (which != 0 && sc->done) return;
-  const double s = seq_dot(n1, a, b, buf);
-  if (threadIdx.x == 0) {
-    if (which == 0) {
-      sc->rz = s;
-    } else if (which == 1) {
-      sc->pkp = s;
-      sc->alpha = sc->rz / s;
-    } else {
-      sc->beta = s / sc->rz;
-      sc->rz = s;
-    }
-  }
-}
-
-// p = z + beta p
-__global__ __launch_bounds__(SGEO_VEC_THREADS) void k_pcg_pupdate(
-    int64_t n1, const double* __restrict__ z, double* __restrict__ p,
-    const PcgScalars* __restrict__ sc) {
-  if (sc->done) return;
-  const double beta = sc->beta;
-  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n1;
-       g += (int64_t)gridDim.x * blockDim.x)
-    p[g] = z[g] + beta * p[g];
-}
-
-// --------------------------------------------------------------------------
-// plastic-iteration vector kernels (src/slope.F90:288-309)
-// --------------------------------------------------------------------------
-// load = extload + bodyload; load(0) = 0
-__global__ void k_load(int64_t n1, const double* __restrict__ ext,
-                       const double* __restrict__ body, double* __restrict__ load) {
-  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n1;
-       g += (int64_t)gridDim.x * blockDim.x)
-    load[g] = (g == 0) ? 0.0 : ext[g] + body[g];
-}
-
-// x(0)=0; uerr = max|x-oldx| / max|x|; oldx = x; nl_isconv = uerr <= nl_tol
-__global__ __launch_bounds__(SGEO_VEC_THREADS) void k_uerr(
-    int64_t n1, double* __restrict__ x, double* __restrict__ oldx,
-    double* __restrict__ part, PlasticScalars* __restrict__ ps,
-    double nl_tol) {
-  __shared__ double red[SGEO_VEC_THREADS / 64];
-  const int64_t base = (int64_t)blockIdx.x * SGEO_VEC_THREADS * SGEO_VEC_ITEMS;
-  double md = 0.0, mx = 0.0;
-#pragma unroll
-  for (int k = 0; k < SGEO_VEC_ITEMS; k++) {
-    const int64_t g = base + threadIdx.x + (int64_t)k * SGEO_VEC_THREADS;
-    if (g < n1) {
-      double xg = x[g];
-      if (g == 0) { xg = 0.0; x[0] = 0.0; }
-      md = fmax(md, fabs(xg - oldx[g]));
-      mx = fmax(mx, fabs(xg));
-      oldx[g] = xg;
-    }
-  }
-  const double bd = block_max<SGEO_VEC_THREADS / 64>(md, red);
-  const double bx = block_max<SGEO_VEC_THREADS / 64>(mx, red);
-  const int nb = gridDim.x;
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = bd;
-    part[nb + blockIdx.x] = bx;
-  }
-  if (last_block(&ps->ticket[0])) {
-    double vd = 0.0, vx = 0.0;
-    for (int b = threadIdx.x; b < nb; b += SGEO_VEC_THREADS) {
-      vd = fmax(vd, ld_coherent(part + b));
-      vx = fmax(vx, ld_coherent(part + nb + b));
-    }
-    vd = block_max<SGEO_VEC_THREADS / 64>(vd, red);
-    vx = block_max<SGEO_VEC_THREADS / 64>(vx, red);
-    if (threadIdx.x == 0) {
-      const double uerr = vd / vx;
-      ps->uerr = uerr;
-      ps->xmax = vx;
-      ps->nl_isconv = uerr <= nl_tol ? 1 : 0;
-      ps->fmax_bits = 0ull;  // fmax = zero (src/slope.F90:286)
-      ps->n_plastic = 0;
-    }
-  }
-}
-
-// --------------------------------------------------------------------------
 // element x GLL plastic body: src/slope.F90:312-383 (and the excavation
 // variant src/excavation.F90:546-616 when excav != 0)
 // --------------------------------------------------------------------------
@@ -901,81 +514,6 @@ void launch_round_f32(hipStream_t s, int64_t n, const double* a, float* b) {
                      dim3(SGEO_VEC_THREADS), 0, s, n, a, b);
 }
 
-void launch_ebe_elem_f64(hipStream_t s, int32_t nelmt, const double* K,
-                         const int32_t* gdof, const double* p, double* Y,
-                         const PcgScalars* sc, int skip_mode) {
-  if (nelmt <= 0) return;
-  hipLaunchKernelGGL(k_ebe_elem<double>, dim3(nblk(nelmt, 4)), dim3(256), 0, s,
-                     nelmt, K, gdof, p, Y, sc, skip_mode);
-}
-
-void launch_ebe_elem_f32k(hipStream_t s, int32_t nelmt, const float* K,
-                          const int32_t* gdof, const double* p, double* Y,
-                          const PcgScalars* sc, int skip_mode) {
-  if (nelmt <= 0) return;
-  hipLaunchKernelGGL(k_ebe_elem<float>, dim3(nblk(nelmt, 4)), dim3(256), 0, s,
-                     nelmt, K, gdof, p, Y, sc, skip_mode);
-}
-
-void launch_assemble(hipStream_t s, int64_t n1, const int32_t* rowptr,
-                     const int32_t* slots, const double* Y, double* out,
-                     const PcgScalars* sc, int skip_mode) {
-  hipLaunchKernelGGL(k_assemble, dim3(grid_stride_blocks(n1)),
-                     dim3(SGEO_VEC_THREADS), 0, s, n1, rowptr, slots, Y, out,
-                     sc, skip_mode);
-}
-
-void launch_assemble_acc(hipStream_t s, int64_t n1, const int32_t* rowptr,
-                         const int32_t* slots, const double* Y, double* out,
-                         int subtract) {
-  hipLaunchKernelGGL(k_assemble_acc, dim3(grid_stride_blocks(n1)),
-                     dim3(SGEO_VEC_THREADS), 0, s, n1, rowptr, slots, Y, out,
-                     subtract);
-}
-
-void launch_pcg_begin(hipStream_t s, int64_t n1, const double* u, double* part,
-                      PcgScalars* sc, double tol, int32_t maxiter) {
-  hipLaunchKernelGGL(k_pcg_begin, dim3(vec_reduce_blocks(n1)),
-                     dim3(SGEO_VEC_THREADS), 0, s, n1, u, part, sc, tol, maxiter);
-}
-
-void launch_pcg_init(hipStream_t s, int64_t n1, const double* f,
-                     const double* kp, const double* dinv, double* r, double* z,
-                     double* p, double* part, PcgScalars* sc) {
-  hipLaunchKernelGGL(k_pcg_init, dim3(vec_reduce_blocks(n1)),
-                     dim3(SGEO_VEC_THREADS), 0, s, n1, f, kp, dinv, r, z, p,
-                     part, sc);
-}
-
-void launch_pcg_assemble_dot(hipStream_t s, int64_t n1, const int32_t* rowptr,
-                             const int32_t* slots, const double* Y,
-                             const double* p, double* kp, double* part,
-                             PcgScalars* sc, int seq) {
-  hipLaunchKernelGGL(k_pcg_assemble_dot, dim3(vec_reduce_blocks(n1)),
-                     dim3(SGEO_VEC_THREADS), 0, s, n1, rowptr, slots, Y, p, kp,
-                     part, sc, seq);
-}
-
-void launch_pcg_seqdot(hipStream_t s, int64_t n1, const double* a,
-                       const double* b, PcgScalars* sc, int which) {
-  hipLaunchKernelGGL(k_pcg_seqdot, dim3(1), dim3(256), 0, s, n1, a, b, sc, which);
-}
-
-void launch_pcg_update(hipStream_t s, int64_t n1, const double* p,
-                       const double* kp, const double* dinv, double* u,
-                       double* r, double* z, double* part, PcgScalars* sc,
-                       int seq) {
-  hipLaunchKernelGGL(k_pcg_update, dim3(vec_reduce_blocks(n1)),
-                     dim3(SGEO_VEC_THREADS), 0, s, n1, p, kp, dinv, u, r, z,
-                     part, sc, seq);
-}
-
-void launch_pcg_pupdate(hipStream_t s, int64_t n1, const double* z, double* p,
-                        const PcgScalars* sc) {
-  hipLaunchKernelGGL(k_pcg_pupdate, dim3(grid_stride_blocks(n1)),
-                     dim3(SGEO_VEC_THREADS), 0, s, n1, z, p, sc);
-}
-
 void launch_invert_diag(hipStream_t s, int64_t n1, double* d) {
   hipLaunchKernelGGL(k_invert_diag, dim3(grid_stride_blocks(n1)),
                      dim3(SGEO_VEC_THREADS), 0, s, n1, d);
@@ -985,19 +523,6 @@ void launch_scale_by_mat(hipStream_t s, int32_t nelmt, const double* eld,
                          const int32_t* mat_id, const double* gam, double* Y) {
   hipLaunchKernelGGL(k_scale_by_mat, dim3(grid_stride_blocks((int64_t)nelmt * 81)),
                      dim3(SGEO_VEC_THREADS), 0, s, nelmt, eld, mat_id, gam, Y);
-}
-
-void launch_load(hipStream_t s, int64_t n1, const double* ext,
-                 const double* body, double* load) {
-  hipLaunchKernelGGL(k_load, dim3(grid_stride_blocks(n1)),
-                     dim3(SGEO_VEC_THREADS), 0, s, n1, ext, body, load);
-}
-
-void launch_uerr(hipStream_t s, int64_t n1, double* x, double* oldx,
-                 double* part, PlasticScalars* ps, double nl_tol) {
-  hipLaunchKernelGGL(k_uerr, dim3(vec_reduce_blocks(n1)),
-                     dim3(SGEO_VEC_THREADS), 0, s, n1, x, oldx, part, ps,
-                     nl_tol);
 }
 
 void launch_plastic_body(hipStream_t s, int32_t nel, const int32_t* elmt,

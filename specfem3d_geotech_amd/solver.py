@@ -42,6 +42,15 @@ class Context:
     def synchronize(self):
         check(L.lib().sgeo_synchronize(self.h), "sgeo_synchronize")
 
+    def profile(self, enable: int = -1):
+        """(launches, total_ms) of the timed element-product kernel since the
+        last reset; enable 1/0 then resets and switches timing on/off."""
+        n = C.c_int64()
+        ms = C.c_double()
+        check(L.lib().sgeo_profile(self.h, enable, C.byref(n), C.byref(ms)),
+              "sgeo_profile")
+        return n.value, ms.value
+
     def set_reduction(self, mode: int):
         """REDUCE_TREE (fast) or REDUCE_SEQUENTIAL (reference order)."""
         check(L.lib().sgeo_set_reduction(self.h, mode), "sgeo_set_reduction")

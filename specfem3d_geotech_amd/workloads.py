@@ -1,0 +1,55 @@
+"""Seeded synthetic workloads of the benchmark configurations.
+
+c0  1,152 elements (24x4x12), homogeneous 2H:1V slope, c=10 kPa, phi=20 deg,
+    psi=0, gamma=20 kN/m3, E=1e5 kPa, nu=0.3; base fixed, side rollers.
+c1  144,000 elements (120x20x60), same geometry, two layers split at
+    z=10 m (upper c=8 kPa, phi=25 deg; lower c=20 kPa, phi=30 deg), interior
+    nodes jittered by U[-0.02,0.02] x local spacing (seed 1).
+Control of both: SRF 1.00:0.05:2.00, cg_tol 1e-8, cg_maxiter 5000,
+nl_tol 5e-4, nl_maxiter 300.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Callable, Dict
+
+from .mesh import Material, slope_mesh
+from .spec import hex2spec
+
+SRF_SWEEP = [round(1.0 + 0.05 * i, 2) for i in range(21)]
+CONTROL = dict(cg_tol=1e-8, cg_maxiter=5000, nl_tol=5e-4, nl_maxiter=300)
+
+
+@dataclass
+class Workload:
+    name: str
+    description: str
+    make: Callable
+    control: Dict = field(default_factory=lambda: dict(CONTROL, srf=SRF_SWEEP))
+
+    def build(self):
+        mesh = self.make()
+        return mesh, hex2spec(mesh)
+
+
+def _c0():
+    return slope_mesh(24, 4, 12, materials=[
+        Material(gam=20.0, ym=1e5, nu=0.3, phi=20.0, coh=10.0, psi=0.0)])
+
+
+def _c1():
+    return slope_mesh(120, 20, 60, materials=[
+        Material(gam=20.0, ym=1e5, nu=0.3, phi=25.0, coh=8.0, psi=0.0),
+        Material(gam=20.0, ym=1e5, nu=0.3, phi=30.0, coh=20.0, psi=0.0)],
+        layer_z=[10.0], jitter=0.02, seed=1)
+
+
+WORKLOADS = {
+    "c0": Workload("c0", "slope 24x4x12 (1152 el), homogeneous, FP64", _c0),
+    "c1": Workload("c1", "slope 120x20x60 (144000 el, 3489480 eq), two layers, "
+                         "2% jitter seed 1", _c1),
+}
+
+
+def get(name: str) -> Workload:
+    return WORKLOADS[name]
