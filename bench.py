@@ -31,6 +31,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+# form -> (operator precision, storage, reduction order, bytes/entry, entries)
+FORMS = {
+    "fp64": ("f64", "packed", "tree", 8, 3328),
+    "fp64full": ("f64", "full", "tree", 8, 6561),
+    "fp64ref": ("f64", "full", "sequential", 8, 6561),
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -38,7 +45,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--mode", default="pcg", choices=["analysis", "pcg"])
-    ap.add_argument("--form", default="fp64", choices=["fp64", "fp64ref"])
+    ap.add_argument("--form", default="fp64", choices=list(FORMS))
     ap.add_argument("--workload", default="c1", choices=["c1", "c0"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-iters", type=int, default=30)
@@ -53,7 +60,8 @@ def main():
               "DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     from specfem3d_geotech_amd import workloads
-    from specfem3d_geotech_amd.solver import (K_F64, REDUCE_SEQUENTIAL, Context,
+    from specfem3d_geotech_amd.solver import (K_F32, K_F64, KFMT_FULL, KFMT_PACKED,
+                                              REDUCE_SEQUENTIAL, Context,
                                               DeviceMesh)
     from specfem3d_geotech_amd.slope import DeviceSlope
     from specfem3d_geotech_amd.plastic import dt_viscoplas, strength_reduction
@@ -63,17 +71,21 @@ def main():
     mesh, spec = wl.build()
     prep_s = time.time() - t0
     ctx = Context(0)
-    if a.form == "fp64ref":
+    kp, kf, red, bk, S = FORMS[a.form]
+    if red == "sequential":
         ctx.set_reduction(REDUCE_SEQUENTIAL)
+    copy_gbs = ctx.calibrate_copy(4 << 30, 10)
     gam, ym, nu, phi, coh, psi = mesh.material_arrays()
     dm = DeviceMesh(ctx, spec, len(gam))
     t0 = time.time()
     ext, _ = dm.build_stiffness(gam, ym, nu, kprec=K_F64)
+    dm.set_operator(K_F64 if kp == "f64" else K_F32,
+                    KFMT_PACKED if kf == "packed" else KFMT_FULL)
     kbuild_s = time.time() - t0
     E, N = spec.nelmt, spec.neq
-    # algorithmic bytes of one element-product launch: K (E*6561*8) plus the
+    # algorithmic bytes of one element-product launch: K (E*S*bk) plus the
     # element dof indices (E*81*4), the gathered p (E*81*8) and Y (E*81*8)
-    k_bytes = E * 6561 * 8 + E * 81 * 4 + 2 * E * 81 * 8
+    k_bytes = E * S * bk + E * 81 * 4 + 2 * E * 81 * 8
     result = dict(metric="FOS analyses/hour + p50/p99 latency; EBE-PCG HBM GB/s "
                          "vs peak, 1xMI355X")
 
@@ -106,9 +118,11 @@ def main():
         n_gpus=1, higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype="f64", data="synthetic (seeded slope mesh, reference input format)",
         config=dict(workload=wl.description, nelmt=E, neq=N, form=a.form,
-                    mode=a.mode, k_storage="full 81x81 FP64 (S=6561)",
+                    mode=a.mode, k_storage=f"{kf} {kp} (S={S})", reduction=red,
+                    copy_kernel_gbs=round(copy_gbs, 1),
                     host_prep_s=round(prep_s, 2), k_build_s=round(kbuild_s, 4)),
-        roofline=dict(kernel="k_ebe_elem<double>", bound="hbm",
+        roofline=dict(kernel=("k_ebe_packed" if kf == "packed" else "k_ebe_elem")
+                      + ("<double>" if kp == "f64" else "<float>"), bound="hbm",
                       achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                       frac=round(achieved / HBM_PEAK_GBS, 4),
                       avg_launch_ms=round(avg_ms, 4), launches=nlaunch,

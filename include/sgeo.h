@@ -112,6 +112,19 @@ int sgeo_mesh_build_stiffness(sgeo_mesh *mesh, const double *gam,
 /* Uploads a host storekm (reference layout, FP64) and inverted dprecon. */
 int sgeo_mesh_upload_stiffness(sgeo_mesh *mesh, const double *storekm,
                                const double *dprecon, int kprec);
+/* Storage format of the operator the PCG streams. */
+#define SGEO_KFMT_FULL 0    /* 81x81 per element, reference layout */
+#define SGEO_KFMT_PACKED 1  /* upper triangle, 3328 entries per element */
+/* Selects the operator used by the PCG of this mesh: precision (SGEO_K_F64 /
+ * SGEO_K_F32) and format.  Derived storages are built on the device from
+ * the canonical FP64 matrices (FP32 by rounding; packed keeps the upper
+ * triangle k(r,c), r <= c, which the reference's matrices match to the last
+ * bit only).  The full FP64 operator reproduces the reference bit for bit;
+ * the others trade that for fewer bytes per iteration. */
+int sgeo_mesh_set_operator(sgeo_mesh *mesh, int kprec, int kfmt);
+/* Achieved HBM bandwidth of a plain device copy of `bytes` (read + write,
+ * HIP events on the context's stream), for calibration. */
+int sgeo_calibrate_copy(sgeo_ctx *ctx, size_t bytes, int reps, double *gbs);
 /* Copies the device storekm back (FP64 meshes only). */
 int sgeo_mesh_download_stiffness(sgeo_mesh *mesh, double *storekm);
 /* kp = K p on the resident operator (host vectors, neq+1). */

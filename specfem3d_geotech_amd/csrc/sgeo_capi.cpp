@@ -104,8 +104,12 @@ struct sgeo_mesh {
   double* d_corners = nullptr;
   int32_t* d_mat = nullptr;
   int kprec = -1;  // -1: no stiffness yet
-  double* d_K64 = nullptr;
-  float* d_K32 = nullptr;
+  int kfmt = SGEO_KFMT_FULL;
+  double* d_K64 = nullptr;  // canonical: FP64, reference layout
+  float* d_K32 = nullptr;   // derived operators (see refresh_operator)
+  double* d_KP64 = nullptr;
+  float* d_KP32 = nullptr;
+  int k_ver = 0, k32_ver = -1, kp64_ver = -1, kp32_ver = -1;
   double* d_dinv = nullptr;
   // work vectors
   double* d_Y = nullptr;
@@ -315,7 +319,8 @@ extern "C" int sgeo_mesh_destroy(sgeo_mesh* m) {
   hipSetDevice(m->ctx->device);
   hipStreamSynchronize(m->ctx->stream);
   void* ptrs[] = {m->d_gdof, m->d_rowptr, m->d_slots, m->d_corners, m->d_mat,
-                  m->d_K64,  m->d_K32,    m->d_dinv,  m->d_Y,       m->d_r,
+                  m->d_K64,  m->d_K32,    m->d_KP64,  m->d_KP32,
+                  m->d_dinv,  m->d_Y,       m->d_r,
                   m->d_z,    m->d_p,      m->d_kp};
   for (void* p : ptrs)
     if (p) hipFree(p);
@@ -334,8 +339,49 @@ static int set_k_storage(sgeo_mesh* m, int kprec) {
     return fail(SGEO_EARG, "kprec %d", kprec);
   const size_t nk = (size_t)m->nelmt * 6561;
   if (!m->d_K64) TRY(dalloc(&m->d_K64, nk, &m->bytes));
-  if (kprec == SGEO_K_F32 && !m->d_K32) TRY(dalloc(&m->d_K32, nk, &m->bytes));
   m->kprec = kprec;
+  return 0;
+}
+
+// Derived storage of the selected operator, rebuilt from the canonical FP64
+// full K whenever K changed (k_ver).
+static int refresh_operator(sgeo_mesh* m) {
+  hipStream_t s = m->ctx->stream;
+  const size_t nk = (size_t)m->nelmt * 6561;
+  const size_t np = (size_t)m->nelmt * SGEO_KPACK;
+  if (m->kfmt == SGEO_KFMT_FULL && m->kprec == SGEO_K_F32) {
+    if (!m->d_K32) TRY(dalloc(&m->d_K32, nk, &m->bytes));
+    if (m->k32_ver != m->k_ver) {
+      launch_round_f32(s, (int64_t)nk, m->d_K64, m->d_K32);
+      m->k32_ver = m->k_ver;
+    }
+  } else if (m->kfmt == SGEO_KFMT_PACKED && m->kprec == SGEO_K_F64) {
+    if (!m->d_KP64) TRY(dalloc(&m->d_KP64, np, &m->bytes));
+    if (m->kp64_ver != m->k_ver) {
+      launch_pack_upper_f64(s, m->nelmt, m->d_K64, m->d_KP64);
+      m->kp64_ver = m->k_ver;
+    }
+  } else if (m->kfmt == SGEO_KFMT_PACKED && m->kprec == SGEO_K_F32) {
+    if (!m->d_KP32) TRY(dalloc(&m->d_KP32, np, &m->bytes));
+    if (m->kp32_ver != m->k_ver) {
+      launch_pack_upper_f32(s, m->nelmt, m->d_K64, m->d_KP32);
+      m->kp32_ver = m->k_ver;
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int sgeo_mesh_set_operator(sgeo_mesh* m, int kprec, int kfmt) {
+  if (!m || m->kprec < 0) return fail(SGEO_EARG, "mesh has no stiffness");
+  if ((kprec != SGEO_K_F64 && kprec != SGEO_K_F32) ||
+      (kfmt != SGEO_KFMT_FULL && kfmt != SGEO_KFMT_PACKED))
+    return fail(SGEO_EARG, "operator kprec=%d kfmt=%d", kprec, kfmt);
+  HIPCHK(hipSetDevice(m->ctx->device));
+  m->kprec = kprec;
+  m->kfmt = kfmt;
+  TRY(refresh_operator(m));
+  HIPCHK(hipStreamSynchronize(m->ctx->stream));
   return 0;
 }
 
@@ -374,8 +420,8 @@ extern "C" int sgeo_mesh_build_stiffness(sgeo_mesh* m, const double* gam,
     launch_assemble_acc(s, n1, m->d_rowptr, m->d_slots, d_kdiag, d_ext, 1, nullptr);
     HIPCHK(hipMemcpyAsync(extload, d_ext, n1 * 8, hipMemcpyDeviceToHost, s));
   }
-  if (kprec == SGEO_K_F32)
-    launch_round_f32(s, (int64_t)m->nelmt * 6561, m->d_K64, m->d_K32);
+  m->k_ver++;
+  TRY(refresh_operator(m));
   if (dprecon_out)
     HIPCHK(hipMemcpyAsync(dprecon_out, m->d_dinv, n1 * 8, hipMemcpyDeviceToHost, s));
   HIPCHK(hipGetLastError());
@@ -396,8 +442,8 @@ extern "C" int sgeo_mesh_upload_stiffness(sgeo_mesh* m, const double* k,
   HIPCHK(hipMemcpyAsync(m->d_K64, k, nk * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(m->d_dinv, dprecon, ((size_t)m->neq + 1) * 8,
                         hipMemcpyHostToDevice, s));
-  if (kprec == SGEO_K_F32) launch_round_f32(s, (int64_t)nk, m->d_K64, m->d_K32);
-  HIPCHK(hipGetLastError());
+  m->k_ver++;
+  TRY(refresh_operator(m));
   HIPCHK(hipStreamSynchronize(s));
   return 0;
 }
@@ -413,12 +459,45 @@ extern "C" int sgeo_mesh_download_stiffness(sgeo_mesh* m, double* k) {
 
 static void ebe_elem(sgeo_mesh* m, const double* p, double* Y, int skip_mode) {
   hipStream_t s = m->ctx->stream;
-  if (m->kprec == SGEO_K_F32)
-    launch_ebe_elem_f32k(s, m->nelmt, m->d_K32, m->d_gdof, p, Y, m->ctx->d_sc,
-                         skip_mode);
-  else
-    launch_ebe_elem_f64(s, m->nelmt, m->d_K64, m->d_gdof, p, Y, m->ctx->d_sc,
-                        skip_mode);
+  const PcgScalars* sc = m->ctx->d_sc;
+  if (m->kfmt == SGEO_KFMT_PACKED) {
+    if (m->kprec == SGEO_K_F32)
+      launch_ebe_packed_f32(s, m->nelmt, m->d_KP32, m->d_gdof, p, Y, sc, skip_mode);
+    else
+      launch_ebe_packed_f64(s, m->nelmt, m->d_KP64, m->d_gdof, p, Y, sc, skip_mode);
+  } else if (m->kprec == SGEO_K_F32) {
+    launch_ebe_elem_f32k(s, m->nelmt, m->d_K32, m->d_gdof, p, Y, sc, skip_mode);
+  } else {
+    launch_ebe_elem_f64(s, m->nelmt, m->d_K64, m->d_gdof, p, Y, sc, skip_mode);
+  }
+}
+
+extern "C" int sgeo_calibrate_copy(sgeo_ctx* c, size_t bytes, int reps,
+                                   double* gbs) {
+  if (!c || !gbs || bytes < 1024 || reps < 1)
+    return fail(SGEO_EARG, "sgeo_calibrate_copy: bad arguments");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t n2 = bytes / 16;
+  double *a = nullptr, *b = nullptr;
+  TRY(dalloc(&a, n2 * 2));
+  TRY(dalloc(&b, n2 * 2));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipMemsetAsync(a, 0, n2 * 16, c->stream));
+  launch_copy(c->stream, (int64_t)n2, a, b);  // warm-up
+  HIPCHK(hipEventRecord(e0, c->stream));
+  for (int i = 0; i < reps; i++) launch_copy(c->stream, (int64_t)n2, a, b);
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *gbs = 2.0 * (double)n2 * 16.0 * reps / (ms * 1e-3) / 1e9;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  return 0;
 }
 
 extern "C" int sgeo_mesh_matvec(sgeo_mesh* m, const double* p, double* kp) {

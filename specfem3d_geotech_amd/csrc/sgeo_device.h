@@ -7,6 +7,8 @@
 #define SGEO_NEDOF 81
 #define SGEO_NST 6
 #define SGEO_KSIZE (SGEO_NEDOF * SGEO_NEDOF)
+// packed upper triangle (3321 entries) padded to a multiple of 64
+#define SGEO_KPACK 3328
 
 // Threads per block of the vector (per-equation) kernels.  The grid of every
 // reduction is a fixed function of the vector length, so partial sums are

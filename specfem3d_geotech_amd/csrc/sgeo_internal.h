@@ -36,6 +36,17 @@ void launch_ebe_elem_f64(hipStream_t s, int32_t nelmt, const double* K,
 void launch_ebe_elem_f32k(hipStream_t s, int32_t nelmt, const float* K,
                           const int32_t* gdof, const double* p, double* Y,
                           const PcgScalars* sc, int skip_mode);
+void launch_ebe_packed_f64(hipStream_t s, int32_t nelmt, const double* KP,
+                           const int32_t* gdof, const double* p, double* Y,
+                           const PcgScalars* sc, int skip_mode);
+void launch_ebe_packed_f32(hipStream_t s, int32_t nelmt, const float* KP,
+                           const int32_t* gdof, const double* p, double* Y,
+                           const PcgScalars* sc, int skip_mode);
+void launch_pack_upper_f64(hipStream_t s, int32_t nelmt, const double* K,
+                           double* KP);
+void launch_pack_upper_f32(hipStream_t s, int32_t nelmt, const double* K,
+                           float* KP);
+void launch_copy(hipStream_t s, int64_t n_double2, const double* a, double* b);
 void launch_assemble(hipStream_t s, int64_t n1, const int32_t* rowptr,
                      const int32_t* slots, const double* Y, double* out,
                      const PcgScalars* sc, int skip_mode);

@@ -14,6 +14,8 @@
 // operation as the reference does.
 #pragma clang fp contract(off)
 
+#include <type_traits>
+
 #include "sgeo_device.h"
 #include "sgeo_internal.h"
 
@@ -59,6 +61,81 @@ __global__ __launch_bounds__(256) void k_ebe_elem(
 #pragma unroll 27
   for (int c = 0; c < 81; c++) a = a + (double)ke[c * 81] * pl[c];
   Y[e * 81 + r] = a;
+}
+
+// --------------------------------------------------------------------------
+// EBE product on symmetric-packed storage (the fast path)
+// --------------------------------------------------------------------------
+// K_e keeps its upper triangle U(r,c), r <= c, column by column:
+// U(r,c) at c(c+1)/2 + r, 3321 entries padded to SGEO_KPACK = 3328 so each
+// element starts 16-byte aligned.  One 128-thread block per element: the
+// block stages the packed matrix in LDS with 16-byte loads (each stored
+// entry crosses HBM once), then thread r < 81 forms
+// y_r = sum_c K(r,c) p_c, c ascending, with K(r,c) = U(min(r,c), max(r,c)).
+// The reference's K_e is symmetric only to the last bit, so this path
+// agrees with the full-storage product to rounding, not bit for bit.
+template <typename KT>
+__global__ __launch_bounds__(128) void k_ebe_packed(
+    int32_t nelmt, const KT* __restrict__ KP, const int32_t* __restrict__ gdof,
+    const double* __restrict__ p, double* __restrict__ Y,
+    const PcgScalars* __restrict__ sc, int skip_mode) {
+  if (skip_mode == 1 && sc->done) return;
+  if (skip_mode == 2 && !sc->u_nonzero) return;
+  constexpr int PER = 16 / sizeof(KT);
+  constexpr int NV = SGEO_KPACK / PER;
+  __shared__ __align__(16) KT u[SGEO_KPACK];
+  __shared__ double pe[81];
+  const int64_t e = blockIdx.x;
+  const int t = threadIdx.x;
+  typedef double v2d __attribute__((ext_vector_type(2)));
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  using V = typename std::conditional<sizeof(KT) == 8, v2d, v4f>::type;
+  const V* src = reinterpret_cast<const V*>(KP + e * SGEO_KPACK);
+  V* dst = reinterpret_cast<V*>(u);
+#pragma unroll 4
+  for (int q = t; q < NV; q += 128) dst[q] = __builtin_nontemporal_load(src + q);
+  if (t < 81) pe[t] = p[gdof[e * 81 + t]];
+  __syncthreads();
+  if (t >= 81) return;
+  const int r = t;
+  const int rbase = r * (r + 1) / 2;
+  double a = 0.0;
+#pragma unroll 9
+  for (int c = 0; c < 81; c++) {
+    const int idx = (r <= c) ? (c * (c + 1) / 2 + r) : (rbase + c);
+    a = a + (double)u[idx] * pe[c];
+  }
+  Y[e * 81 + r] = a;
+}
+
+// full (reference layout) -> packed upper triangle, zero padding
+template <typename KT>
+__global__ void k_pack_upper(int32_t nelmt, const double* __restrict__ K,
+                             KT* __restrict__ KP) {
+  const int64_t n = (int64_t)nelmt * SGEO_KPACK;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = q / SGEO_KPACK;
+    const int i = (int)(q - e * SGEO_KPACK);
+    KT v = (KT)0;
+    if (i < 3321) {
+      // column c with c(c+1)/2 <= i < (c+1)(c+2)/2, row r = i - c(c+1)/2
+      int c = (int)((sqrt(8.0 * i + 1.0) - 1.0) * 0.5);
+      while (c * (c + 1) / 2 > i) c--;
+      while ((c + 1) * (c + 2) / 2 <= i) c++;
+      const int r = i - c * (c + 1) / 2;
+      v = (KT)K[e * 6561 + r + 81 * c];
+    }
+    KP[q] = v;
+  }
+}
+
+// plain copy of n doubles (bandwidth calibration)
+__global__ __launch_bounds__(256) void k_copy(int64_t n, const double2* __restrict__ a,
+                                              double2* __restrict__ b) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x)
+    b[q] = a[q];
 }
 
 // --------------------------------------------------------------------------
@@ -412,6 +489,41 @@ void launch_ebe_elem_f32k(hipStream_t s, int32_t nelmt, const float* K,
   if (nelmt <= 0) return;
   hipLaunchKernelGGL(k_ebe_elem<float>, dim3(nblk(nelmt, 3)), dim3(256), 0, s,
                      nelmt, K, gdof, p, Y, sc, skip_mode);
+}
+
+void launch_ebe_packed_f64(hipStream_t s, int32_t nelmt, const double* KP,
+                           const int32_t* gdof, const double* p, double* Y,
+                           const PcgScalars* sc, int skip_mode) {
+  if (nelmt <= 0) return;
+  hipLaunchKernelGGL(k_ebe_packed<double>, dim3(nelmt), dim3(128), 0, s, nelmt,
+                     KP, gdof, p, Y, sc, skip_mode);
+}
+
+void launch_ebe_packed_f32(hipStream_t s, int32_t nelmt, const float* KP,
+                           const int32_t* gdof, const double* p, double* Y,
+                           const PcgScalars* sc, int skip_mode) {
+  if (nelmt <= 0) return;
+  hipLaunchKernelGGL(k_ebe_packed<float>, dim3(nelmt), dim3(128), 0, s, nelmt,
+                     KP, gdof, p, Y, sc, skip_mode);
+}
+
+void launch_pack_upper_f64(hipStream_t s, int32_t nelmt, const double* K,
+                           double* KP) {
+  hipLaunchKernelGGL(k_pack_upper<double>,
+                     dim3(grid_stride_blocks((int64_t)nelmt * SGEO_KPACK)),
+                     dim3(SGEO_VEC_THREADS), 0, s, nelmt, K, KP);
+}
+
+void launch_pack_upper_f32(hipStream_t s, int32_t nelmt, const double* K,
+                           float* KP) {
+  hipLaunchKernelGGL(k_pack_upper<float>,
+                     dim3(grid_stride_blocks((int64_t)nelmt * SGEO_KPACK)),
+                     dim3(SGEO_VEC_THREADS), 0, s, nelmt, K, KP);
+}
+
+void launch_copy(hipStream_t s, int64_t n_double2, const double* a, double* b) {
+  hipLaunchKernelGGL(k_copy, dim3(grid_stride_blocks(n_double2)),
+                     dim3(256), 0, s, n_double2, (const double2*)a, (double2*)b);
 }
 
 void launch_assemble(hipStream_t s, int64_t n1, const int32_t* rowptr,

@@ -20,6 +20,8 @@ K_F64 = 0
 K_F32 = 1
 REDUCE_TREE = 0
 REDUCE_SEQUENTIAL = 1
+KFMT_FULL = 0
+KFMT_PACKED = 1
 
 
 class Context:
@@ -50,6 +52,13 @@ class Context:
         check(L.lib().sgeo_profile(self.h, enable, C.byref(n), C.byref(ms)),
               "sgeo_profile")
         return n.value, ms.value
+
+    def calibrate_copy(self, nbytes: int = 4 << 30, reps: int = 10) -> float:
+        """GB/s (read + write) of a plain device copy of `nbytes`."""
+        g = C.c_double()
+        check(L.lib().sgeo_calibrate_copy(self.h, nbytes, reps, C.byref(g)),
+              "sgeo_calibrate_copy")
+        return g.value
 
     def set_reduction(self, mode: int):
         """REDUCE_TREE (fast) or REDUCE_SEQUENTIAL (reference order)."""
@@ -132,6 +141,12 @@ class DeviceMesh:
         check(L.lib().sgeo_mesh_upload_stiffness(self.h, ptr(f64(storekm)),
                                                  ptr(f64(dinv)), kprec),
               "sgeo_mesh_upload_stiffness")
+        self.kprec = kprec
+
+    def set_operator(self, kprec: int = K_F64, kfmt: int = KFMT_FULL):
+        """Operator streamed by the PCG: precision and storage format."""
+        check(L.lib().sgeo_mesh_set_operator(self.h, kprec, kfmt),
+              "sgeo_mesh_set_operator")
         self.kprec = kprec
 
     def download_stiffness(self) -> np.ndarray:

@@ -219,3 +219,26 @@ def test_slope_sweep_matches_oracle_displacement(ctx):
     e = rel_l2(res.x, o["x"])
     print("final x relative L2 vs oracle:", e)
     assert e < 1e-10
+
+
+def test_packed_operator_matches_full(ctx):
+    """Symmetric-packed FP64 operator: the reference's K_e is symmetric to
+    the last bit, so the packed product agrees with the full one to a few
+    ulps of the largest term."""
+    from specfem3d_geotech_amd.solver import KFMT_PACKED
+    z = golden("slope_8x4x6_layered")
+    mesh = mesh_from_golden(z)
+    spec = hex2spec(mesh)
+    gam, ym, nu, *_ = mesh.material_arrays()
+    km, _, _ = O.stiffness_bodyload(spec, gam, nu, ym)
+    dm = DeviceMesh(ctx, spec, len(gam))
+    dm.build_stiffness(gam, ym, nu)
+    dm.set_operator(K_F64, KFMT_PACKED)
+    p = np.random.default_rng(1).normal(size=spec.neq + 1)
+    p[0] = 0.0
+    kp = dm.matvec(p)
+    ref = O.ebe_matvec(spec.neq, km, spec.gdof_elmt, p)
+    e = rel_l2(kp, ref)
+    print("packed vs full relative L2:", e)
+    assert e < 1e-14
+    dm.close()
