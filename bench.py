@@ -110,7 +110,28 @@ def main():
                       ms_per_step=ms_step, steps=steps, warmup=warm)
         st.close()
     else:
-        raise SystemExit("analysis mode: see --mode pcg for now")
+        from specfem3d_geotech_amd.slope import slope3d
+        steps = a.steps or 1
+        ctl = dict(wl.control)
+        dm.close()
+        ctx.profile(1)
+        t0 = time.perf_counter()
+        analyses = []
+        for k in range(steps):
+            res = slope3d(ctx, mesh, spec=spec, kprec=K_F64 if kp == "f64" else K_F32,
+                          kfmt=KFMT_PACKED if kf == "packed" else KFMT_FULL,
+                          progress=lambda r: print("  row", json.dumps(r), flush=True),
+                          **ctl)
+            analyses.append(dict(bracket=res.fos_bracket, rows=res.rows,
+                                 n_pcg=len(res.cg_log), cg_total=int(sum(res.cg_log))))
+        dt = time.perf_counter() - t0
+        nlaunch, kms = ctx.profile(0)
+        cg_total = sum(x["cg_total"] for x in analyses)
+        result.update(value=steps * 3600.0 / dt, unit="FOS analyses/hour",
+                      ms_per_step=dt / steps * 1e3, steps=steps, warmup=0,
+                      analyses=analyses, dof_iterations_per_s=N * cg_total / dt,
+                      seconds_per_analysis=dt / steps)
+        dm = DeviceMesh(ctx, spec, len(gam))  # closed below
 
     avg_ms = kms / max(nlaunch, 1)
     achieved = k_bytes / (avg_ms * 1e-3) / 1e9

@@ -166,6 +166,43 @@ int sgeo_slope_run_srf(sgeo_slope *slope, const int32_t *gdof_node,
 int sgeo_slope_download(sgeo_slope *slope, double *x, double *bodyload,
                         double *evpt, double *stress_elmt, uint8_t *flags);
 
+/* ---- multi-trial mixed-precision engine ---------------------------------
+ * Up to 64 (a power of two) strength-reduction trials on one resident mesh,
+ * sharing its elastic stiffness.  Each PCG call of a trial is an iterative
+ * refinement: FP64 residual f - K x on the packed FP64 operator, inner
+ * Jacobi-PCG in FP32 (packed FP32 operator, FP32 vectors, FP64 dot
+ * products) to the relative update inner_tol, x += d, until
+ * max|d|/max|x| <= cg_tol -- the reference's relative-update criterion
+ * (src/solver.f90:65) applied to the refinement correction.  cg_iter counts
+ * the inner iterations.  The yield/return-mapping step stays FP64.  Each
+ * column's arithmetic is independent of the others and of ncol, so a trial
+ * gives the same bits alone and in any batch.  (No reference counterpart:
+ * the reference runs one trial at a time.) */
+int sgeo_batch_create(sgeo_mesh *mesh, int32_t ncol, const double *extload,
+                      sgeo_batch **batch);
+int sgeo_batch_destroy(sgeo_batch *batch);
+int sgeo_batch_bytes(sgeo_batch *batch, size_t *bytes);
+/* Starts a trial in column col (src/slope.F90:248-249, 281-282) with the
+ * factored materials of its SRF and dt.  x0 (may be NULL) seeds x; when
+ * given it is taken as the converged result of the trial's first PCG call
+ * (the elastic solve, identical for every trial of a mesh), reported with
+ * x0_cg_iter iterations. */
+int sgeo_batch_begin(sgeo_batch *batch, int32_t col, const double *ym,
+                     const double *nuf, const double *phif, const double *cohf,
+                     const double *psif, double dt, const double *x0,
+                     int32_t x0_cg_iter);
+int sgeo_batch_release(sgeo_batch *batch, int32_t col);
+/* One plastic iteration of every active column (src/slope.F90:286-391);
+ * per-column arrays of length ncol (entries of inactive columns untouched). */
+int sgeo_batch_plastic_iter(sgeo_batch *batch, const int32_t *is_last,
+                            double cg_tol, int32_t cg_maxiter, double inner_tol,
+                            double nl_tol, int32_t *cg_iter, int32_t *errcode,
+                            double *uerr, int32_t *nl_isconv, double *fmax);
+int sgeo_batch_download(sgeo_batch *batch, int32_t col, double *x,
+                        uint8_t *flags);
+int sgeo_batch_stats(sgeo_batch *batch, int64_t *inner_iters,
+                     int64_t *outer_iters, int64_t *steps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,17 @@ SIGNATURES = {
                                      C.c_double, C.c_int32, i_p, i_p, d_p, d_p,
                                      d_p, i_p]),
     "sgeo_slope_download": (C.c_int, [vp, d_p, d_p, d_p, d_p, u8_p]),
+    "sgeo_batch_create": (C.c_int, [vp, C.c_int32, d_p, C.POINTER(vp)]),
+    "sgeo_batch_destroy": (C.c_int, [vp]),
+    "sgeo_batch_bytes": (C.c_int, [vp, C.POINTER(C.c_size_t)]),
+    "sgeo_batch_begin": (C.c_int, [vp, C.c_int32, d_p, d_p, d_p, d_p, d_p, C.c_double,
+                                   d_p, C.c_int32]),
+    "sgeo_batch_release": (C.c_int, [vp, C.c_int32]),
+    "sgeo_batch_plastic_iter": (C.c_int, [vp, i_p, C.c_double, C.c_int32, C.c_double,
+                                          C.c_double, i_p, i_p, d_p, i_p, d_p]),
+    "sgeo_batch_download": (C.c_int, [vp, C.c_int32, d_p, u8_p]),
+    "sgeo_batch_stats": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int64)]),
 }
 
 

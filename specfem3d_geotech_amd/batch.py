@@ -1,0 +1,295 @@
+"""Multi-trial FOS sweeps on one resident mesh (the batched form).
+
+The viscoplastic method keeps the elastic stiffness fixed
+(src/slope.F90:288), and every strength-reduction trial restarts from zero
+state (src/slope.F90:248-249, 281-282), so trials of one mesh are
+independent: here up to 64 of them share each CG iteration's pass over K
+(sgeo_batch, mixed precision with FP64 refinement).
+
+A *FOS analysis* is the reference's sweep for one set of strengths: SRFs in
+list order until the first SRF whose plastic loop reaches nl_maxiter
+(src/slope.F90:441); its bracket is (last stable SRF, first failing SRF).
+The scheduler keeps every column busy: each analysis walks its SRF list
+(optionally several SRFs of one analysis speculatively at once), and a
+column freed by a finished analysis takes the next waiting analysis.  The
+first PCG call of every trial solves K x = extload from x = 0 -- the same
+system for every trial of the mesh -- so it is solved once and reused.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, f64, i32, ptr
+from .mesh import HexMesh
+from .plastic import dt_viscoplas, strength_reduction
+from .slope import SlopeResult, uxmax_umax
+from .solver import K_F64, Context, DeviceMesh
+from .spec import SpecMesh, hex2spec
+
+
+class BatchEngine:
+    """sgeo_batch: `ncol` trial columns on one DeviceMesh."""
+
+    def __init__(self, dmesh: DeviceMesh, extload: np.ndarray, ncol: int):
+        self.dmesh = dmesh
+        self.ncol = ncol
+        self._ext = f64(extload)
+        h = C.c_void_p()
+        check(L.lib().sgeo_batch_create(dmesh.h, ncol, ptr(self._ext), C.byref(h)),
+              "sgeo_batch_create")
+        self.h = h
+
+    def begin(self, col, ym, nuf, phif, cohf, psif, dt, x0=None, x0_cg=0):
+        x0a = f64(x0) if x0 is not None else None
+        check(L.lib().sgeo_batch_begin(self.h, col, ptr(f64(ym)), ptr(f64(nuf)),
+                                       ptr(f64(phif)), ptr(f64(cohf)), ptr(f64(psif)),
+                                       dt, ptr(x0a), int(x0_cg)),
+              "sgeo_batch_begin")
+
+    def release(self, col):
+        check(L.lib().sgeo_batch_release(self.h, col), "sgeo_batch_release")
+
+    def plastic_iter(self, is_last, cg_tol, cg_maxiter, inner_tol, nl_tol):
+        n = self.ncol
+        il = i32(is_last)
+        it = np.zeros(n, np.int32); ec = np.zeros(n, np.int32)
+        ue = np.zeros(n); cv = np.zeros(n, np.int32); fm = np.zeros(n)
+        check(L.lib().sgeo_batch_plastic_iter(
+            self.h, ptr(il, C.c_int32), cg_tol, cg_maxiter, inner_tol, nl_tol,
+            ptr(it, C.c_int32), ptr(ec, C.c_int32), ptr(ue), ptr(cv, C.c_int32),
+            ptr(fm)), "sgeo_batch_plastic_iter")
+        return dict(cg_iter=it, errcode=ec, uerr=ue, nl_isconv=cv, fmax=fm)
+
+    def download(self, col):
+        sp = self.dmesh.spec
+        x = np.zeros(sp.neq + 1)
+        flags = np.zeros((sp.nelmt, 27), np.uint8)
+        check(L.lib().sgeo_batch_download(self.h, col, ptr(x), ptr(flags, C.c_uint8)),
+              "sgeo_batch_download")
+        return x, flags
+
+    def stats(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        check(L.lib().sgeo_batch_stats(self.h, C.byref(a), C.byref(b), C.byref(c)),
+              "sgeo_batch_stats")
+        return dict(inner_iterations=a.value, refinements=b.value, batched_steps=c.value)
+
+    def nbytes(self):
+        b = C.c_size_t()
+        check(L.lib().sgeo_batch_bytes(self.h, C.byref(b)), "sgeo_batch_bytes")
+        return b.value
+
+    def close(self):
+        if self.h:
+            L.lib().sgeo_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class Analysis:
+    """One FOS analysis: a strength set and its SRF list."""
+    coh: np.ndarray
+    phi: np.ndarray
+    psi: np.ndarray
+    srf: Sequence[float]
+    tag: object = None
+    rows: Dict[int, dict] = field(default_factory=dict)   # srf index -> row
+    bracket: Optional[tuple] = None
+    done: bool = False
+    next_srf: int = 0
+    fail_idx: Optional[int] = None
+    t_arrival: Optional[float] = None
+    t_start: Optional[float] = None
+    t_end: Optional[float] = None
+    flags_last: Optional[np.ndarray] = None
+    x_last: Optional[np.ndarray] = None
+
+
+@dataclass
+class _Col:
+    analysis: Optional[Analysis] = None
+    srf_idx: int = -1
+    nl_iter: int = 0
+    cg_tot: int = 0
+    fmax: float = 0.0
+
+
+def run_analyses(engine: BatchEngine, analyses: List[Analysis], *, ym, nu,
+                 cg_tol: float, cg_maxiter: int, nl_tol: float, nl_maxiter: int,
+                 inner_tol: float = 1e-5, srf_parallel: int = 1,
+                 keep_state: bool = False, arrivals: Optional[List[float]] = None,
+                 clock=time.perf_counter, on_done=None):
+    """Runs every analysis to its FOS bracket with all columns kept busy.
+
+    srf_parallel > 1 lets one analysis run that many consecutive SRFs at once
+    (the bracket is still the first failing SRF in list order; trials above a
+    known failure are abandoned).  `arrivals` (seconds from start, one per
+    analysis) delays admission for request streams.  Returns engine stats."""
+    spec = engine.dmesh.spec
+    ncol = engine.ncol
+    cols = [_Col() for _ in range(ncol)]
+    queue = list(range(len(analyses)))
+    t0 = clock()
+    for i, a in enumerate(analyses):
+        a.t_arrival = t0 + (arrivals[i] if arrivals is not None else 0.0)
+    x_el = None
+    cg_el = 0
+    elastic_pending: List[int] = []
+    admitted: List[Analysis] = []  # in admission order
+
+    def admit_ready(now):
+        return [i for i in queue if arrivals is None or arrivals[i] <= now - t0]
+
+    def start_trial(j, a, k):
+        s = a.srf[k]
+        cohf, nuf, phif, psif, _ = strength_reduction(s, False, a.coh, nu, a.phi,
+                                                      a.psi)
+        dt = dt_viscoplas(nuf, phif, ym)
+        engine.begin(j, ym, nuf, phif, cohf, psif, dt,
+                     x0=x_el if x_el is not None else None, x0_cg=cg_el)
+        cols[j] = _Col(analysis=a, srf_idx=k)
+        if a.t_start is None:
+            a.t_start = clock()
+
+    def fill_free_columns():
+        if x_el is None and elastic_pending:
+            return  # one column solves the elastic system before the others start
+        now = clock()
+        for j in range(ncol):
+            if cols[j].analysis is not None:
+                continue
+            # the next SRF of an admitted analysis (up to srf_parallel at once)
+            placed = False
+            for a in admitted:
+                if a.done:
+                    continue
+                running = sum(1 for cc in cols if cc.analysis is a)
+                if (running < srf_parallel and a.next_srf < len(a.srf)
+                        and (a.fail_idx is None or a.next_srf < a.fail_idx)):
+                    start_trial(j, a, a.next_srf)
+                    a.next_srf += 1
+                    placed = True
+                    break
+            if placed:
+                continue
+            ready = admit_ready(now)
+            if not ready:
+                continue
+            i = ready[0]
+            queue.remove(i)
+            a = analyses[i]
+            admitted.append(a)
+            start_trial(j, a, a.next_srf)
+            a.next_srf += 1
+            if x_el is None:
+                elastic_pending.append(j)
+                return
+
+    def finish_analysis_if_possible(a):
+        if a.done:
+            return
+        # the bracket is known when every SRF before the first failure has
+        # converged (src/slope.F90:441: the sweep stops at the first failure)
+        if a.fail_idx is not None and all(k in a.rows for k in range(a.fail_idx)):
+            prev = a.srf[a.fail_idx - 1] if a.fail_idx > 0 else None
+            a.bracket = (prev, a.srf[a.fail_idx])
+            a.done = True
+        elif a.fail_idx is None and len(a.rows) == len(a.srf):
+            a.bracket = None  # no failure in the list
+            a.done = True
+        if a.done:
+            a.t_end = clock()
+            if a.fail_idx is not None:  # rows the reference would not print
+                a.rows = {k: v for k, v in a.rows.items() if k <= a.fail_idx}
+            for j, c in enumerate(cols):  # abandon speculative trials
+                if c.analysis is a:
+                    engine.release(j)
+                    cols[j] = _Col()
+            if on_done:
+                on_done(a)
+
+    fill_free_columns()
+    while True:
+        active = [j for j in range(ncol) if cols[j].analysis is not None]
+        if not active:
+            if queue:
+                fill_free_columns()
+                if not any(c.analysis for c in cols):
+                    time.sleep(0.001)
+                continue
+            break
+        is_last = np.array([1 if (c.analysis is not None and c.nl_iter + 1 == nl_maxiter)
+                            else 0 for c in cols], np.int32)
+        r = engine.plastic_iter(is_last, cg_tol, cg_maxiter, inner_tol, nl_tol)
+        for j in active:
+            c = cols[j]
+            c.nl_iter += 1
+            if r["errcode"][j] != 0:
+                raise RuntimeError("ERROR: PCG solver doesn't converge!")
+            c.cg_tot += int(r["cg_iter"][j])
+            c.fmax = float(r["fmax"][j])
+            if x_el is None and j in elastic_pending and c.nl_iter == 1:
+                x_el, _ = engine.download(j)
+                cg_el = int(r["cg_iter"][j])
+                elastic_pending.clear()
+            if r["nl_isconv"][j] or c.nl_iter == nl_maxiter:
+                a, k = c.analysis, c.srf_idx
+                x, flags = engine.download(j)
+                ux, um = uxmax_umax(spec, x)
+                a.rows[k] = dict(srf=float(a.srf[k]), cg_tot=c.cg_tot,
+                                 nl_tot=c.nl_iter, uxmax=ux, umax=um, fmax=c.fmax)
+                if keep_state:
+                    a.flags_last, a.x_last = flags, x
+                if c.nl_iter == nl_maxiter and (a.fail_idx is None or k < a.fail_idx):
+                    a.fail_idx = k
+                engine.release(j)
+                cols[j] = _Col()
+                finish_analysis_if_possible(a)
+        fill_free_columns()
+    return engine.stats()
+
+
+def device_mesh(ctx: Context, mesh: HexMesh, spec: Optional[SpecMesh] = None):
+    """Resident mesh with its elastic stiffness; returns (DeviceMesh, extload)."""
+    spec = spec or hex2spec(mesh)
+    gam, ym, nu, *_ = mesh.material_arrays()
+    dm = DeviceMesh(ctx, spec, len(gam))
+    ext, _ = dm.build_stiffness(gam, ym, nu, kprec=K_F64)
+    return dm, ext
+
+
+def slope3d_mixed(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
+                  nl_tol: float, nl_maxiter: int, srf: Sequence[float],
+                  inner_tol: float = 1e-5, ncol: int = 1, spec=None) -> SlopeResult:
+    """The reference's SRF sweep (src/slope.F90:244-443) in the mixed-precision
+    form: FP32 stiffness and inner CG, FP64 refinement to cg_tol.  With
+    ncol > 1 up to ncol SRFs of the sweep run at once; the rows and the
+    bracket are those of the sequential sweep."""
+    spec = spec or hex2spec(mesh)
+    _, ym, nu, phi, coh, psi = mesh.material_arrays()
+    dm, ext = device_mesh(ctx, mesh, spec)
+    eng = BatchEngine(dm, ext, ncol)
+    a = Analysis(coh=coh, phi=phi, psi=psi, srf=list(srf))
+    try:
+        run_analyses(eng, [a], ym=ym, nu=nu, cg_tol=cg_tol, cg_maxiter=cg_maxiter,
+                     nl_tol=nl_tol, nl_maxiter=nl_maxiter, inner_tol=inner_tol,
+                     srf_parallel=ncol, keep_state=True)
+        stats = eng.stats()
+    finally:
+        eng.close()
+        dm.close()
+    rows = [a.rows[k] for k in sorted(a.rows)]
+    return SlopeResult(rows=rows, cg_log=[], fos_bracket=a.bracket, x=a.x_last,
+                       flags=a.flags_last, extra=dict(stats=stats))

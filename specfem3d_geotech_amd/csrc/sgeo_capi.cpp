@@ -174,15 +174,6 @@ extern "C" int sgeo_init(int device, sgeo_ctx** out) {
   HIPCHK(hipHostMalloc((void**)&c->h_ps, sizeof(PlasticScalars)));
   HIPCHK(hipMemsetAsync(c->d_sc, 0, sizeof(PcgScalars), c->stream));
   HIPCHK(hipMemsetAsync(c->d_ps, 0, sizeof(PlasticScalars), c->stream));
-  // GLL tables: l_m'(xi_q) at {-1,0,1} (src/gll_library.f90:257-299) and
-  // weights wx*wy*wz with wx = {1/3, 4/3, 1/3} (zwgljd, np=3)
-  const double d1[9] = {-1.5, 2.0, -0.5, -0.5, 0.0, 0.5, 0.5, -2.0, 1.5};
-  const double wx[3] = {1.0 / 3.0, 4.0 / 3.0, 1.0 / 3.0};
-  double w[27];
-  for (int k = 0, n = 0; k < 3; k++)
-    for (int j = 0; j < 3; j++)
-      for (int i = 0; i < 3; i++) w[n++] = wx[i] * wx[j] * wx[k];
-  upload_tables(c->stream, d1, w);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   *out = c;
@@ -740,9 +731,21 @@ extern "C" int sgeo_slope_plastic_iter(sgeo_slope* t, double cg_tol,
   if (errcode) *errcode = ec;
   if (ec != 0) return 0;  // the caller stops (src/slope.F90:295)
   launch_uerr(s, n1, t->d_x, t->d_oldx, c->d_part, c->d_ps, nl_tol);
-  launch_plastic_body(s, m->nelmt, nullptr, m->d_corners, m->d_gdof, m->d_mat,
-                      t->d_mp, t->d_x, t->d_evpt, t->d_stress, t->d_BL,
-                      t->d_flags, c->d_ps, is_last, 0, nullptr, nullptr);
+  BodyArgs b;
+  b.nel = m->nelmt;
+  b.corners = m->d_corners;
+  b.gdof = m->d_gdof;
+  b.mat_id = m->d_mat;
+  b.mp = t->d_mp;
+  b.nmat = m->nmat;
+  b.x = t->d_x;
+  b.evpt = t->d_evpt;
+  b.stress = t->d_stress;
+  b.BL = t->d_BL;
+  b.flags = t->d_flags;
+  b.ps = c->d_ps;
+  b.is_last = is_last;
+  launch_plastic_body(s, b);
   // bodyload(egdof) += bload unless converged/last (src/slope.F90:380-384);
   // the kernel itself returns when nl_isconv is set on the device
   if (!is_last)
@@ -829,5 +832,340 @@ extern "C" int sgeo_slope_download(sgeo_slope* t, double* x, double* bodyload,
     HIPCHK(hipMemcpyAsync(flags, t->d_flags, (size_t)m->nelmt * 27,
                           hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// multi-trial mixed-precision engine
+// --------------------------------------------------------------------------
+struct sgeo_batch {
+  sgeo_mesh* mesh = nullptr;
+  int B = 0;
+  int64_t n1 = 0;
+  double *d_x = nullptr, *d_oldx = nullptr, *d_load = nullptr, *d_body = nullptr;
+  double *d_Y64 = nullptr, *d_ext = nullptr;
+  float *d_r = nullptr, *d_z = nullptr, *d_p = nullptr, *d_kp = nullptr;
+  float *d_d = nullptr, *d_Y32 = nullptr, *d_dinv32 = nullptr;
+  double *d_evpt = nullptr, *d_stress = nullptr, *d_BL = nullptr;
+  uint8_t* d_flags = nullptr;
+  MatParams* d_mp = nullptr;
+  PlasticScalars *d_ps = nullptr, *h_ps = nullptr;
+  BCol *d_bc = nullptr, *h_bc = nullptr;
+  int32_t *d_cst = nullptr, *h_cst = nullptr;
+  int32_t *d_active = nullptr, *d_islast = nullptr, *d_cols = nullptr;
+  double* d_part = nullptr;
+  std::vector<int32_t> active, memo_cg, memo_pending;
+  std::vector<MatParams> h_mp;
+  int64_t stat_inner = 0, stat_outer = 0, stat_steps = 0;
+  int k_ver = -1;  // mesh stiffness version the FP32 diagonal was made from
+  size_t bytes = 0;
+};
+
+static int ensure_packed(sgeo_mesh* m, int kprec) {
+  const int kp = m->kprec, kf = m->kfmt;
+  m->kprec = kprec;
+  m->kfmt = SGEO_KFMT_PACKED;
+  const int rc = refresh_operator(m);
+  m->kprec = kp;
+  m->kfmt = kf;
+  return rc;
+}
+
+// Packed FP32/FP64 operators and the FP32 inverse diagonal follow the mesh's
+// current stiffness (rebuilt when K changed since the last call).
+static int batch_sync_operator(sgeo_batch* b) {
+  sgeo_mesh* m = b->mesh;
+  TRY(ensure_packed(m, SGEO_K_F32));
+  TRY(ensure_packed(m, SGEO_K_F64));
+  if (b->k_ver != m->k_ver) {
+    launch_to_f32(m->ctx->stream, b->n1, m->d_dinv, b->d_dinv32);
+    b->k_ver = m->k_ver;
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extload,
+                                 sgeo_batch** out) {
+  if (!m || !extload || !out || m->kprec < 0 || !m->d_corners || !m->d_mat)
+    return fail(SGEO_EARG, "sgeo_batch_create: mesh needs stiffness, corners, mat_id");
+  if (ncol < 1 || ncol > 64 || (ncol & (ncol - 1)))
+    return fail(SGEO_EARG, "sgeo_batch_create: ncol %d not a power of 2 <= 64", ncol);
+  sgeo_ctx* c = m->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  sgeo_batch* b = new sgeo_batch();
+  b->mesh = m;
+  b->B = ncol;
+  b->n1 = (int64_t)m->neq + 1;
+  const size_t nv = (size_t)b->n1 * ncol;
+  const size_t ny = (size_t)m->nelmt * 81 * ncol;
+  const size_t ns = (size_t)m->nelmt * 27 * 6 * ncol;
+  size_t* acc = &b->bytes;
+  TRY(dalloc(&b->d_x, nv, acc));
+  TRY(dalloc(&b->d_oldx, nv, acc));
+  TRY(dalloc(&b->d_load, nv, acc));
+  TRY(dalloc(&b->d_body, nv, acc));
+  TRY(dalloc(&b->d_Y64, ny, acc));
+  TRY(dalloc(&b->d_ext, (size_t)b->n1, acc));
+  TRY(dalloc(&b->d_r, nv, acc));
+  TRY(dalloc(&b->d_z, nv, acc));
+  TRY(dalloc(&b->d_p, nv, acc));
+  TRY(dalloc(&b->d_kp, nv, acc));
+  TRY(dalloc(&b->d_d, nv, acc));
+  TRY(dalloc(&b->d_Y32, ny, acc));
+  TRY(dalloc(&b->d_dinv32, (size_t)b->n1, acc));
+  TRY(dalloc(&b->d_evpt, ns, acc));
+  TRY(dalloc(&b->d_stress, ns, acc));
+  TRY(dalloc(&b->d_BL, ny, acc));
+  TRY(dalloc(&b->d_flags, (size_t)m->nelmt * 27 * ncol, acc));
+  TRY(dalloc(&b->d_mp, (size_t)std::max(1, m->nmat) * ncol, acc));
+  TRY(dalloc(&b->d_ps, (size_t)ncol, acc));
+  TRY(dalloc(&b->d_bc, (size_t)ncol, acc));
+  TRY(dalloc(&b->d_cst, (size_t)ncol, acc));
+  TRY(dalloc(&b->d_active, (size_t)ncol, acc));
+  TRY(dalloc(&b->d_islast, (size_t)ncol, acc));
+  TRY(dalloc(&b->d_cols, (size_t)ncol, acc));
+  TRY(dalloc(&b->d_part, (size_t)batch_partials(b->n1) * ncol, acc));
+  HIPCHK(hipHostMalloc((void**)&b->h_ps, sizeof(PlasticScalars) * ncol));
+  HIPCHK(hipHostMalloc((void**)&b->h_bc, sizeof(BCol) * ncol));
+  HIPCHK(hipHostMalloc((void**)&b->h_cst, sizeof(int32_t) * ncol));
+  b->active.assign(ncol, 0);
+  b->memo_cg.assign(ncol, 0);
+  b->memo_pending.assign(ncol, 0);
+  b->h_mp.assign((size_t)std::max(1, m->nmat) * ncol, MatParams{});
+  HIPCHK(hipMemcpyAsync(b->d_ext, extload, b->n1 * 8, hipMemcpyHostToDevice, s));
+  TRY(batch_sync_operator(b));
+  HIPCHK(hipMemsetAsync(b->d_active, 0, ncol * 4, s));
+  HIPCHK(hipMemsetAsync(b->d_cst, 0, ncol * 4, s));
+  HIPCHK(hipMemsetAsync(b->d_ps, 0, ncol * sizeof(PlasticScalars), s));
+  HIPCHK(hipMemsetAsync(b->d_x, 0, nv * 8, s));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  *out = b;
+  return 0;
+}
+
+extern "C" int sgeo_batch_destroy(sgeo_batch* b) {
+  if (!b) return 0;
+  hipSetDevice(b->mesh->ctx->device);
+  hipStreamSynchronize(b->mesh->ctx->stream);
+  void* ptrs[] = {b->d_x, b->d_oldx, b->d_load, b->d_body, b->d_Y64, b->d_ext,
+                  b->d_r, b->d_z, b->d_p, b->d_kp, b->d_d, b->d_Y32,
+                  b->d_dinv32, b->d_evpt, b->d_stress, b->d_BL, b->d_flags,
+                  b->d_mp, b->d_ps, b->d_bc, b->d_cst, b->d_active, b->d_islast,
+                  b->d_cols, b->d_part};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (b->h_ps) hipHostFree(b->h_ps);
+  if (b->h_bc) hipHostFree(b->h_bc);
+  if (b->h_cst) hipHostFree(b->h_cst);
+  delete b;
+  return 0;
+}
+
+extern "C" int sgeo_batch_bytes(sgeo_batch* b, size_t* bytes) {
+  if (!b || !bytes) return fail(SGEO_EARG, "null argument");
+  *bytes = b->bytes;
+  return 0;
+}
+
+extern "C" int sgeo_batch_begin(sgeo_batch* b, int32_t col, const double* ym,
+                                const double* nuf, const double* phif,
+                                const double* cohf, const double* psif,
+                                double dt, const double* x0, int32_t x0_cg_iter) {
+  if (!b || col < 0 || col >= b->B || !ym || !nuf || !phif || !cohf || !psif)
+    return fail(SGEO_EARG, "sgeo_batch_begin: bad arguments");
+  sgeo_mesh* m = b->mesh;
+  sgeo_ctx* c = m->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const int nmat = std::max(1, m->nmat);
+  for (int i = 0; i < m->nmat; i++)
+    b->h_mp[(size_t)col * nmat + i] =
+        MatParams{ym[i], nuf[i], phif[i], cohf[i], psif[i], dt};
+  HIPCHK(hipMemcpyAsync(b->d_mp + (size_t)col * nmat, &b->h_mp[(size_t)col * nmat],
+                        nmat * sizeof(MatParams), hipMemcpyHostToDevice, s));
+  const size_t ns = (size_t)m->nelmt * 27 * 6;
+  // stress_elmt=0, evpt=0, bodyload=0, x=0 or x0, oldx=0
+  // (src/slope.F90:249, 281-282)
+  HIPCHK(hipMemsetAsync(b->d_evpt + col * ns, 0, ns * 8, s));
+  HIPCHK(hipMemsetAsync(b->d_stress + col * ns, 0, ns * 8, s));
+  HIPCHK(hipMemsetAsync(b->d_flags + (size_t)col * m->nelmt * 27, 0,
+                        (size_t)m->nelmt * 27, s));
+  launch_col_io(s, b->n1, b->B, col, b->d_body, nullptr, nullptr);
+  launch_col_io(s, b->n1, b->B, col, b->d_oldx, nullptr, nullptr);
+  double* d_x0 = nullptr;
+  if (x0) {
+    TRY(dalloc(&d_x0, (size_t)b->n1));
+    HIPCHK(hipMemcpyAsync(d_x0, x0, b->n1 * 8, hipMemcpyHostToDevice, s));
+  }
+  launch_col_io(s, b->n1, b->B, col, b->d_x, d_x0, nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  if (d_x0) hipFree(d_x0);
+  b->active[col] = 1;
+  b->memo_pending[col] = x0 ? 1 : 0;
+  b->memo_cg[col] = x0_cg_iter;
+  return 0;
+}
+
+extern "C" int sgeo_batch_release(sgeo_batch* b, int32_t col) {
+  if (!b || col < 0 || col >= b->B) return fail(SGEO_EARG, "bad column");
+  b->active[col] = 0;
+  b->memo_pending[col] = 0;
+  return 0;
+}
+
+// One plastic iteration of every active column in lockstep: load assembly,
+// refinement PCG (FP64 residual, FP32 inner CG, per-column masking), uerr,
+// element x GLL body and body-load assembly (src/slope.F90:286-391).
+extern "C" int sgeo_batch_plastic_iter(sgeo_batch* b, const int32_t* is_last,
+                                       double cg_tol, int32_t cg_maxiter,
+                                       double inner_tol, double nl_tol,
+                                       int32_t* cg_iter, int32_t* errcode,
+                                       double* uerr, int32_t* nl_isconv,
+                                       double* fmax) {
+  if (!b || !is_last) return fail(SGEO_EARG, "sgeo_batch_plastic_iter: bad arguments");
+  sgeo_mesh* m = b->mesh;
+  sgeo_ctx* c = m->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const int B = b->B;
+  const int64_t n1 = b->n1;
+  std::vector<int32_t> cols, cst(B, BCOL_OFF), islast(B, 0);
+  for (int j = 0; j < B; j++) {
+    if (!b->active[j]) continue;
+    cols.push_back(j);
+    islast[j] = is_last[j] ? 1 : 0;
+    cst[j] = b->memo_pending[j] ? BCOL_DONE : BCOL_NEED_RESID;
+    BCol& bc = b->h_bc[j];
+    bc = BCol{};
+    bc.tol = cg_tol;
+    bc.inner_tol = inner_tol;
+    bc.inner_max = cg_maxiter;
+    bc.outer_max = 50;
+    bc.cg_maxiter = cg_maxiter;
+    bc.cg_total = b->memo_pending[j] ? b->memo_cg[j] : 0;
+  }
+  if (cols.empty()) return fail(SGEO_ESTATE, "no active column");
+  TRY(batch_sync_operator(b));
+  HIPCHK(hipMemcpyAsync(b->d_active, b->active.data(), B * 4, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(b->d_islast, islast.data(), B * 4, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(b->d_cols, cols.data(), cols.size() * 4,
+                        hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(b->d_cst, cst.data(), B * 4, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(b->d_bc, b->h_bc, B * sizeof(BCol), hipMemcpyHostToDevice, s));
+  launch_bload(s, n1, B, b->d_ext, b->d_body, b->d_load, b->d_active);
+  // refinement PCG
+  for (int outer = 0; outer < 64; outer++) {
+    bool need = false;
+    for (int j : cols) need = need || cst[j] == BCOL_NEED_RESID;
+    if (!need) break;
+    launch_bmv_f64(s, m->nelmt, B, m->d_KP64, m->d_gdof, b->d_x, b->d_Y64, b->d_cst,
+                   BCOL_NEED_RESID);
+    launch_bresid(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y64, b->d_load,
+                  b->d_dinv32, b->d_r, b->d_z, b->d_p, b->d_d, b->d_part, b->d_cst);
+    launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_RZ);
+    b->stat_outer++;
+    int chunk = 8;
+    for (;;) {
+      for (int it = 0; it < chunk; it++) {
+        launch_bmv_f32(s, m->nelmt, B, m->d_KP32, m->d_gdof, b->d_p, b->d_Y32,
+                       b->d_cst, BCOL_INNER);
+        launch_bassemble_dot(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y32, b->d_p,
+                             b->d_kp, b->d_part, b->d_cst);
+        launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_ALPHA);
+        launch_bupdate(s, n1, B, b->d_p, b->d_kp, b->d_dinv32, b->d_d, b->d_r,
+                       b->d_z, b->d_part, b->d_cst, b->d_bc);
+        launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_INNER);
+        launch_bpupdate(s, n1, B, b->d_z, b->d_p, b->d_cst, b->d_bc);
+        b->stat_steps++;
+      }
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(b->h_cst, b->d_cst, B * 4, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      bool inner = false;
+      for (int j : cols) inner = inner || b->h_cst[j] == BCOL_INNER;
+      if (!inner) break;
+      chunk = std::min(chunk * 2, 64);
+    }
+    launch_bxupdate(s, n1, B, b->d_d, b->d_x, b->d_part, b->d_cst);
+    launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_OUTER);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(b->h_cst, b->d_cst, B * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int j : cols) cst[j] = b->h_cst[j];
+  }
+  // uerr, body, body load for every active column
+  launch_buerr(s, n1, B, b->d_x, b->d_oldx, b->d_part, b->d_ps, b->d_active, nl_tol);
+  BodyArgs ba;
+  ba.nel = m->nelmt;
+  ba.corners = m->d_corners;
+  ba.gdof = m->d_gdof;
+  ba.mat_id = m->d_mat;
+  ba.mp = b->d_mp;
+  ba.nmat = std::max(1, m->nmat);
+  ba.x = b->d_x;
+  ba.xs = B;
+  ba.evpt = b->d_evpt;
+  ba.stress = b->d_stress;
+  ba.BL = b->d_BL;
+  ba.flags = b->d_flags;
+  ba.ps = b->d_ps;
+  ba.is_last_col = b->d_islast;
+  ba.cols = b->d_cols;
+  ba.ncols = (int)cols.size();
+  launch_plastic_body(s, ba);
+  launch_bbody_assemble(s, n1, B, m->nelmt, m->d_rowptr, m->d_slots, b->d_BL,
+                        b->d_body, b->d_active, b->d_islast, b->d_ps);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(b->h_ps, b->d_ps, B * sizeof(PlasticScalars),
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(b->h_bc, b->d_bc, B * sizeof(BCol), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int j = 0; j < B; j++) {
+    if (!b->active[j]) continue;
+    b->stat_inner += b->memo_pending[j] ? 0 : b->h_bc[j].cg_total;
+    if (cg_iter) cg_iter[j] = b->h_bc[j].cg_total;
+    if (errcode) errcode[j] = cst[j] == BCOL_DONE ? 0 : -1;
+    if (uerr) uerr[j] = b->h_ps[j].uerr;
+    if (nl_isconv) nl_isconv[j] = b->h_ps[j].nl_isconv;
+    if (fmax) {
+      unsigned long long bits = b->h_ps[j].fmax_bits;
+      memcpy(&fmax[j], &bits, 8);
+    }
+    b->memo_pending[j] = 0;
+  }
+  return 0;
+}
+
+extern "C" int sgeo_batch_download(sgeo_batch* b, int32_t col, double* x,
+                                   uint8_t* flags) {
+  if (!b || col < 0 || col >= b->B) return fail(SGEO_EARG, "bad column");
+  sgeo_mesh* m = b->mesh;
+  hipStream_t s = m->ctx->stream;
+  HIPCHK(hipSetDevice(m->ctx->device));
+  if (x) {
+    double* tmp = nullptr;
+    TRY(dalloc(&tmp, (size_t)b->n1));
+    launch_col_io(s, b->n1, b->B, col, b->d_x, nullptr, tmp);
+    HIPCHK(hipMemcpyAsync(x, tmp, b->n1 * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    hipFree(tmp);
+  }
+  if (flags)
+    HIPCHK(hipMemcpyAsync(flags, b->d_flags + (size_t)col * m->nelmt * 27,
+                          (size_t)m->nelmt * 27, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int sgeo_batch_stats(sgeo_batch* b, int64_t* inner_iters,
+                                int64_t* outer_iters, int64_t* steps) {
+  if (!b) return fail(SGEO_EARG, "null batch");
+  if (inner_iters) *inner_iters = b->stat_inner;
+  if (outer_iters) *outer_iters = b->stat_outer;
+  if (steps) *steps = b->stat_steps;
   return 0;
 }

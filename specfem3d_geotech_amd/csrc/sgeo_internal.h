@@ -5,14 +5,9 @@
 #include <stdint.h>
 
 #include "sgeo_device.h"
+#include "sgeo_element.h"
 
 namespace sgeo {
-
-// Factored material block of one strength-reduction trial
-// (src/plastic_library.f90:42-88) plus the trial's pseudo-time step.
-struct MatParams {
-  double ym, nuf, phif, cohf, psif, dt;
-};
 
 // Scalars of one plastic iteration (src/slope.F90:286-309), device resident.
 struct PlasticScalars {
@@ -23,9 +18,32 @@ struct PlasticScalars {
   int32_t n_plastic;
 };
 
+// Arguments of the (single- or multi-trial) plastic body launch.
+struct BodyArgs {
+  int32_t nel = 0;                 // elements processed
+  const int32_t* elmt = nullptr;   // element ids (NULL: 0..nel-1)
+  const double* corners = nullptr;
+  const int32_t* gdof = nullptr;   // gdof of the processed elements
+  const int32_t* mat_id = nullptr;
+  const MatParams* mp = nullptr;   // [col][nmat]
+  int nmat = 0;
+  const double* x = nullptr;       // x[g*xs + col]
+  int xs = 1;
+  double* evpt = nullptr;          // [col][nel][27][6]
+  double* stress = nullptr;
+  double* BL = nullptr;            // [col][nel][81]
+  uint8_t* flags = nullptr;        // [col][nel][27]
+  PlasticScalars* ps = nullptr;    // [col]
+  int is_last = 0;                 // used when is_last_col == NULL
+  const int32_t* is_last_col = nullptr;
+  const int32_t* cols = nullptr;   // active columns (NULL: column 0)
+  int ncols = 1;
+  int excav = 0;
+  double* vm_pt = nullptr;         // [col][nel][27] (excavation)
+};
+
 unsigned vec_reduce_blocks(int64_t n1);
 
-void upload_tables(hipStream_t s, const double* d1, const double* w);
 void launch_stiffness(hipStream_t s, int32_t nelmt, const double* corners,
                       const int32_t* mat_id, const double* ym, const double* nu,
                       double* K, double* kdiag, double* eld);
@@ -75,12 +93,60 @@ void launch_load(hipStream_t s, int64_t n1, const double* ext,
                  const double* body, double* load);
 void launch_uerr(hipStream_t s, int64_t n1, double* x, double* oldx,
                  double* part, PlasticScalars* ps, double nl_tol);
-void launch_plastic_body(hipStream_t s, int32_t nel, const int32_t* elmt,
-                         const double* corners, const int32_t* gdof,
-                         const int32_t* mat_id, const MatParams* mp,
-                         const double* x, double* evpt, double* stress,
-                         double* BL, uint8_t* flags, PlasticScalars* ps,
-                         int is_last, int excav, const int32_t* g_num,
-                         double* vm_pt);
+void launch_plastic_body(hipStream_t s, const BodyArgs& b);
+
+// ---- multi-trial mixed-precision engine (sgeo_batch.hip) -----------------
+// column states of the refinement PCG
+enum BColState {
+  BCOL_OFF = 0,         // column not in this solve
+  BCOL_NEED_RESID = 1,  // next: FP64 residual r = f - K x, then inner start
+  BCOL_INNER = 2,       // inner FP32 CG running
+  BCOL_INNER_DONE = 3,  // inner stopped: next x += d and the outer test
+  BCOL_DONE = 4,        // converged
+  BCOL_FAIL = 5,        // cg_maxiter or outer_max reached
+};
+enum BFinWhat { BFIN_RZ = 0, BFIN_ALPHA = 1, BFIN_INNER = 2, BFIN_OUTER = 3 };
+
+struct BCol {  // per-column scalars, device resident
+  double rz, alpha, beta, maxp, maxd, maxx;
+  double tol, inner_tol;
+  int32_t inner_it, inner_max, outer_it, outer_max, cg_total, cg_maxiter;
+};
+
+int64_t batch_partials(int64_t n1);
+void launch_bmv_f32(hipStream_t s, int32_t nelmt, int B, const float* KP,
+                    const int32_t* gdof, const float* P, float* Y,
+                    const int32_t* cst, int want);
+void launch_bmv_f64(hipStream_t s, int32_t nelmt, int B, const double* KP,
+                    const int32_t* gdof, const double* X, double* Y,
+                    const int32_t* cst, int want);
+void launch_bassemble_dot(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
+                          const int32_t* slots, const float* Y, const float* p,
+                          float* kp, double* part, const int32_t* cst);
+void launch_bupdate(hipStream_t s, int64_t n1, int B, const float* p,
+                    const float* kp, const float* dinv, float* d, float* r,
+                    float* z, double* part, const int32_t* cst, const BCol* bc);
+void launch_bpupdate(hipStream_t s, int64_t n1, int B, const float* z, float* p,
+                     const int32_t* cst, const BCol* bc);
+void launch_bresid(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
+                   const int32_t* slots, const double* Y64, const double* load,
+                   const float* dinv, float* r, float* z, float* p, float* d,
+                   double* part, const int32_t* cst);
+void launch_bxupdate(hipStream_t s, int64_t n1, int B, const float* d, double* x,
+                     double* part, const int32_t* cst);
+void launch_bfinalize(hipStream_t s, int64_t n1, int B, const double* part,
+                      int32_t* cst, BCol* bc, int what);
+void launch_bload(hipStream_t s, int64_t n1, int B, const double* ext,
+                  const double* body, double* load, const int32_t* active);
+void launch_buerr(hipStream_t s, int64_t n1, int B, double* x, double* oldx,
+                  double* part, PlasticScalars* ps, const int32_t* active,
+                  double nl_tol);
+void launch_col_io(hipStream_t s, int64_t n1, int B, int j, double* a,
+                   const double* src, double* dst);
+void launch_to_f32(hipStream_t s, int64_t n, const double* a, float* b);
+void launch_bbody_assemble(hipStream_t s, int64_t n1, int B, int32_t nel,
+                           const int32_t* rowptr, const int32_t* slots,
+                           const double* BL, double* body, const int32_t* active,
+                           const int32_t* is_last, const PlasticScalars* ps);
 
 }  // namespace sgeo

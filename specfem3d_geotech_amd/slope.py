@@ -97,13 +97,18 @@ def uxmax_umax(spec: SpecMesh, x: np.ndarray):
 
 def slope3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
             nl_tol: float, nl_maxiter: int, srf: Sequence[float],
-            phinu: bool = False, kprec: int = K_F64,
-            spec: Optional[SpecMesh] = None) -> SlopeResult:
-    """Strength-reduction sweep of src/slope.F90:244-443 on the GPU."""
+            phinu: bool = False, kprec: int = K_F64, kfmt: int = 0,
+            spec: Optional[SpecMesh] = None, progress=None) -> SlopeResult:
+    """Strength-reduction sweep of src/slope.F90:244-443 on the GPU.
+
+    kprec/kfmt select the operator the PCG streams (FP64 full reproduces
+    the reference's arithmetic; packed halves the bytes).  `progress`, if
+    given, is called with each summary row."""
     spec = spec or hex2spec(mesh)
     gam, ym, nu, phi, coh, psi = mesh.material_arrays()
     dm = DeviceMesh(ctx, spec, len(gam))
-    ext, _ = dm.build_stiffness(gam, ym, nu, kprec=kprec)
+    ext, _ = dm.build_stiffness(gam, ym, nu, kprec=K_F64)
+    dm.set_operator(kprec, kfmt)
     st = DeviceSlope(dm, ext)
     rows: List[Dict[str, float]] = []
     cg_log: List[int] = []
@@ -117,7 +122,8 @@ def slope3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
                                                                phi, psi)
             dt = dt_viscoplas(nuf, phif, ym)
             if istat == 1:  # src/slope.F90:259-270
-                dm.build_stiffness(gam, ym, nuf, kprec=kprec, want_extload=False)
+                dm.build_stiffness(gam, ym, nuf, kprec=K_F64, want_extload=False)
+                dm.set_operator(kprec, kfmt)
             st.begin_srf(ym, nuf, phif, cohf, psif, dt)
             cg_tot = 0
             fmax = 0.0
@@ -137,6 +143,8 @@ def slope3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
             ux, um = uxmax_umax(spec, state["x"])
             rows.append(dict(srf=float(s), cg_tot=cg_tot, nl_tot=nl_iter, uxmax=ux,
                              umax=um, fmax=fmax))
+            if progress:
+                progress(rows[-1])
             if nl_iter == nl_maxiter:  # src/slope.F90:441
                 bracket = (prev, float(s))
                 break

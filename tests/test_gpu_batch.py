@@ -1,0 +1,140 @@
+"""Mixed-precision multi-trial engine (sgeo_batch, through the C ABI).
+
+* A trial's bits do not depend on the batch: the same analysis run alone
+  (one column) and among other analyses that start at other times gives
+  identical rows, displacements and plastic-point flags.
+* Against the FP64 reference (golden summaries from the compiled reference):
+  the same FOS bracket, the same plastic iteration counts on converged SRFs,
+  and displacements within the refinement tolerance.
+* One refined solve of K x = f against the FP64 oracle's PCG run to a tight
+  tolerance: the FP32 inner CG with FP64 refinement reaches the FP64 answer
+  to within the refinement tolerance, no worse than the FP64 PCG at cg_tol.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import control_from_golden, golden, mesh_from_golden
+from specfem3d_geotech_amd.batch import (Analysis, BatchEngine, device_mesh,
+                                         run_analyses, slope3d_mixed)
+from specfem3d_geotech_amd.spec import hex2spec
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _analyses(mesh, n, srf, seed):
+    """`n` strength realisations around the mesh's own (the first is the
+    mesh's own strengths)."""
+    _, _, _, phi, coh, psi = mesh.material_arrays()
+    rng = np.random.default_rng(seed)
+    out = [Analysis(coh=coh.copy(), phi=phi.copy(), psi=psi.copy(), srf=list(srf),
+                    tag=0)]
+    for i in range(1, n):
+        out.append(Analysis(coh=coh * rng.uniform(0.8, 1.25, coh.shape),
+                            phi=phi * rng.uniform(0.85, 1.15, phi.shape),
+                            psi=psi.copy(), srf=list(srf), tag=i))
+    return out
+
+
+def test_batch_trial_identical_alone_and_in_mix(ctx):
+    z = golden("slope_8x4x6_layered")
+    mesh = mesh_from_golden(z)
+    spec = hex2spec(mesh)
+    ctl = control_from_golden(z)
+    ctl["nl_maxiter"] = 40
+    srf = [1.0, 1.3, 1.6, 1.9]
+    _, ym, nu, *_ = mesh.material_arrays()
+    kw = dict(ym=ym, nu=nu, cg_tol=ctl["cg_tol"], cg_maxiter=ctl["cg_maxiter"],
+              nl_tol=ctl["nl_tol"], nl_maxiter=ctl["nl_maxiter"], keep_state=True)
+    dm, ext = device_mesh(ctx, mesh, spec)
+    try:
+        alone = _analyses(mesh, 6, srf, seed=7)
+        target = alone[3]
+        eng = BatchEngine(dm, ext, 1)
+        try:
+            run_analyses(eng, [target], **kw)
+        finally:
+            eng.close()
+        mix = _analyses(mesh, 6, srf, seed=7)
+        # the target enters third, behind analyses already in flight
+        order = [0, 1, 3, 2, 4, 5]
+        eng = BatchEngine(dm, ext, 8)
+        try:
+            run_analyses(eng, [mix[i] for i in order], srf_parallel=2, **kw)
+        finally:
+            eng.close()
+        got = mix[3]
+        print("alone", target.rows, target.bracket)
+        print("mix  ", got.rows, got.bracket)
+        assert got.bracket == target.bracket
+        assert got.rows == target.rows
+        assert np.array_equal(got.x_last, target.x_last)
+        assert np.array_equal(got.flags_last, target.flags_last)
+    finally:
+        dm.close()
+
+
+@pytest.mark.parametrize("name,ncol", [("slope_4x2x3", 1), ("slope_8x4x6_layered", 1),
+                                       ("slope_8x4x6_layered", 4)])
+def test_mixed_sweep_matches_reference(ctx, name, ncol):
+    """Same FOS bracket as the FP64 reference; converged SRFs have the same
+    plastic iteration count and displacement maxima within 1e-7 (the FP64
+    reference's own PCG stops at a relative update of cg_tol = 1e-8)."""
+    z = golden(name)
+    mesh = mesh_from_golden(z)
+    ctl = control_from_golden(z)
+    res = slope3d_mixed(ctx, mesh, ncol=ncol, **ctl)
+    ref = z["summary"]
+    print(name, "stats", res.extra["stats"])
+    assert len(res.rows) == ref.shape[0]
+    worst = 0.0
+    for r, rr in zip(res.rows, ref):
+        print(name, r, rr.tolist())
+        assert r["srf"] == rr[0]
+        if rr[2] < ctl["nl_maxiter"]:
+            assert r["nl_tot"] == rr[2]
+            e = abs(r["uxmax"] - rr[3]) / rr[3]
+            worst = max(worst, e)
+            assert e <= 1e-7
+        else:
+            assert r["nl_tot"] == ctl["nl_maxiter"]
+    print(f"{name}: worst converged-SRF uxmax relative difference {worst:.3e}")
+    bracket = (float(ref[-2, 0]) if ref.shape[0] > 1 else None, float(ref[-1, 0])) \
+        if ref[-1, 2] == ctl["nl_maxiter"] else None
+    assert res.fos_bracket == bracket
+
+
+def test_refined_solve_matches_fp64(ctx):
+    """First plastic iteration = the elastic solve K x = extload from x = 0."""
+    z = golden("slope_8x4x6_layered")
+    mesh = mesh_from_golden(z)
+    spec = hex2spec(mesh)
+    gam, ym, nu, phi, coh, psi = mesh.material_arrays()
+    km, dp, ext_o = O.stiffness_bodyload(spec, gam, nu, ym)
+    f = z["extload"]
+    x_tight, it_t, ec_t = O.pcg_solver(spec.neq, km, np.zeros(spec.neq + 1), f,
+                                       z["dinv"], spec.gdof_elmt, 1e-14, 20000)
+    x_ref, _, _ = O.pcg_solver(spec.neq, km, np.zeros(spec.neq + 1), f, z["dinv"],
+                               spec.gdof_elmt, 1e-8, 5000)
+    dm, ext = device_mesh(ctx, mesh, spec)
+    try:
+        eng = BatchEngine(dm, ext, 1)
+        try:
+            from specfem3d_geotech_amd.plastic import dt_viscoplas
+            eng.begin(0, ym, nu, phi, coh, psi, dt_viscoplas(nu, phi, ym))
+            r = eng.plastic_iter(np.array([0], np.int32), 1e-8, 5000, 1e-5, 5e-4)
+            x, _ = eng.download(0)
+        finally:
+            eng.close()
+    finally:
+        dm.close()
+    e_mixed = rel_l2(x, x_tight)
+    e_fp64 = rel_l2(x_ref, x_tight)
+    print(f"mixed {e_mixed:.3e} (inner its {int(r['cg_iter'][0])}), "
+          f"fp64 cg_tol 1e-8: {e_fp64:.3e}; tight solve its {it_t}")
+    assert ec_t == 0
+    assert e_mixed <= max(1e-8, e_fp64)
