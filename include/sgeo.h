@@ -168,16 +168,18 @@ int sgeo_slope_download(sgeo_slope *slope, double *x, double *bodyload,
 
 /* ---- multi-trial mixed-precision engine ---------------------------------
  * Up to 64 (a power of two) strength-reduction trials on one resident mesh,
- * sharing its elastic stiffness.  Each PCG call of a trial is an iterative
- * refinement: FP64 residual f - K x on the packed FP64 operator, inner
- * Jacobi-PCG in FP32 (packed FP32 operator, FP32 vectors, FP64 dot
- * products) to the relative update inner_tol, x += d, until
- * max|d|/max|x| <= cg_tol -- the reference's relative-update criterion
- * (src/solver.f90:65) applied to the refinement correction.  cg_iter counts
- * the inner iterations.  The yield/return-mapping step stays FP64.  Each
- * column's arithmetic is independent of the others and of ncol, so a trial
- * gives the same bits alone and in any batch.  (No reference counterpart:
- * the reference runs one trial at a time.) */
+ * sharing its elastic stiffness.  Each PCG call of a trial is a Jacobi-PCG
+ * whose iteration runs in FP32 (packed FP32 operator on MFMA, FP32 r, p,
+ * K p) with FP64 dot products and an FP64 solution x.  The FP64 residual
+ * f - K x (packed FP64 operator) starts the solve and replaces the FP32
+ * recursive residual whenever r.z has dropped by inner_tol since the last
+ * replacement; the search direction is kept across replacements.  The solve
+ * stops by the reference's own test |alpha| max|p| / max|x| <= cg_tol
+ * (src/solver.f90:65), so cg_iter counts CG iterations as the reference
+ * does.  The yield/return-mapping step stays FP64.  Each column's
+ * arithmetic is independent of the others and of ncol, so a trial gives the
+ * same bits alone and in any batch.  (No reference counterpart: the
+ * reference runs one trial at a time.) */
 int sgeo_batch_create(sgeo_mesh *mesh, int32_t ncol, const double *extload,
                       sgeo_batch **batch);
 int sgeo_batch_destroy(sgeo_batch *batch);
@@ -200,6 +202,9 @@ int sgeo_batch_plastic_iter(sgeo_batch *batch, const int32_t *is_last,
                             double *uerr, int32_t *nl_isconv, double *fmax);
 int sgeo_batch_download(sgeo_batch *batch, int32_t col, double *x,
                         uint8_t *flags);
+/* Totals since creation: CG iterations of all trials, FP64 residuals of all
+ * trials, and batched steps (one step = one CG iteration of every running
+ * column). */
 int sgeo_batch_stats(sgeo_batch *batch, int64_t *inner_iters,
                      int64_t *outer_iters, int64_t *steps);
 

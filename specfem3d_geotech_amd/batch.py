@@ -78,7 +78,7 @@ class BatchEngine:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
         check(L.lib().sgeo_batch_stats(self.h, C.byref(a), C.byref(b), C.byref(c)),
               "sgeo_batch_stats")
-        return dict(inner_iterations=a.value, refinements=b.value, batched_steps=c.value)
+        return dict(cg_iterations=a.value, fp64_residuals=b.value, batched_steps=c.value)
 
     def nbytes(self):
         b = C.c_size_t()

@@ -4,13 +4,13 @@
 // every CG iteration streams K once for all B columns.
 //
 // Vectors are interleaved [n1][B] (column fastest): an element gathers 81
-// contiguous B-vectors and every update is unit-stride.  Inner CG vectors
-// and the inner element product are FP32; dot products, the refinement
-// residual f - K x (FP64 operator), x and all plastic state are FP64.
-// Every reduction is two-phase (per-block partials, then one block per
-// column) and combines in a fixed order, so a column's arithmetic does not
-// depend on B or on the other columns: a trial gives the same bits alone and
-// in any batch.
+// contiguous B-vectors and every update is unit-stride.  The inner CG
+// vectors (r, p, kp) and the element product are FP32 (the product on
+// MFMA); dot products, the solution x, the FP64 true residual f - K x and
+// all plastic state are FP64.  Every reduction is two-phase (per-chunk
+// partials, then one block per column) and combines in a fixed order, so a
+// column's arithmetic does not depend on B or on the other columns: a trial
+// gives the same bits alone and in any batch.
 #pragma clang fp contract(off)
 
 #include "sgeo_device.h"
@@ -18,73 +18,202 @@
 
 namespace sgeo {
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bool wanted(const int32_t* cst, int j, int mask) {
+  return (mask >> cst[j]) & 1;
+}
+
 // --------------------------------------------------------------------------
-// batched element product: Y[e][r][j] = sum_c K(r,c) P[g(e,c)][j]
+// FP32 batched element product on MFMA:
+//   Y[e][r][j] = sum_c K_e(r,c) P[g(e,c)][j]
+// with v_mfma_f32_16x16x4f32 (A 16x4, B 4x16, D 16x16): rows in 6 tiles of
+// 16 (81 -> 96), columns in tiles of 16 (B < 16: one partial tile), k in 21
+// steps of 4 (81 -> 84).  One element per block of 4 waves; wave w takes
+// column tile n = w % NT and row tiles m = w / NT (+ 4 / NT ...).
+// The packed upper triangle of K_e is unpacked (with its mirror) into LDS
+// as Kf[c * BMV_LDK + r], zero padded; BMV_LDK = 112 puts the four 16-lane
+// groups of an A-operand read (c, c+1, c+2, c+3) on disjoint banks.
+// A column's result depends only on its own P column, so trials are
+// independent of the batch; columns not in `want_mask` are not written.
 // --------------------------------------------------------------------------
-// One element per block.  The packed upper triangle of K_e (KT) and the
-// gathered P_e (81 x B, VT) are staged in LDS; thread t owns column
-// j = t % B and rows r = t / B + k * (nthreads / B).  Rows sum c = 0..80 in
-// order in AT precision.  Columns with cmask[j] != want read zero and write
-// nothing.
-// Columns col0 .. col0+B-1 of arrays with column stride Bs are processed.
-template <typename KT, typename VT, typename AT>
-__global__ __launch_bounds__(256) void k_bmv(int32_t nelmt, int B, int Bs,
-                                             int col0,
-                                             const KT* __restrict__ KP,
-                                             const int32_t* __restrict__ gdof,
-                                             const VT* __restrict__ P,
-                                             VT* __restrict__ Y,
-                                             const int32_t* __restrict__ cst,
-                                             int want) {
+#define BMV_LDK 112
+#define BMV_KC 84
+
+__device__ __forceinline__ int tri_col(int q) {  // c with c(c+1)/2 <= q < (c+1)(c+2)/2
+  int c = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
+  if ((c + 1) * (c + 2) / 2 <= q) c++;
+  if (c * (c + 1) / 2 > q) c--;
+  return c;
+}
+
+// Persistent and software-pipelined: the grid holds a few blocks per CU;
+// each XCD (blocks are dispatched to the 8 XCDs round-robin) walks its own
+// contiguous eighth of the elements, so elements in flight together share
+// nodes and their gathered P rows hit in that XCD's L2.  While element e is
+// multiplied, K and gdof of the block's next element are already in
+// registers, and the next element's P rows are gathered before its K is
+// unpacked.
+#define BMV_XCD 8
+
+template <int NT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+void k_bmv_mfma(int32_t nelmt, int B,
+                                                  const float* __restrict__ KP,
+                                                  const int32_t* __restrict__ gdof,
+                                                  const float* __restrict__ P,
+                                                  float* __restrict__ Y,
+                                                  const int32_t* __restrict__ cst,
+                                                  int want_mask) {
+  constexpr int NS = 4 / NT;             // waves per column tile
+  constexpr int NM = (6 + NS - 1) / NS;  // row tiles per wave
+  constexpr int KPT = (3321 + 255) / 256;  // packed entries per thread
+  __shared__ float Kf[BMV_KC * BMV_LDK];
+  __shared__ int32_t sg[2][BMV_KC];
+  const int t = threadIdx.x;
+  // element range of this block's XCD, and the block's stride within it
+  const int xcd = blockIdx.x % BMV_XCD;
+  const int nbx = gridDim.x / BMV_XCD;  // grid is a multiple of BMV_XCD
+  const int64_t xe0 = (int64_t)nelmt * xcd / BMV_XCD;
+  const int64_t xe1 = (int64_t)nelmt * (xcd + 1) / BMV_XCD;
+  int64_t e = xe0 + blockIdx.x / BMV_XCD;
+  // column tile of this wave; columns outside the mask are neither
+  // gathered nor multiplied nor stored, but the wave keeps to the block's
+  // barriers
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int l = t & 63;
+  const int n = w % NT, mpart = w / NT;
+  const int j = n * 16 + (l & 15);
+  const bool jv = j < B && wanted(cst, j, want_mask);
+  const bool wave_on = __any(jv);
+  const int jl = j < B ? j : B - 1;  // lanes past B read column B-1, store nothing
+  const int kq = l >> 4;
+  // zero padding, once: rows 81..95 of every k column, and k columns 81..83
+  for (int q = t; q < BMV_KC * 15; q += 256)
+    Kf[(q / 15) * BMV_LDK + 81 + q % 15] = 0.0f;
+  for (int q = t; q < 3 * 81; q += 256) Kf[(81 + q / 81) * BMV_LDK + q % 81] = 0.0f;
+  if (t >= 81 && t < BMV_KC) { sg[0][t] = 0; sg[1][t] = 0; }
+  // prologue: K and gdof of the first element
+  float kreg[KPT];
+  int32_t greg = 0;
+  if (e < xe1) {
+    const float* src = KP + e * SGEO_KPACK;
+#pragma unroll
+    for (int i = 0; i < KPT; i++) {
+      const int q = t + 256 * i;
+      kreg[i] = q < 3321 ? src[q] : 0.0f;
+    }
+    if (t < 81) greg = gdof[e * 81 + t];
+  }
+  if (t < 81) sg[0][t] = greg;
+  __syncthreads();
+  float bv[21];
+  if (wave_on && e < xe1) {
+#pragma unroll
+    for (int s = 0; s < 21; s++) bv[s] = P[(uint32_t)(sg[0][4 * s + kq] * B + jl)];
+  }
+  int cur = 0;
+  for (; e < xe1; e += nbx) {
+    // unpack the packed upper triangle (and its mirror) of element e
+#pragma unroll
+    for (int i = 0; i < KPT; i++) {
+      const int q = t + 256 * i;
+      if (q < 3321) {
+        const int c = tri_col(q);
+        const int r = q - c * (c + 1) / 2;
+        Kf[c * BMV_LDK + r] = kreg[i];
+        Kf[r * BMV_LDK + c] = kreg[i];
+      }
+    }
+    __syncthreads();
+    const int64_t en = e + nbx;
+    if (en < xe1) {  // next element's K and gdof, in flight during the MFMAs
+      const float* src = KP + en * SGEO_KPACK;
+#pragma unroll
+      for (int i = 0; i < KPT; i++) {
+        const int q = t + 256 * i;
+        kreg[i] = q < 3321 ? src[q] : 0.0f;
+      }
+      if (t < 81) greg = gdof[en * 81 + t];
+    }
+    if (wave_on) {
+      f32x4 acc[NM];
+#pragma unroll
+      for (int i = 0; i < NM; i++) acc[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int s = 0; s < 21; s++) {
+        const float* kc = Kf + (4 * s + kq) * BMV_LDK + (l & 15);
+#pragma unroll
+        for (int i = 0; i < NM; i++) {
+          const int mt = mpart + i * NS;  // wave-uniform
+          if (NM * NS > 6 && mt >= 6) break;
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(kc[mt * 16], bv[s], acc[i], 0, 0,
+                                                        0);
+        }
+      }
+      if (jv) {
+#pragma unroll
+        for (int i = 0; i < NM; i++) {
+          const int mt = mpart + i * NS;
+          if (NM * NS > 6 && mt >= 6) break;
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int row = mt * 16 + 4 * kq + v;
+            if (row < 81) Y[(e * 81 + row) * B + j] = acc[i][v];
+          }
+        }
+      }
+    }
+    if (t < 81) sg[cur ^ 1][t] = greg;
+    __syncthreads();  // Kf free for the next element, its gdof visible
+    cur ^= 1;
+    if (wave_on && en < xe1) {
+#pragma unroll
+      for (int s = 0; s < 21; s++) bv[s] = P[(uint32_t)(sg[cur][4 * s + kq] * B + jl)];
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// FP64 batched element product (the true residual's K x): one element per
+// block; the packed upper triangle of K_e and the gathered X_e (81 x B) in
+// LDS; thread t owns column j = t % B and rows t / B + k * (nthreads / B),
+// each summed over c = 0..80 in order.  Columns col0 .. col0+B-1 of arrays
+// with column stride Bs; columns not in want_mask read zero, write nothing.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bmv64(int32_t nelmt, int B, int Bs, int col0,
+                                               const double* __restrict__ KP,
+                                               const int32_t* __restrict__ gdof,
+                                               const double* __restrict__ X,
+                                               double* __restrict__ Y,
+                                               const int32_t* __restrict__ cst,
+                                               int want_mask) {
   extern __shared__ __align__(16) unsigned char smem[];
-  KT* u = reinterpret_cast<KT*>(smem);
-  VT* pe = reinterpret_cast<VT*>(smem + SGEO_KPACK * sizeof(KT));
-  const int64_t e = blockIdx.x;
+  double* u = reinterpret_cast<double*>(smem);
+  double* pe = u + SGEO_KPACK;
   const int t = threadIdx.x, nt = blockDim.x;
-  const KT* src = KP + e * SGEO_KPACK;
+  // skip the element entirely when no column of this launch is wanted
+  bool anyw = false;
+  for (int q = 0; q < B; q++) anyw = anyw || wanted(cst, col0 + q, want_mask);
+  if (!anyw) return;  // uniform per block
+  const int64_t e = blockIdx.x;
+  const double* src = KP + e * SGEO_KPACK;
   for (int q = t; q < SGEO_KPACK; q += nt) u[q] = src[q];
   for (int q = t; q < 81 * B; q += nt) {
     const int c = q / B, j = q - c * B;
-    pe[q] = (cst[col0 + j] == want) ? P[(int64_t)gdof[e * 81 + c] * Bs + col0 + j]
-                                    : (VT)0;
+    pe[q] = wanted(cst, col0 + j, want_mask)
+                ? X[(int64_t)gdof[e * 81 + c] * Bs + col0 + j] : 0.0;
   }
   __syncthreads();
   const int j = t % B;
   const int nrg = nt / B, rg = t / B;
-  if (rg >= nrg || cst[col0 + j] != want) return;
-  constexpr int MAXR = 81;
-  AT acc[MAXR / 4 + 1];
-  const int nr = (81 - rg + nrg - 1) / nrg;  // rows rg, rg+nrg, ...
-#pragma unroll
-  for (int k = 0; k < MAXR / 4 + 1; k++) acc[k] = (AT)0;
-  if (nr <= MAXR / 4 + 1) {
-    for (int c = 0; c < 81; c++) {
-      const AT pv = (AT)pe[c * B + j];
-      const int cb = c * (c + 1) / 2;
-#pragma unroll
-      for (int k = 0; k < MAXR / 4 + 1; k++) {
-        const int r = rg + k * nrg;
-        if (k < nr) {
-          const int idx = (r <= c) ? (cb + r) : (r * (r + 1) / 2 + c);
-          acc[k] = acc[k] + (AT)u[idx] * pv;
-        }
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < MAXR / 4 + 1; k++) {
-      const int r = rg + k * nrg;
-      if (k < nr) Y[(e * 81 + r) * Bs + col0 + j] = (VT)acc[k];
-    }
-  } else {
-    // fewer than 4 row groups (B > 64 / nt): rows one by one
-    for (int r = rg; r < 81; r += nrg) {
-      AT a = (AT)0;
-      for (int c = 0; c < 81; c++) {
-        const int idx = (r <= c) ? (c * (c + 1) / 2 + r) : (r * (r + 1) / 2 + c);
-        a = a + (AT)u[idx] * (AT)pe[c * B + j];
-      }
-      Y[(e * 81 + r) * Bs + col0 + j] = (VT)a;
-    }
+  if (!wanted(cst, col0 + j, want_mask)) return;
+  for (int r = rg; r < 81; r += nrg) {
+    const int rb = r * (r + 1) / 2;
+    double a = 0.0;
+    for (int c = 0; c < r; c++) a = a + u[rb + c] * pe[c * B + j];
+    for (int c = r; c < 81; c++) a = a + u[c * (c + 1) / 2 + r] * pe[c * B + j];
+    Y[(e * 81 + r) * Bs + col0 + j] = a;
   }
 }
 
@@ -119,21 +248,57 @@ __device__ __forceinline__ int64_t nchunks(int64_t n1) {
   return (n1 + BCHUNK - 1) / BCHUNK;
 }
 
+// p = D r + beta p for INNER columns with a pending beta
+__global__ __launch_bounds__(256) void k_bpupdate(int64_t n1, int B,
+                                                  const float* __restrict__ r,
+                                                  const float* __restrict__ dinv,
+                                                  float* __restrict__ p,
+                                                  const int32_t* __restrict__ cst,
+                                                  const BCol* __restrict__ bc) {
+  const ChunkIdx c = chunk_of(n1, B);
+  if (!c.valid || cst[c.j] != BCOL_INNER || !bc[c.j].pending) return;
+  const float beta = (float)bc[c.j].beta;
+  for (int64_t g = c.g0; g < c.g1; g++) {
+    const int64_t q = g * B + c.j;
+    p[q] = dinv[g] * r[q] + beta * p[q];
+  }
+}
+
 // kp[g][j] = sum over slots of Y (element order, FP32); partial p.kp (FP64)
-// and max|p| for columns in state INNER
+// and max|p| for INNER columns.  The (at most 8 on hexahedral meshes)
+// element contributions of an equation are loaded together, then summed in
+// slot order.  WAVE_CHUNK: B = 64, so a wave's 64 lanes are the 64 columns
+// of one chunk and the index loads (rowptr, slots) are wave-uniform scalar
+// loads.
+template <bool WAVE_CHUNK>
 __global__ __launch_bounds__(256) void k_bassemble_dot(
     int64_t n1, int B, const int32_t* __restrict__ rowptr,
     const int32_t* __restrict__ slots, const float* __restrict__ Y,
     const float* __restrict__ p, float* __restrict__ kp,
     double* __restrict__ part, const int32_t* __restrict__ cst) {
-  const ChunkIdx c = chunk_of(n1, B);
+  ChunkIdx c = chunk_of(n1, B);
+  if (WAVE_CHUNK) {
+    const int ch = __builtin_amdgcn_readfirstlane((int)c.chunk);
+    c.chunk = ch;
+    c.g0 = (int64_t)ch * BCHUNK;
+    c.g1 = c.g0 + BCHUNK < n1 ? c.g0 + BCHUNK : n1;
+    c.valid = c.g0 < n1;
+  }
   if (!c.valid || cst[c.j] != BCOL_INNER) return;
   double s = 0.0, m = 0.0;
   for (int64_t g = c.g0; g < c.g1; g++) {
     float acc = 0.0f;
-    if (g > 0)
-      for (int32_t q = rowptr[g]; q < rowptr[g + 1]; q++)
-        acc = acc + Y[(int64_t)slots[q] * B + c.j];
+    if (g > 0) {
+      const int32_t q0 = rowptr[g], q1 = rowptr[g + 1];
+      float v[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        v[k] = (q0 + k < q1) ? Y[(int64_t)slots[q0 + k] * B + c.j] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (q0 + k < q1) acc = acc + v[k];
+      for (int32_t q = q0 + 8; q < q1; q++) acc = acc + Y[(int64_t)slots[q] * B + c.j];
+    }
     kp[g * B + c.j] = acc;
     const double pg = (double)p[g * B + c.j];
     s = s + pg * (double)acc;
@@ -144,60 +309,46 @@ __global__ __launch_bounds__(256) void k_bassemble_dot(
   part[(1 * nch + c.chunk) * B + c.j] = m;
 }
 
-// d += alpha p; r -= alpha kp; z = dinv r; partial max|d| and r.z (FP64)
+// x += alpha p (FP64); r -= alpha kp; partial max|x| (after the update, with
+// x(0) = 0) and r.(D r) (FP64)
 __global__ __launch_bounds__(256) void k_bupdate(
     int64_t n1, int B, const float* __restrict__ p, const float* __restrict__ kp,
-    const float* __restrict__ dinv, float* __restrict__ d, float* __restrict__ r,
-    float* __restrict__ z, double* __restrict__ part,
-    const int32_t* __restrict__ cst, const BCol* __restrict__ bc) {
+    const float* __restrict__ dinv, double* __restrict__ x, float* __restrict__ r,
+    double* __restrict__ part, const int32_t* __restrict__ cst,
+    const BCol* __restrict__ bc) {
   const ChunkIdx c = chunk_of(n1, B);
   if (!c.valid || cst[c.j] != BCOL_INNER) return;
-  const float alpha = (float)bc[c.j].alpha;
-  double md = 0.0, s = 0.0;
+  const double alpha = bc[c.j].alpha;
+  const float alpha32 = (float)alpha;
+  double mx = 0.0, s = 0.0;
   for (int64_t g = c.g0; g < c.g1; g++) {
     const int64_t q = g * B + c.j;
-    const float dg = d[q] + alpha * p[q];
-    d[q] = dg;
-    md = fmax(md, fabs((double)dg));
-    const float rg = r[q] - alpha * kp[q];
-    const float zg = dinv[g] * rg;
+    const double xg = (g == 0) ? 0.0 : x[q] + alpha * (double)p[q];
+    x[q] = xg;
+    mx = fmax(mx, fabs(xg));
+    const float rg = r[q] - alpha32 * kp[q];
     r[q] = rg;
-    z[q] = zg;
+    const float zg = dinv[g] * rg;
     s = s + (double)rg * (double)zg;
   }
   const int64_t nch = nchunks(n1);
-  part[(0 * nch + c.chunk) * B + c.j] = md;
+  part[(0 * nch + c.chunk) * B + c.j] = mx;
   part[(1 * nch + c.chunk) * B + c.j] = s;
 }
 
-// p = z + beta p
-__global__ __launch_bounds__(256) void k_bpupdate(int64_t n1, int B,
-                                                  const float* __restrict__ z,
-                                                  float* __restrict__ p,
-                                                  const int32_t* __restrict__ cst,
-                                                  const BCol* __restrict__ bc) {
-  const int64_t n = n1 * B;
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
-       q += (int64_t)gridDim.x * blockDim.x) {
-    const int j = (int)(q % B);
-    if (cst[j] != BCOL_INNER) continue;
-    p[q] = z[q] + (float)bc[j].beta * p[q];
-  }
-}
-
-// refinement residual and inner start, columns in NEED_RESID:
-// r64 = load - sum Y64 (element order, FP64); r = (float) r64;
-// z = dinv r; p = z; d = 0; partial r.z and max|r64|
+// FP64 residual for START / REPLACE columns: r64 = load - sum Y64 (element
+// order, FP64); r = (float) r64; START: p = D r; partial r.(D r)
 __global__ __launch_bounds__(256) void k_bresid(
     int64_t n1, int B, const int32_t* __restrict__ rowptr,
     const int32_t* __restrict__ slots, const double* __restrict__ Y64,
     const double* __restrict__ load, const float* __restrict__ dinv,
-    float* __restrict__ r, float* __restrict__ z, float* __restrict__ p,
-    float* __restrict__ d, double* __restrict__ part,
+    float* __restrict__ r, float* __restrict__ p, double* __restrict__ part,
     const int32_t* __restrict__ cst) {
   const ChunkIdx c = chunk_of(n1, B);
-  if (!c.valid || cst[c.j] != BCOL_NEED_RESID) return;
-  double s = 0.0, m = 0.0;
+  if (!c.valid) return;
+  const int st = cst[c.j];
+  if (st != BCOL_START && st != BCOL_REPLACE) return;
+  double s = 0.0;
   for (int64_t g = c.g0; g < c.g1; g++) {
     const int64_t q = g * B + c.j;
     double kx = 0.0;
@@ -208,57 +359,69 @@ __global__ __launch_bounds__(256) void k_bresid(
     const float rg = (float)r64;
     const float zg = dinv[g] * rg;
     r[q] = rg;
-    z[q] = zg;
-    p[q] = zg;
-    d[q] = 0.0f;
+    if (st == BCOL_START) p[q] = zg;
     s = s + (double)rg * (double)zg;
-    m = fmax(m, fabs(r64));
   }
   const int64_t nch = nchunks(n1);
   part[(0 * nch + c.chunk) * B + c.j] = s;
-  part[(1 * nch + c.chunk) * B + c.j] = m;
+  part[(1 * nch + c.chunk) * B + c.j] = 0.0;
 }
 
-// x += d for columns in INNER_DONE; partial max|d| and max|x| (after update)
-__global__ __launch_bounds__(256) void k_bxupdate(
-    int64_t n1, int B, const float* __restrict__ d, double* __restrict__ x,
-    double* __restrict__ part, const int32_t* __restrict__ cst) {
-  const ChunkIdx c = chunk_of(n1, B);
-  if (!c.valid || cst[c.j] != BCOL_INNER_DONE) return;
-  double md = 0.0, mx = 0.0;
-  for (int64_t g = c.g0; g < c.g1; g++) {
-    const int64_t q = g * B + c.j;
-    const double dg = (g == 0) ? 0.0 : (double)d[q];
-    const double xg = x[q] + dg;
-    x[q] = xg;
-    md = fmax(md, fabs(dg));
-    mx = fmax(mx, fabs(xg));
+// Reductions of the chunk partials, in a fixed order independent of B:
+// stage 1 combines groups of BGROUP consecutive chunks per column (thread
+// per (group, column), column fastest: coalesced), stage 2 (one block per
+// column) combines the groups and advances the column's state machine.
+#define BGROUP 64
+
+__device__ __forceinline__ int fin_states(int what) {
+  return what == BFIN_RESID ? (BCOL_BIT(BCOL_START) | BCOL_BIT(BCOL_REPLACE))
+                            : BCOL_BIT(BCOL_INNER);
+}
+
+// partial kinds: RESID (sum r.z, -), ALPHA (sum p.kp, max |p|),
+// INNER (max |x|, sum r.z)
+__device__ __forceinline__ bool fin_max0(int what) { return what == BFIN_INNER; }
+__device__ __forceinline__ bool fin_max1(int what) { return what == BFIN_ALPHA; }
+
+__global__ __launch_bounds__(256) void k_bgroup(int64_t nch, int B,
+                                                const double* __restrict__ part,
+                                                double* __restrict__ part2,
+                                                const int32_t* __restrict__ cst,
+                                                int what) {
+  const int64_t ng = (nch + BGROUP - 1) / BGROUP;
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= ng * B) return;
+  const int j = (int)(q % B);
+  const int64_t g = q / B;
+  if (!wanted(cst, j, fin_states(what))) return;
+  const bool max0 = fin_max0(what), max1 = fin_max1(what);
+  const int64_t b0 = g * BGROUP, b1 = b0 + BGROUP < nch ? b0 + BGROUP : nch;
+  double v0 = 0.0, v1 = 0.0;
+  for (int64_t b = b0; b < b1; b++) {
+    const double a0 = part[(0 * nch + b) * B + j];
+    const double a1 = part[(1 * nch + b) * B + j];
+    v0 = max0 ? fmax(v0, a0) : v0 + a0;
+    v1 = max1 ? fmax(v1, a1) : v1 + a1;
   }
-  const int64_t nch = nchunks(n1);
-  part[(0 * nch + c.chunk) * B + c.j] = md;
-  part[(1 * nch + c.chunk) * B + c.j] = mx;
+  part2[(0 * ng + g) * B + j] = v0;
+  part2[(1 * ng + g) * B + j] = v1;
 }
 
-// one block per column: combine the chunk partials of column j in a fixed
-// order and advance the column's state machine
 __global__ __launch_bounds__(256) void k_bfinalize(int64_t nch, int B,
-                                                   const double* __restrict__ part,
+                                                   const double* __restrict__ part2,
                                                    int32_t* __restrict__ cst,
                                                    BCol* __restrict__ bc,
                                                    int what) {
   __shared__ double red[4];
   const int j = blockIdx.x;
   const int st = cst[j];
-  const bool mine = (what == BFIN_RZ && st == BCOL_NEED_RESID) ||
-                    ((what == BFIN_ALPHA || what == BFIN_INNER) && st == BCOL_INNER) ||
-                    (what == BFIN_OUTER && st == BCOL_INNER_DONE);
-  if (!mine) return;  // uniform per block
-  const bool max0 = (what == BFIN_INNER || what == BFIN_OUTER);
-  const bool max1 = (what != BFIN_INNER);
+  if (!wanted(cst, j, fin_states(what))) return;  // uniform per block
+  const int64_t ng = (nch + BGROUP - 1) / BGROUP;
+  const bool max0 = fin_max0(what), max1 = fin_max1(what);
   double v0 = 0.0, v1 = 0.0;
-  for (int64_t b = threadIdx.x; b < nch; b += 256) {
-    const double a0 = part[(0 * nch + b) * B + j];
-    const double a1 = part[(1 * nch + b) * B + j];
+  for (int64_t b = threadIdx.x; b < ng; b += 256) {
+    const double a0 = part2[(0 * ng + b) * B + j];
+    const double a1 = part2[(1 * ng + b) * B + j];
     v0 = max0 ? fmax(v0, a0) : v0 + a0;
     v1 = max1 ? fmax(v1, a1) : v1 + a1;
   }
@@ -267,36 +430,44 @@ __global__ __launch_bounds__(256) void k_bfinalize(int64_t nch, int B,
   if (threadIdx.x != 0) return;
   BCol& c = bc[j];
   switch (what) {
-    case BFIN_RZ:  // v0 = r.z, v1 = max|r64|
-      c.rz = v0;
-      c.inner_it = 0;
-      c.outer_it += 1;
-      cst[j] = (v1 == 0.0) ? BCOL_INNER_DONE : BCOL_INNER;  // exact: d = 0
+    case BFIN_RESID:  // v0 = r.z of the FP64 residual
+      c.nrep += 1;
+      c.req = 0;
+      if (st == BCOL_START) {
+        c.rz = v0;
+        c.rz0 = v0;
+        c.it = 0;
+        c.pending = 0;
+        cst[j] = (v0 == 0.0) ? BCOL_DONE : BCOL_INNER;  // r = 0: x is exact
+      } else {  // replaces r_{k+1}: beta = r.z (true) / r_k.z_k
+        c.beta = v0 / c.rz_prev;
+        c.rz = v0;
+        c.rz0 = v0;
+        c.pending = 1;
+        cst[j] = BCOL_INNER;
+      }
       break;
     case BFIN_ALPHA:  // v0 = p.kp, v1 = max|p|
       c.maxp = v1;
       c.alpha = c.rz / v0;
+      c.pending = 0;
       break;
-    case BFIN_INNER: {  // v0 = max|d|, v1 = r.z
-      c.inner_it += 1;
-      c.cg_total += 1;
-      const bool stop = fabs(c.alpha) * c.maxp / v0 <= c.inner_tol;
-      if (stop || c.inner_it >= c.inner_max || c.cg_total >= c.cg_maxiter) {
-        cst[j] = BCOL_INNER_DONE;
+    case BFIN_INNER: {  // v0 = max|x|, v1 = r.z
+      c.it += 1;
+      // relative update test of src/solver.f90:65 (after the x update)
+      if (fabs(c.alpha) * c.maxp / v0 <= c.tol) {
+        cst[j] = BCOL_DONE;
+      } else if (c.it >= c.maxiter) {
+        cst[j] = BCOL_FAIL;
       } else {
+        c.rz_prev = c.rz;
         c.beta = v1 / c.rz;
         c.rz = v1;
+        c.pending = 1;
+        if (v1 <= c.inner_tol * c.rz0) c.req = 1;
       }
       break;
     }
-    case BFIN_OUTER:  // v0 = max|d|, v1 = max|x|
-      c.maxd = v0;
-      c.maxx = v1;
-      if (v0 / v1 <= c.tol) cst[j] = BCOL_DONE;
-      else if (c.cg_total >= c.cg_maxiter || c.outer_it >= c.outer_max)
-        cst[j] = BCOL_FAIL;
-      else cst[j] = BCOL_NEED_RESID;
-      break;
   }
 }
 
@@ -415,7 +586,10 @@ static inline int64_t nchunks_host(int64_t n1) {
   return (n1 + BCHUNK - 1) / BCHUNK;
 }
 
-int64_t batch_partials(int64_t n1) { return 2 * ((n1 + BCHUNK - 1) / BCHUNK); }
+int64_t batch_partials(int64_t n1) {
+  const int64_t nch = (n1 + BCHUNK - 1) / BCHUNK;
+  return 2 * nch + 2 * ((nch + BGROUP - 1) / BGROUP);  // chunk + group partials
+}
 
 static inline unsigned chunk_blocks(int64_t n1, int B) {
   const int64_t per = 256 / B;  // chunks per block
@@ -429,64 +603,82 @@ static inline int bmv_threads(int B) {
   return t < 64 ? 64 : t;
 }
 
-void launch_bmv_f32(hipStream_t s, int32_t nelmt, int B, const float* KP,
-                    const int32_t* gdof, const float* P, float* Y,
-                    const int32_t* cst, int want) {
-  const size_t lds = SGEO_KPACK * sizeof(float) + (size_t)81 * B * sizeof(float);
-  hipLaunchKernelGGL((k_bmv<float, float, float>), dim3(nelmt), dim3(bmv_threads(B)),
-                     lds, s, nelmt, B, B, 0, KP, gdof, P, Y, cst, want);
+// blocks per CU of the persistent MFMA product (3 waves per SIMD: the
+// kernel is built for at most 168 VGPRs)
+#define BMV_BLOCKS_PER_CU 3
+
+void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu,
+                         const float* KP, const int32_t* gdof, const float* P,
+                         float* Y, const int32_t* cst, int want_mask) {
+  int64_t g = (int64_t)n_cu * BMV_BLOCKS_PER_CU;
+  const int64_t need = ((int64_t)nelmt + BMV_XCD - 1) / BMV_XCD * BMV_XCD;
+  if (g > need) g = need;
+  g = (g + BMV_XCD - 1) / BMV_XCD * BMV_XCD;  // a multiple of the XCD count
+  const dim3 grid((unsigned)g);
+  if (B >= 64)
+    hipLaunchKernelGGL(k_bmv_mfma<4>, grid, dim3(256), 0, s, nelmt, B, KP, gdof, P,
+                       Y, cst, want_mask);
+  else if (B == 32)
+    hipLaunchKernelGGL(k_bmv_mfma<2>, grid, dim3(256), 0, s, nelmt, B, KP, gdof, P,
+                       Y, cst, want_mask);
+  else
+    hipLaunchKernelGGL(k_bmv_mfma<1>, grid, dim3(256), 0, s, nelmt, B, KP, gdof, P,
+                       Y, cst, want_mask);
 }
 
 // FP64: columns in chunks of at most 32 (LDS <= 48 KB per block)
 void launch_bmv_f64(hipStream_t s, int32_t nelmt, int B, const double* KP,
                     const int32_t* gdof, const double* X, double* Y,
-                    const int32_t* cst, int want) {
+                    const int32_t* cst, int want_mask) {
   const int Bc = B < 32 ? B : 32;
   const size_t lds = SGEO_KPACK * sizeof(double) + (size_t)81 * Bc * sizeof(double);
   for (int c0 = 0; c0 < B; c0 += Bc)
-    hipLaunchKernelGGL((k_bmv<double, double, double>), dim3(nelmt),
-                       dim3(bmv_threads(Bc)), lds, s, nelmt, Bc, B, c0, KP, gdof,
-                       X, Y, cst, want);
+    hipLaunchKernelGGL(k_bmv64, dim3(nelmt), dim3(bmv_threads(Bc)), lds, s, nelmt,
+                       Bc, B, c0, KP, gdof, X, Y, cst, want_mask);
 }
 
 void launch_bassemble_dot(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
                           const int32_t* slots, const float* Y, const float* p,
                           float* kp, double* part, const int32_t* cst) {
-  hipLaunchKernelGGL(k_bassemble_dot, dim3(chunk_blocks(n1, B)), dim3(256), 0, s,
-                     n1, B, rowptr, slots, Y, p, kp, part, cst);
+  if (B == 64)
+    hipLaunchKernelGGL(k_bassemble_dot<true>, dim3(chunk_blocks(n1, B)), dim3(256), 0,
+                       s, n1, B, rowptr, slots, Y, p, kp, part, cst);
+  else
+    hipLaunchKernelGGL(k_bassemble_dot<false>, dim3(chunk_blocks(n1, B)), dim3(256), 0,
+                       s, n1, B, rowptr, slots, Y, p, kp, part, cst);
 }
 
 void launch_bupdate(hipStream_t s, int64_t n1, int B, const float* p,
-                    const float* kp, const float* dinv, float* d, float* r,
-                    float* z, double* part, const int32_t* cst, const BCol* bc) {
+                    const float* kp, const float* dinv, double* x, float* r,
+                    double* part, const int32_t* cst, const BCol* bc) {
   hipLaunchKernelGGL(k_bupdate, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
-                     p, kp, dinv, d, r, z, part, cst, bc);
+                     p, kp, dinv, x, r, part, cst, bc);
 }
 
-void launch_bpupdate(hipStream_t s, int64_t n1, int B, const float* z, float* p,
-                     const int32_t* cst, const BCol* bc) {
-  hipLaunchKernelGGL(k_bpupdate, dim3(grid_stride(n1 * B)), dim3(256), 0, s, n1,
-                     B, z, p, cst, bc);
+void launch_bpupdate(hipStream_t s, int64_t n1, int B, const float* r,
+                     const float* dinv, float* p, const int32_t* cst,
+                     const BCol* bc) {
+  hipLaunchKernelGGL(k_bpupdate, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
+                     r, dinv, p, cst, bc);
 }
 
 void launch_bresid(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
                    const int32_t* slots, const double* Y64, const double* load,
-                   const float* dinv, float* r, float* z, float* p, float* d,
-                   double* part, const int32_t* cst) {
+                   const float* dinv, float* r, float* p, double* part,
+                   const int32_t* cst) {
   hipLaunchKernelGGL(k_bresid, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
-                     rowptr, slots, Y64, load, dinv, r, z, p, d, part, cst);
+                     rowptr, slots, Y64, load, dinv, r, p, part, cst);
 }
 
-void launch_bxupdate(hipStream_t s, int64_t n1, int B, const float* d, double* x,
-                     double* part, const int32_t* cst) {
-  hipLaunchKernelGGL(k_bxupdate, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1,
-                     B, d, x, part, cst);
-}
-
-void launch_bfinalize(hipStream_t s, int64_t n1, int B, const double* part,
+void launch_bfinalize(hipStream_t s, int64_t n1, int B, double* part,
                       int32_t* cst, BCol* bc, int what) {
-  hipLaunchKernelGGL(k_bfinalize, dim3(B), dim3(256), 0, s, nchunks_host(n1), B,
-                     part, cst, bc, what);
+  const int64_t nch = nchunks_host(n1);
+  const int64_t ng = (nch + BGROUP - 1) / BGROUP;
+  double* part2 = part + 2 * nch * B;
+  hipLaunchKernelGGL(k_bgroup, dim3((unsigned)((ng * B + 255) / 256)), dim3(256), 0,
+                     s, nch, B, part, part2, cst, what);
+  hipLaunchKernelGGL(k_bfinalize, dim3(B), dim3(256), 0, s, nch, B, part2, cst, bc,
+                     what);
 }
 
 void launch_bload(hipStream_t s, int64_t n1, int B, const double* ext,

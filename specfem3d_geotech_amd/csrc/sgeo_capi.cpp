@@ -47,6 +47,7 @@ extern "C" const char* sgeo_last_error(void) { return g_err.c_str(); }
 // --------------------------------------------------------------------------
 struct sgeo_ctx {
   int device = 0;
+  int n_cu = 0;  // compute units of the device
   hipStream_t stream = nullptr;
   PcgScalars* d_sc = nullptr;
   PcgScalars* h_sc = nullptr;  // pinned
@@ -167,6 +168,8 @@ extern "C" int sgeo_init(int device, sgeo_ctx** out) {
   HIPCHK(hipSetDevice(device));
   sgeo_ctx* c = new sgeo_ctx();
   c->device = device;
+  HIPCHK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
+  if (c->n_cu < 1) c->n_cu = 1;
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   TRY(dalloc(&c->d_sc, 1));
   TRY(dalloc(&c->d_ps, 1));
@@ -844,8 +847,8 @@ struct sgeo_batch {
   int64_t n1 = 0;
   double *d_x = nullptr, *d_oldx = nullptr, *d_load = nullptr, *d_body = nullptr;
   double *d_Y64 = nullptr, *d_ext = nullptr;
-  float *d_r = nullptr, *d_z = nullptr, *d_p = nullptr, *d_kp = nullptr;
-  float *d_d = nullptr, *d_Y32 = nullptr, *d_dinv32 = nullptr;
+  float *d_r = nullptr, *d_p = nullptr, *d_kp = nullptr;
+  float *d_Y32 = nullptr, *d_dinv32 = nullptr;
   double *d_evpt = nullptr, *d_stress = nullptr, *d_BL = nullptr;
   uint8_t* d_flags = nullptr;
   MatParams* d_mp = nullptr;
@@ -909,10 +912,8 @@ extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extlo
   TRY(dalloc(&b->d_Y64, ny, acc));
   TRY(dalloc(&b->d_ext, (size_t)b->n1, acc));
   TRY(dalloc(&b->d_r, nv, acc));
-  TRY(dalloc(&b->d_z, nv, acc));
   TRY(dalloc(&b->d_p, nv, acc));
   TRY(dalloc(&b->d_kp, nv, acc));
-  TRY(dalloc(&b->d_d, nv, acc));
   TRY(dalloc(&b->d_Y32, ny, acc));
   TRY(dalloc(&b->d_dinv32, (size_t)b->n1, acc));
   TRY(dalloc(&b->d_evpt, ns, acc));
@@ -951,7 +952,7 @@ extern "C" int sgeo_batch_destroy(sgeo_batch* b) {
   hipSetDevice(b->mesh->ctx->device);
   hipStreamSynchronize(b->mesh->ctx->stream);
   void* ptrs[] = {b->d_x, b->d_oldx, b->d_load, b->d_body, b->d_Y64, b->d_ext,
-                  b->d_r, b->d_z, b->d_p, b->d_kp, b->d_d, b->d_Y32,
+                  b->d_r, b->d_p, b->d_kp, b->d_Y32,
                   b->d_dinv32, b->d_evpt, b->d_stress, b->d_BL, b->d_flags,
                   b->d_mp, b->d_ps, b->d_bc, b->d_cst, b->d_active, b->d_islast,
                   b->d_cols, b->d_part};
@@ -1018,7 +1019,7 @@ extern "C" int sgeo_batch_release(sgeo_batch* b, int32_t col) {
 }
 
 // One plastic iteration of every active column in lockstep: load assembly,
-// refinement PCG (FP64 residual, FP32 inner CG, per-column masking), uerr,
+// mixed-precision PCG (FP32 inner CG, FP64 residual replacement), uerr,
 // element x GLL body and body-load assembly (src/slope.F90:286-391).
 extern "C" int sgeo_batch_plastic_iter(sgeo_batch* b, const int32_t* is_last,
                                        double cg_tol, int32_t cg_maxiter,
@@ -1038,15 +1039,13 @@ extern "C" int sgeo_batch_plastic_iter(sgeo_batch* b, const int32_t* is_last,
     if (!b->active[j]) continue;
     cols.push_back(j);
     islast[j] = is_last[j] ? 1 : 0;
-    cst[j] = b->memo_pending[j] ? BCOL_DONE : BCOL_NEED_RESID;
+    cst[j] = b->memo_pending[j] ? BCOL_DONE : BCOL_START;
     BCol& bc = b->h_bc[j];
     bc = BCol{};
     bc.tol = cg_tol;
     bc.inner_tol = inner_tol;
-    bc.inner_max = cg_maxiter;
-    bc.outer_max = 50;
-    bc.cg_maxiter = cg_maxiter;
-    bc.cg_total = b->memo_pending[j] ? b->memo_cg[j] : 0;
+    bc.maxiter = cg_maxiter;
+    bc.it = b->memo_pending[j] ? b->memo_cg[j] : 0;
   }
   if (cols.empty()) return fail(SGEO_ESTATE, "no active column");
   TRY(batch_sync_operator(b));
@@ -1057,45 +1056,55 @@ extern "C" int sgeo_batch_plastic_iter(sgeo_batch* b, const int32_t* is_last,
   HIPCHK(hipMemcpyAsync(b->d_cst, cst.data(), B * 4, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(b->d_bc, b->h_bc, B * sizeof(BCol), hipMemcpyHostToDevice, s));
   launch_bload(s, n1, B, b->d_ext, b->d_body, b->d_load, b->d_active);
-  // refinement PCG
-  for (int outer = 0; outer < 64; outer++) {
-    bool need = false;
-    for (int j : cols) need = need || cst[j] == BCOL_NEED_RESID;
-    if (!need) break;
-    launch_bmv_f64(s, m->nelmt, B, m->d_KP64, m->d_gdof, b->d_x, b->d_Y64, b->d_cst,
-                   BCOL_NEED_RESID);
-    launch_bresid(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y64, b->d_load,
-                  b->d_dinv32, b->d_r, b->d_z, b->d_p, b->d_d, b->d_part, b->d_cst);
-    launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_RZ);
-    b->stat_outer++;
-    int chunk = 8;
-    for (;;) {
-      for (int it = 0; it < chunk; it++) {
-        launch_bmv_f32(s, m->nelmt, B, m->d_KP32, m->d_gdof, b->d_p, b->d_Y32,
-                       b->d_cst, BCOL_INNER);
-        launch_bassemble_dot(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y32, b->d_p,
-                             b->d_kp, b->d_part, b->d_cst);
-        launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_ALPHA);
-        launch_bupdate(s, n1, B, b->d_p, b->d_kp, b->d_dinv32, b->d_d, b->d_r,
-                       b->d_z, b->d_part, b->d_cst, b->d_bc);
-        launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_INNER);
-        launch_bpupdate(s, n1, B, b->d_z, b->d_p, b->d_cst, b->d_bc);
-        b->stat_steps++;
-      }
-      HIPCHK(hipGetLastError());
-      HIPCHK(hipMemcpyAsync(b->h_cst, b->d_cst, B * 4, hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
-      bool inner = false;
-      for (int j : cols) inner = inner || b->h_cst[j] == BCOL_INNER;
-      if (!inner) break;
-      chunk = std::min(chunk * 2, 64);
+  // Mixed-precision PCG of every active column, in lockstep steps; each
+  // column advances its own state (BColState), so finished columns cost no
+  // vector traffic.  The FP64 residual runs at the start and, for columns
+  // that asked for a replacement, at host synchronisation points.
+  const int64_t max_steps = 2 * (int64_t)cg_maxiter + 64;
+  int64_t steps = 0;
+  const int resid_mask = BCOL_BIT(BCOL_START) | BCOL_BIT(BCOL_REPLACE);
+  bool resid = true;  // START columns
+  for (int chunk = 8;; chunk = std::min(chunk * 2, 32)) {
+    if (resid) {
+      launch_bmv_f64(s, m->nelmt, B, m->d_KP64, m->d_gdof, b->d_x, b->d_Y64,
+                     b->d_cst, resid_mask);
+      launch_bresid(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y64, b->d_load,
+                    b->d_dinv32, b->d_r, b->d_p, b->d_part, b->d_cst);
+      launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_RESID);
     }
-    launch_bxupdate(s, n1, B, b->d_d, b->d_x, b->d_part, b->d_cst);
-    launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_OUTER);
+    for (int it = 0; it < chunk; it++) {
+      launch_bpupdate(s, n1, B, b->d_r, b->d_dinv32, b->d_p, b->d_cst, b->d_bc);
+      launch_bmv_mfma_f32(s, m->nelmt, B, c->n_cu, m->d_KP32, m->d_gdof, b->d_p,
+                          b->d_Y32, b->d_cst, BCOL_BIT(BCOL_INNER));
+      launch_bassemble_dot(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y32, b->d_p,
+                           b->d_kp, b->d_part, b->d_cst);
+      launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_ALPHA);
+      launch_bupdate(s, n1, B, b->d_p, b->d_kp, b->d_dinv32, b->d_x, b->d_r,
+                     b->d_part, b->d_cst, b->d_bc);
+      launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_INNER);
+    }
+    steps += chunk;
+    b->stat_steps += chunk;
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(b->h_cst, b->d_cst, B * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(b->h_bc, b->d_bc, B * sizeof(BCol), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    for (int j : cols) cst[j] = b->h_cst[j];
+    bool running = false;
+    resid = false;
+    for (int j : cols) {
+      cst[j] = b->h_cst[j];
+      if (cst[j] == BCOL_INNER && b->h_bc[j].req) {
+        cst[j] = BCOL_REPLACE;
+        resid = true;
+      }
+      running = running || cst[j] == BCOL_INNER || cst[j] == BCOL_REPLACE;
+    }
+    if (!running) break;
+    if (resid)
+      HIPCHK(hipMemcpyAsync(b->d_cst, cst.data(), B * 4, hipMemcpyHostToDevice, s));
+    if (steps > max_steps)  // every step advances each running column
+      return fail(SGEO_ESTATE, "batch PCG did not terminate in %lld steps",
+                  (long long)steps);
   }
   // uerr, body, body load for every active column
   launch_buerr(s, n1, B, b->d_x, b->d_oldx, b->d_part, b->d_ps, b->d_active, nl_tol);
@@ -1126,8 +1135,11 @@ extern "C" int sgeo_batch_plastic_iter(sgeo_batch* b, const int32_t* is_last,
   HIPCHK(hipStreamSynchronize(s));
   for (int j = 0; j < B; j++) {
     if (!b->active[j]) continue;
-    b->stat_inner += b->memo_pending[j] ? 0 : b->h_bc[j].cg_total;
-    if (cg_iter) cg_iter[j] = b->h_bc[j].cg_total;
+    if (!b->memo_pending[j]) {
+      b->stat_inner += b->h_bc[j].it;
+      b->stat_outer += b->h_bc[j].nrep;
+    }
+    if (cg_iter) cg_iter[j] = b->h_bc[j].it;
     if (errcode) errcode[j] = cst[j] == BCOL_DONE ? 0 : -1;
     if (uerr) uerr[j] = b->h_ps[j].uerr;
     if (nl_isconv) nl_isconv[j] = b->h_ps[j].nl_isconv;

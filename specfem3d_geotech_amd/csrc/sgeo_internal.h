@@ -96,45 +96,57 @@ void launch_uerr(hipStream_t s, int64_t n1, double* x, double* oldx,
 void launch_plastic_body(hipStream_t s, const BodyArgs& b);
 
 // ---- multi-trial mixed-precision engine (sgeo_batch.hip) -----------------
-// column states of the refinement PCG
+// Per-column states of the mixed-precision PCG.  The inner iteration runs in
+// FP32 (operator, r, p, kp) with FP64 dot products and an FP64 solution x;
+// the FP64 true residual f - K x (FP64 operator) starts each solve and
+// replaces the recursive FP32 residual once r.z has dropped by inner_tol
+// since the last replacement (requested by the column, carried out at the
+// next host synchronisation), keeping the search direction.
 enum BColState {
-  BCOL_OFF = 0,         // column not in this solve
-  BCOL_NEED_RESID = 1,  // next: FP64 residual r = f - K x, then inner start
-  BCOL_INNER = 2,       // inner FP32 CG running
-  BCOL_INNER_DONE = 3,  // inner stopped: next x += d and the outer test
-  BCOL_DONE = 4,        // converged
-  BCOL_FAIL = 5,        // cg_maxiter or outer_max reached
+  BCOL_OFF = 0,      // column not in this solve
+  BCOL_START = 1,    // next: FP64 residual, p = D r
+  BCOL_INNER = 2,    // FP32 CG iteration
+  BCOL_REPLACE = 3,  // next: FP64 residual replaces r, p = D r + beta p
+  BCOL_DONE = 4,     // stop test passed
+  BCOL_FAIL = 5,     // maxiter reached
 };
-enum BFinWhat { BFIN_RZ = 0, BFIN_ALPHA = 1, BFIN_INNER = 2, BFIN_OUTER = 3 };
+#define BCOL_BIT(s) (1 << (s))
+enum BFinWhat { BFIN_RESID = 0, BFIN_ALPHA = 1, BFIN_INNER = 2 };
 
 struct BCol {  // per-column scalars, device resident
-  double rz, alpha, beta, maxp, maxd, maxx;
+  double rz;        // r.z of the current residual
+  double rz0;       // r.z at the last FP64 residual
+  double alpha, beta, maxp;
   double tol, inner_tol;
-  int32_t inner_it, inner_max, outer_it, outer_max, cg_total, cg_maxiter;
+  double rz_prev;   // r.z of the previous iteration (beta after a replacement)
+  int32_t it, maxiter;  // CG iterations of this solve, limit
+  int32_t pending;      // 1: p = D r + beta p before the next product
+  int32_t nrep;         // FP64 residuals of this solve
+  int32_t req;          // 1: the column asks for an FP64 residual
+  int32_t pad_;
 };
 
 int64_t batch_partials(int64_t n1);
-void launch_bmv_f32(hipStream_t s, int32_t nelmt, int B, const float* KP,
-                    const int32_t* gdof, const float* P, float* Y,
-                    const int32_t* cst, int want);
+void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu,
+                         const float* KP, const int32_t* gdof, const float* P,
+                         float* Y, const int32_t* cst, int want_mask);
 void launch_bmv_f64(hipStream_t s, int32_t nelmt, int B, const double* KP,
                     const int32_t* gdof, const double* X, double* Y,
-                    const int32_t* cst, int want);
+                    const int32_t* cst, int want_mask);
 void launch_bassemble_dot(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
                           const int32_t* slots, const float* Y, const float* p,
                           float* kp, double* part, const int32_t* cst);
 void launch_bupdate(hipStream_t s, int64_t n1, int B, const float* p,
-                    const float* kp, const float* dinv, float* d, float* r,
-                    float* z, double* part, const int32_t* cst, const BCol* bc);
-void launch_bpupdate(hipStream_t s, int64_t n1, int B, const float* z, float* p,
-                     const int32_t* cst, const BCol* bc);
+                    const float* kp, const float* dinv, double* x, float* r,
+                    double* part, const int32_t* cst, const BCol* bc);
+void launch_bpupdate(hipStream_t s, int64_t n1, int B, const float* r,
+                     const float* dinv, float* p, const int32_t* cst,
+                     const BCol* bc);
 void launch_bresid(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
                    const int32_t* slots, const double* Y64, const double* load,
-                   const float* dinv, float* r, float* z, float* p, float* d,
-                   double* part, const int32_t* cst);
-void launch_bxupdate(hipStream_t s, int64_t n1, int B, const float* d, double* x,
-                     double* part, const int32_t* cst);
-void launch_bfinalize(hipStream_t s, int64_t n1, int B, const double* part,
+                   const float* dinv, float* r, float* p, double* part,
+                   const int32_t* cst);
+void launch_bfinalize(hipStream_t s, int64_t n1, int B, double* part,
                       int32_t* cst, BCol* bc, int what);
 void launch_bload(hipStream_t s, int64_t n1, int B, const double* ext,
                   const double* body, double* load, const int32_t* active);
