@@ -1,22 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark of the EBE-PCG + viscoplastic hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode analysis|pcg]
-                    [--form fp64|mixed|batched] [--workload c1|c0]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--mode sweep|pcg|analysis] [--form ...] [--workload c1|c0]
 
-Workloads (synthetic, seeded; see specfem3d_geotech_amd.workloads):
+Workloads (synthetic, seeded; specfem3d_geotech_amd.workloads):
   c1  144,000-element two-layer 2H:1V slope with +-2% interior jitter
-      (3,489,480 equations), the plain FP64 GPU drop-in configuration.
+      (3,489,480 equations).
   c0  1,152-element homogeneous slope (the reference's CPU-runnable case).
 
 Modes:
-  analysis  one step = one complete FOS analysis (strength-reduction sweep
-            1.00:0.05:2.00 up to the first SRF whose plastic loop reaches
-            nl_maxiter); value = FOS analyses/hour.
-  pcg       one step = one CG iteration of the elastic solve (fixed count,
-            unconverged); value = DOF-iterations/s.
-In both modes the element-product kernel is timed with HIP events on its own
-stream over the timed region and reported against the HBM roofline.
+  sweep (default)  the batched mixed-precision form on the c1 mesh: 64
+            strength realisations (form A: lognormal c, mean 10 kPa, CoV 0.3;
+            lognormal tan phi, mean tan 20 deg, CoV 0.2), each an FOS sweep
+            over SRF 1.00:0.05:2.00, 64 trials resident at once.  A step is
+            one batched CG iteration (every running trial advances one CG
+            iteration; trials that finish a PCG call take their plastic body
+            pass and go on).  value = DOF-iterations/s over all trials
+            (equations x CG iterations / s).  The same run times the plain
+            FP64 drop-in PCG on the same mesh for the per-trial speedup.
+  pcg       the plain FP64 drop-in: one step = one CG iteration of the
+            elastic solve (fixed count); value = DOF-iterations/s.
+  analysis  complete FOS analyses: --form fp64 runs the reference's sweep
+            with the FP64 drop-in, one analysis after another; --form batched
+            runs --analyses N form-A analyses through the continuous-batching
+            scheduler.  value = FOS analyses/hour, with p50/p99 latency.
+The element-product kernel of each mode is timed with HIP events on the
+library's stream and reported against the HBM roofline.
 """
 from __future__ import annotations
 
@@ -30,8 +40,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = ("FOS analyses/hour + p50/p99 latency; EBE-PCG HBM GB/s vs peak, "
+          "1xMI355X")
+DATA = "synthetic (seeded slope mesh in the reference's input format)"
 
-# form -> (operator precision, storage, reduction order, bytes/entry, entries)
+# single-trial forms: (operator precision, storage, reduction, bytes/entry, entries)
 FORMS = {
     "fp64": ("f64", "packed", "tree", 8, 3328),
     "fp64full": ("f64", "full", "tree", 8, 6561),
@@ -44,37 +57,180 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--mode", default="pcg", choices=["analysis", "pcg"])
-    ap.add_argument("--form", default="fp64", choices=list(FORMS))
+    ap.add_argument("--mode", default="sweep", choices=["sweep", "pcg", "analysis"])
+    ap.add_argument("--form", default=None,
+                    choices=list(FORMS) + ["batched"])
     ap.add_argument("--workload", default="c1", choices=["c1", "c0"])
+    ap.add_argument("--trials", type=int, default=64)
+    ap.add_argument("--analyses", type=int, default=None)
+    ap.add_argument("--rate", type=float, default=None,
+                    help="analysis mode: Poisson arrival rate (1/s) of requests")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-iters", type=int, default=30)
     return ap.parse_args()
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part "
-              "of it (trials or mesh partitions would shard across GPUs, see "
-              "DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-    from specfem3d_geotech_amd import workloads
-    from specfem3d_geotech_amd.solver import (K_F32, K_F64, KFMT_FULL, KFMT_PACKED,
-                                              REDUCE_SEQUENTIAL, Context,
-                                              DeviceMesh)
-    from specfem3d_geotech_amd.slope import DeviceSlope
-    from specfem3d_geotech_amd.plastic import dt_viscoplas, strength_reduction
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
 
-    wl = workloads.get(a.workload)
+
+def single_trial_pcg(ctx, dm, ext, ym, nu, phi, coh, psi, steps, warm):
+    """FP64 drop-in: `steps` unconverged CG iterations of the elastic solve
+    (warm-up first); returns (seconds, launches, kernel_ms)."""
+    from specfem3d_geotech_amd.plastic import dt_viscoplas, strength_reduction
+    from specfem3d_geotech_amd.slope import DeviceSlope
+    st = DeviceSlope(dm, ext)
+    try:
+        cohf, nuf, phif, psif, _ = strength_reduction(1.0, False, coh, nu, phi, psi)
+        dt = dt_viscoplas(nuf, phif, ym)
+        st.begin_srf(ym, nuf, phif, cohf, psif, dt)
+        st.plastic_iter(1e-300, warm, 1.0, False)
+        st.begin_srf(ym, nuf, phif, cohf, psif, dt)
+        ctx.synchronize()
+        ctx.profile(1)
+        t0 = time.perf_counter()
+        r = st.plastic_iter(1e-300, steps, 1.0, False)
+        ctx.synchronize()
+        dt_s = time.perf_counter() - t0
+        n, ms = ctx.profile(0)
+        assert r["errcode"] == -1 and r["cg_iter"] == steps + 1
+    finally:
+        st.close()
+    return dt_s, n, ms
+
+
+def roofline(kernel, bytes_per_launch, n, ms, extra=None):
+    avg = ms / max(n, 1)
+    ach = bytes_per_launch / (avg * 1e-3) / 1e9 if n else 0.0
+    out = dict(kernel=kernel, bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS,
+               unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4), avg_launch_ms=round(avg, 4),
+               launches=n, bytes_per_launch=int(bytes_per_launch), traffic=None)
+    if extra:
+        out.update(extra)
+    return out
+
+
+def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
+    import numpy as np
+    from specfem3d_geotech_amd import workloads
+    from specfem3d_geotech_amd.batch import BatchEngine, device_mesh
+    from specfem3d_geotech_amd.plastic import dt_viscoplas, strength_reduction
+    from specfem3d_geotech_amd.solver import K_F64, KFMT_PACKED
+    steps = a.steps or 1000
+    warm = a.warmup if a.warmup is not None else 100
+    B = a.trials
+    ctl = wl.control
+    gam, ym, nu, phi, coh, psi = mesh.material_arrays()
     t0 = time.time()
-    mesh, spec = wl.build()
-    prep_s = time.time() - t0
-    ctx = Context(0)
-    kp, kf, red, bk, S = FORMS[a.form]
+    dm, ext = device_mesh(ctx, mesh, spec)
+    kbuild_s = time.time() - t0
+    E, N = spec.nelmt, spec.neq
+    # plain FP64 drop-in on the same mesh (packed FP64 operator)
+    dm.set_operator(K_F64, KFMT_PACKED)
+    t64, n64, ms64 = single_trial_pcg(ctx, dm, ext, ym, nu, phi, coh, psi, 200, 10)
+    fp64_ms_iter = t64 / 200 * 1e3
+    eng = BatchEngine(dm, ext, B)
+    real = workloads.form_a(B, len(gam))
+    srf = list(ctl["srf"])
+    nxt = [0] * B  # next SRF index per column (each column runs one analysis)
+    kw = dict(cg_tol=ctl["cg_tol"], cg_maxiter=ctl["cg_maxiter"], inner_tol=1e-5,
+              nl_tol=ctl["nl_tol"], nl_maxiter=ctl["nl_maxiter"])
+    done_rows = []
+
+    def start(j):
+        c, p = real[j]
+        cohf, nuf, phif, psif, _ = strength_reduction(srf[nxt[j]], False, c, nu, p, psi)
+        eng.begin(j, ym, nuf, phif, cohf, psif, dt_viscoplas(nuf, phif, ym))
+        nxt[j] += 1
+
+    def handle(r):
+        for j in np.nonzero(r["ended"])[0]:
+            if r["errcode"][j] != 0:
+                raise RuntimeError("ERROR: PCG solver doesn't converge!")
+            done_rows.append((int(j), srf[nxt[j] - 1], int(r["nl_tot"][j]),
+                              int(r["cg_tot"][j])))
+            failed = r["nl_tot"][j] >= ctl["nl_maxiter"]
+            eng.release(j)
+            if failed or nxt[j] >= len(srf):  # analysis ended: start it over
+                nxt[j] = 0
+            start(j)
+
+    try:
+        t0 = time.time()
+        start(0)  # the elastic solution first, then every column seeded with it
+        while True:
+            r = eng.advance(max_steps=1 << 30, **kw)
+            handle(r)
+            if r["memo_ready"]:
+                break
+        for j in range(1, B):
+            start(j)
+        log(f"elastic solve and start: {time.time() - t0:.1f} s")
+        s0 = eng.stats()
+        while eng.stats()["batched_steps"] - s0["batched_steps"] < warm:
+            handle(eng.advance(max_steps=warm, **kw))
+        ctx.synchronize()
+        ctx.profile(1)
+        s0 = eng.stats()
+        t0 = time.perf_counter()
+        while True:
+            left = steps - (eng.stats()["batched_steps"] - s0["batched_steps"])
+            if left <= 0:
+                break
+            handle(eng.advance(max_steps=left, **kw))
+            log(f"  {eng.stats()['batched_steps'] - s0['batched_steps']} steps, "
+                f"{time.perf_counter() - t0:.1f} s")
+        ctx.synchronize()
+        dt_s = time.perf_counter() - t0
+        nl, kms = ctx.profile(0)
+        s1 = eng.stats()
+        nbytes = eng.nbytes()
+    finally:
+        eng.close()
+        dm.close()
+    nsteps = s1["batched_steps"] - s0["batched_steps"]
+    ncg = s1["cg_iterations"] - s0["cg_iterations"]
+    b_eff = ncg / max(nsteps, 1)  # trials iterating per step
+    per_trial_ms = dt_s / max(ncg, 1) * 1e3
+    # algorithmic bytes of one element-product launch at b_eff columns:
+    # packed FP32 K, gdof, the gathered P and the written Y of those columns
+    kbytes = E * (3328 * 4 + 81 * 4 + 2 * 81 * 4 * b_eff)
+    res = dict(
+        metric=METRIC, value=N * ncg / dt_s, unit="DOF-iterations/s", n_gpus=1,
+        steps=nsteps, warmup=warm, ms_per_step=dt_s / nsteps * 1e3,
+        higher_is_better=True, scaling="weak", vs_baseline=None,
+        dtype="f32 inner CG (MFMA) + f64 dots/x/residual/yield", data=DATA,
+        config=dict(workload=wl.description + f"; {B} form-A strength realisations, "
+                    "each an SRF sweep 1.00:0.05:2.00", nelmt=E, neq=N, trials=B,
+                    form="batched mixed precision", k_storage="packed f32 (S=3328) + "
+                    "packed f64 for the residual", cg_tol=ctl["cg_tol"], inner_tol=1e-5,
+                    nl_tol=ctl["nl_tol"], nl_maxiter=ctl["nl_maxiter"],
+                    batch_hbm_gb=round(nbytes / 1e9, 2), copy_kernel_gbs=round(copy_gbs, 1),
+                    host_prep_s=round(prep_s, 2), k_build_s=round(kbuild_s, 3)),
+        trial_iterations=ncg, mean_trials_iterating=round(b_eff, 2),
+        per_trial_iteration_ms=round(per_trial_ms, 5),
+        fp64_dropin=dict(ms_per_iteration=round(fp64_ms_iter, 4),
+                         dof_iterations_per_s=N / (fp64_ms_iter * 1e-3),
+                         operator="packed f64",
+                         ebe_kernel_ms=round(ms64 / max(n64, 1), 4)),
+        speedup_vs_fp64_per_trial_iteration=round(fp64_ms_iter / per_trial_ms, 2),
+        trials_ended_in_run=len(done_rows),
+        roofline=roofline("k_bmv_mfma<4> (v_mfma_f32_16x16x4f32)", kbytes, nl, kms,
+                          dict(bytes_model="E*(3328*4 + 81*4 + 2*81*4*b_eff), "
+                               "b_eff = mean trials iterating per step",
+                               b_eff=round(b_eff, 2))))
+    return res
+
+
+def mode_pcg(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
+    from specfem3d_geotech_amd.solver import (K_F32, K_F64, KFMT_FULL, KFMT_PACKED,
+                                              REDUCE_SEQUENTIAL, DeviceMesh)
+    form = a.form or "fp64"
+    kp, kf, red, bk, S = FORMS[form]
     if red == "sequential":
         ctx.set_reduction(REDUCE_SEQUENTIAL)
-    copy_gbs = ctx.calibrate_copy(4 << 30, 10)
+    steps = a.steps or 300
+    warm = a.warmup if a.warmup is not None else 20
     gam, ym, nu, phi, coh, psi = mesh.material_arrays()
     dm = DeviceMesh(ctx, spec, len(gam))
     t0 = time.time()
@@ -82,84 +238,147 @@ def main():
     dm.set_operator(K_F64 if kp == "f64" else K_F32,
                     KFMT_PACKED if kf == "packed" else KFMT_FULL)
     kbuild_s = time.time() - t0
+    try:
+        dt_s, n, ms = single_trial_pcg(ctx, dm, ext, ym, nu, phi, coh, psi, steps, warm)
+    finally:
+        dm.close()
     E, N = spec.nelmt, spec.neq
-    # algorithmic bytes of one element-product launch: K (E*S*bk) plus the
-    # element dof indices (E*81*4), the gathered p (E*81*8) and Y (E*81*8)
-    k_bytes = E * S * bk + E * 81 * 4 + 2 * E * 81 * 8
-    result = dict(metric="FOS analyses/hour + p50/p99 latency; EBE-PCG HBM GB/s "
-                         "vs peak, 1xMI355X")
+    kbytes = E * (S * bk + 81 * 4 + 2 * 81 * 8)
+    return dict(
+        metric=METRIC, value=N * steps / dt_s, unit="DOF-iterations/s", n_gpus=1,
+        steps=steps, warmup=warm, ms_per_step=dt_s / steps * 1e3, higher_is_better=True,
+        scaling="weak", vs_baseline=None, dtype="f64", data=DATA,
+        config=dict(workload=wl.description, nelmt=E, neq=N, form=form, mode="pcg",
+                    k_storage=f"{kf} {kp} (S={S})", reduction=red,
+                    copy_kernel_gbs=round(copy_gbs, 1), host_prep_s=round(prep_s, 2),
+                    k_build_s=round(kbuild_s, 4)),
+        roofline=roofline(("k_ebe_packed" if kf == "packed" else "k_ebe_elem")
+                          + ("<double>" if kp == "f64" else "<float>"), kbytes, n, ms,
+                          dict(bytes_model=f"E*({S}*{bk} + 81*4 + 2*81*8)")))
 
-    if a.mode == "pcg":
-        steps = a.steps or 300
-        warm = a.warmup if a.warmup is not None else 20
-        st = DeviceSlope(dm, ext)
-        cohf, nuf, phif, psif, _ = strength_reduction(1.0, False, coh, nu, phi, psi)
-        st.begin_srf(ym, nuf, phif, cohf, psif, dt_viscoplas(nuf, phif, ym))
-        st.plastic_iter(1e-300, warm, 1.0, False)  # warm-up CG iterations
-        st.begin_srf(ym, nuf, phif, cohf, psif, dt_viscoplas(nuf, phif, ym))
-        ctx.synchronize()
-        ctx.profile(1)
+
+def percentile(v, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(v), q)) if len(v) else None
+
+
+def mode_analysis(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
+    import numpy as np
+    from specfem3d_geotech_amd import workloads
+    from specfem3d_geotech_amd.solver import K_F64, KFMT_PACKED
+    form = a.form or "fp64"
+    ctl = dict(wl.control)
+    E, N = spec.nelmt, spec.neq
+    gam, ym, nu, phi, coh, psi = mesh.material_arrays()
+    nan = a.analyses or (1 if form != "batched" else a.trials)
+    real = workloads.form_a(nan, len(gam))
+    lat, rows_out = [], []
+    ctx.profile(1)
+    if form == "batched":
+        from specfem3d_geotech_amd.batch import Analysis, BatchEngine, device_mesh, \
+            run_analyses
+        dm, ext = device_mesh(ctx, mesh, spec)
+        eng = BatchEngine(dm, ext, a.trials)
+        an = [Analysis(coh=c, phi=p, psi=psi.copy(), srf=list(ctl["srf"]), tag=i)
+              for i, (c, p) in enumerate(real)]
+        arrivals = None
+        if a.rate:
+            rng = np.random.default_rng(4)
+            arrivals = list(np.cumsum(rng.exponential(1.0 / a.rate, nan)))
+
+        def done(x):
+            log(f"  analysis {x.tag} bracket {x.bracket} latency {x.latency:.1f} s, "
+                f"{len(x.rows)} SRFs")
         t0 = time.perf_counter()
-        r = st.plastic_iter(1e-300, steps, 1.0, False)
-        ctx.synchronize()
-        dt = time.perf_counter() - t0
-        nlaunch, kms = ctx.profile(0)
-        assert r["errcode"] == -1 and r["cg_iter"] == steps + 1
-        ms_step = dt / steps * 1e3
-        result.update(value=N * steps / dt, unit="DOF-iterations/s",
-                      ms_per_step=ms_step, steps=steps, warmup=warm)
-        st.close()
+        try:
+            stats = run_analyses(eng, an, ym=ym, nu=nu, cg_tol=ctl["cg_tol"],
+                                 cg_maxiter=ctl["cg_maxiter"], nl_tol=ctl["nl_tol"],
+                                 nl_maxiter=ctl["nl_maxiter"], arrivals=arrivals,
+                                 on_done=done)
+        finally:
+            eng.close()
+            dm.close()
+        dt_s = time.perf_counter() - t0
+        lat = [x.latency for x in an]
+        rows_out = [dict(tag=x.tag, c=float(x.coh[0]), phi=float(x.phi[0]),
+                         bracket=x.bracket, cg_tot=sum(r["cg_tot"] for r in x.rows.values()),
+                         nl_tot=sum(r["nl_tot"] for r in x.rows.values()))
+                    for x in an]
+        extra = dict(engine=stats)
+        kname = "k_bmv_mfma"
     else:
         from specfem3d_geotech_amd.slope import slope3d
-        steps = a.steps or 1
-        ctl = dict(wl.control)
-        dm.close()
-        ctx.profile(1)
+        kp, kf, red, bk, S = FORMS[form]
         t0 = time.perf_counter()
-        analyses = []
-        for k in range(steps):
-            res = slope3d(ctx, mesh, spec=spec, kprec=K_F64 if kp == "f64" else K_F32,
-                          kfmt=KFMT_PACKED if kf == "packed" else KFMT_FULL,
-                          progress=lambda r: print("  row", json.dumps(r), flush=True),
-                          **ctl)
-            analyses.append(dict(bracket=res.fos_bracket, rows=res.rows,
-                                 n_pcg=len(res.cg_log), cg_total=int(sum(res.cg_log))))
-        dt = time.perf_counter() - t0
-        nlaunch, kms = ctx.profile(0)
-        cg_total = sum(x["cg_total"] for x in analyses)
-        result.update(value=steps * 3600.0 / dt, unit="FOS analyses/hour",
-                      ms_per_step=dt / steps * 1e3, steps=steps, warmup=0,
-                      analyses=analyses, dof_iterations_per_s=N * cg_total / dt,
-                      seconds_per_analysis=dt / steps)
-        dm = DeviceMesh(ctx, spec, len(gam))  # closed below
+        for i, (c, p) in enumerate(real):
+            m2 = mesh.with_strengths(c, p) if a.analyses else mesh
+            t1 = time.perf_counter()
+            last = [time.perf_counter()]
 
-    avg_ms = kms / max(nlaunch, 1)
-    achieved = k_bytes / (avg_ms * 1e-3) / 1e9
-    result.update(
-        n_gpus=1, higher_is_better=True, scaling="weak", vs_baseline=None,
-        dtype="f64", data="synthetic (seeded slope mesh, reference input format)",
-        config=dict(workload=wl.description, nelmt=E, neq=N, form=a.form,
-                    mode=a.mode, k_storage=f"{kf} {kp} (S={S})", reduction=red,
-                    copy_kernel_gbs=round(copy_gbs, 1),
-                    host_prep_s=round(prep_s, 2), k_build_s=round(kbuild_s, 4)),
-        roofline=dict(kernel=("k_ebe_packed" if kf == "packed" else "k_ebe_elem")
-                      + ("<double>" if kp == "f64" else "<float>"), bound="hbm",
-                      achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=round(achieved / HBM_PEAK_GBS, 4),
-                      avg_launch_ms=round(avg_ms, 4), launches=nlaunch,
-                      bytes_per_launch=k_bytes, traffic=None))
-    dm.close()
-    ctx.close()
+            def prog(row):
+                log("  row", json.dumps(row), f"{time.perf_counter() - last[0]:.1f} s")
+                last[0] = time.perf_counter()
 
-    if not a.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(spec, mesh, a.cpu_sample_iters,
-                                              ms_gpu_iter=result.get("ms_per_step"))
-    print(json.dumps(result), flush=True)
+            def it_prog(nl, cg):
+                if nl % 20 == 0:
+                    log(f"    plastic iteration {nl}, cg {cg}")
+            res = slope3d(ctx, m2, spec=spec, kprec=K_F64, kfmt=KFMT_PACKED, progress=prog,
+                          iter_progress=it_prog, **ctl)
+            lat.append(time.perf_counter() - t1)
+            rows_out.append(dict(bracket=res.fos_bracket, rows=res.rows,
+                                 cg_total=int(sum(res.cg_log))))
+        dt_s = time.perf_counter() - t0
+        extra = {}
+        kname = "k_ebe_packed<double>"
+    nl, kms = ctx.profile(0)
+    out = dict(
+        metric=METRIC, value=nan * 3600.0 / dt_s, unit="FOS analyses/hour", n_gpus=1,
+        steps=nan, warmup=0, ms_per_step=dt_s / nan * 1e3, higher_is_better=True,
+        scaling="weak", vs_baseline=None,
+        dtype="f64" if form != "batched" else "f32 inner CG (MFMA) + f64", data=DATA,
+        config=dict(workload=wl.description, nelmt=E, neq=N, form=form, mode="analysis",
+                    analyses=nan, trials=a.trials if form == "batched" else 1,
+                    arrival_rate=a.rate),
+        latency_s=dict(p50=percentile(lat, 50), p99=percentile(lat, 99), max=max(lat)),
+        analyses=rows_out, kernel=dict(name=kname, launches=nl,
+                                       avg_ms=kms / max(nl, 1)), **extra)
+    return out
 
 
-def cpu_baseline(spec, mesh, iters, ms_gpu_iter=None):
-    """The oracle's C restatement (OpenMP) on the same operator: K build,
-    then `iters` unconverged CG iterations of the elastic solve."""
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU; runs on more GPUs are not part "
+              "of it (trials would shard across GPUs with the mesh replicated)",
+              file=sys.stderr)
+        sys.exit(2)
+    from specfem3d_geotech_amd import workloads
+    from specfem3d_geotech_amd.solver import Context
+
+    wl = workloads.get(a.workload)
+    t0 = time.time()
+    mesh, spec = wl.build()
+    prep_s = time.time() - t0
+    ctx = Context(0)
+    copy_gbs = ctx.calibrate_copy(4 << 30, 10)
+    try:
+        if a.mode == "sweep":
+            res = mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs)
+        elif a.mode == "pcg":
+            res = mode_pcg(a, ctx, wl, mesh, spec, prep_s, copy_gbs)
+        else:
+            res = mode_analysis(a, ctx, wl, mesh, spec, prep_s, copy_gbs)
+    finally:
+        ctx.close()
+    if not a.no_cpu_baseline and a.mode != "analysis":
+        res["cpu_baseline"] = cpu_baseline(spec, mesh, a.cpu_sample_iters)
+    print(json.dumps(res), flush=True)
+
+
+def cpu_baseline(spec, mesh, iters):
+    """The oracle's C restatement (OpenMP element products, serial
+    element-order assembly) on the same operator: K build on the host, then
+    `iters` unconverged FP64 CG iterations of the elastic solve."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
@@ -176,10 +395,10 @@ def cpu_baseline(spec, mesh, iters, ms_gpu_iter=None):
     del km
     return dict(value=spec.neq * iters / dt, unit="DOF-iterations/s",
                 cores=int(os.environ.get("OMP_NUM_THREADS", cores)), kind="port",
-                sample=f"{iters} unconverged FP64 CG iterations of the elastic "
-                       f"solve on the same {spec.nelmt}-element mesh (oracle C "
-                       f"restatement, OpenMP element products, serial "
-                       f"element-order assembly)", ms_per_iter=dt / iters * 1e3)
+                sample=f"{iters} unconverged FP64 CG iterations of the elastic solve on "
+                       f"the same {spec.nelmt}-element mesh (oracle C restatement, "
+                       "OpenMP element products, serial element-order assembly)",
+                ms_per_iter=dt / iters * 1e3)
 
 
 if __name__ == "__main__":

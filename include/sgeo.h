@@ -185,26 +185,46 @@ int sgeo_batch_create(sgeo_mesh *mesh, int32_t ncol, const double *extload,
 int sgeo_batch_destroy(sgeo_batch *batch);
 int sgeo_batch_bytes(sgeo_batch *batch, size_t *bytes);
 /* Starts a trial in column col (src/slope.F90:248-249, 281-282) with the
- * factored materials of its SRF and dt.  x0 (may be NULL) seeds x; when
- * given it is taken as the converged result of the trial's first PCG call
- * (the elastic solve, identical for every trial of a mesh), reported with
- * x0_cg_iter iterations. */
+ * factored materials of its SRF and dt.  The first PCG call of every trial
+ * solves K x = extload from x = 0 (bodyload is zero), the same system for
+ * every trial of the mesh: the engine keeps the first such solution and
+ * seeds later trials with it (their first call then reports its iteration
+ * count).  x0 (may be NULL) seeds x explicitly in the same way, reported
+ * with x0_cg_iter iterations. */
 int sgeo_batch_begin(sgeo_batch *batch, int32_t col, const double *ym,
                      const double *nuf, const double *phif, const double *cohf,
                      const double *psif, double dt, const double *x0,
                      int32_t x0_cg_iter);
 int sgeo_batch_release(sgeo_batch *batch, int32_t col);
-/* One plastic iteration of every active column (src/slope.F90:286-391);
- * per-column arrays of length ncol (entries of inactive columns untouched). */
+/* One plastic iteration of every running column in lockstep
+ * (src/slope.F90:286-391); per-column arrays of length ncol (entries of
+ * other columns untouched). */
 int sgeo_batch_plastic_iter(sgeo_batch *batch, const int32_t *is_last,
                             double cg_tol, int32_t cg_maxiter, double inner_tol,
                             double nl_tol, int32_t *cg_iter, int32_t *errcode,
                             double *uerr, int32_t *nl_isconv, double *fmax);
+/* Runs every running column through its own plastic iterations
+ * (src/slope.F90:285-392, is_last = the nl_maxiter-th), columns advancing
+ * independently: a column whose PCG call finished gets its body pass and
+ * starts its next call while the others keep iterating.  Returns when at
+ * least one trial ended (nl_isconv, nl_maxiter reached, or a PCG call that
+ * did not converge: errcode -1), when the elastic solution has just become
+ * available (memo_ready), when no column runs, or after max_steps CG steps.
+ * ended[col] = 1 marks the trials that ended in this call, with their
+ * summary row (src/slope.F90:415): nl_tot, cg_tot, fmax; the column keeps
+ * its state (sgeo_batch_download) until the next begin. */
+int sgeo_batch_advance(sgeo_batch *batch, double cg_tol, int32_t cg_maxiter,
+                       double inner_tol, double nl_tol, int32_t nl_maxiter,
+                       int64_t max_steps, int32_t *ended, int32_t *nl_tot,
+                       int32_t *cg_tot, double *fmax, int32_t *errcode,
+                       int32_t *memo_ready);
 int sgeo_batch_download(sgeo_batch *batch, int32_t col, double *x,
                         uint8_t *flags);
-/* Totals since creation: CG iterations of all trials, FP64 residuals of all
- * trials, and batched steps (one step = one CG iteration of every running
- * column). */
+/* Totals since creation, as of the last host synchronisation: CG
+ * iterations of all trials (seeded elastic solutions not counted), FP64
+ * residuals of all trials, and batched steps (one step = one CG iteration
+ * of every running column).  sgeo_profile times the batched element-product
+ * launches (k_bmv_mfma) like the single-trial one. */
 int sgeo_batch_stats(sgeo_batch *batch, int64_t *inner_iters,
                      int64_t *outer_iters, int64_t *steps);
 

@@ -60,6 +60,9 @@ SIGNATURES = {
     "sgeo_batch_release": (C.c_int, [vp, C.c_int32]),
     "sgeo_batch_plastic_iter": (C.c_int, [vp, i_p, C.c_double, C.c_int32, C.c_double,
                                           C.c_double, i_p, i_p, d_p, i_p, d_p]),
+    "sgeo_batch_advance": (C.c_int, [vp, C.c_double, C.c_int32, C.c_double, C.c_double,
+                                     C.c_int32, C.c_int64, i_p, i_p, i_p, d_p, i_p,
+                                     i_p]),
     "sgeo_batch_download": (C.c_int, [vp, C.c_int32, d_p, u8_p]),
     "sgeo_batch_stats": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_int64)]),

@@ -66,6 +66,19 @@ class BatchEngine:
             ptr(fm)), "sgeo_batch_plastic_iter")
         return dict(cg_iter=it, errcode=ec, uerr=ue, nl_isconv=cv, fmax=fm)
 
+    def advance(self, cg_tol, cg_maxiter, inner_tol, nl_tol, nl_maxiter,
+                max_steps=1 << 40):
+        n = self.ncol
+        ended = np.zeros(n, np.int32); nl = np.zeros(n, np.int32)
+        cg = np.zeros(n, np.int32); fm = np.zeros(n); ec = np.zeros(n, np.int32)
+        memo = C.c_int32()
+        check(L.lib().sgeo_batch_advance(
+            self.h, cg_tol, cg_maxiter, inner_tol, nl_tol, nl_maxiter, int(max_steps),
+            ptr(ended, C.c_int32), ptr(nl, C.c_int32), ptr(cg, C.c_int32), ptr(fm),
+            ptr(ec, C.c_int32), C.byref(memo)), "sgeo_batch_advance")
+        return dict(ended=ended, nl_tot=nl, cg_tot=cg, fmax=fm, errcode=ec,
+                    memo_ready=bool(memo.value))
+
     def download(self, col):
         sp = self.dmesh.spec
         x = np.zeros(sp.neq + 1)
@@ -116,67 +129,66 @@ class Analysis:
     flags_last: Optional[np.ndarray] = None
     x_last: Optional[np.ndarray] = None
 
-
-@dataclass
-class _Col:
-    analysis: Optional[Analysis] = None
-    srf_idx: int = -1
-    nl_iter: int = 0
-    cg_tot: int = 0
-    fmax: float = 0.0
+    @property
+    def latency(self):
+        return None if self.t_end is None else self.t_end - self.t_arrival
 
 
-def run_analyses(engine: BatchEngine, analyses: List[Analysis], *, ym, nu,
-                 cg_tol: float, cg_maxiter: int, nl_tol: float, nl_maxiter: int,
+class PCGNotConverged(RuntimeError):
+    """errcode -1 of a trial's PCG call; the reference stops (error_stop)."""
+
+
+def run_analyses(engine, analyses: List[Analysis], *, ym, nu, cg_tol: float,
+                 cg_maxiter: int, nl_tol: float, nl_maxiter: int,
                  inner_tol: float = 1e-5, srf_parallel: int = 1,
                  keep_state: bool = False, arrivals: Optional[List[float]] = None,
-                 clock=time.perf_counter, on_done=None):
-    """Runs every analysis to its FOS bracket with all columns kept busy.
+                 clock=time.perf_counter, on_done=None, poll_steps: int = 64):
+    """Runs every analysis to its FOS bracket, keeping the engine's columns
+    busy (continuous batching).
 
-    srf_parallel > 1 lets one analysis run that many consecutive SRFs at once
-    (the bracket is still the first failing SRF in list order; trials above a
-    known failure are abandoned).  `arrivals` (seconds from start, one per
-    analysis) delays admission for request streams.  Returns engine stats."""
+    Each analysis walks its SRF list; srf_parallel > 1 lets it run that many
+    consecutive SRFs at once (the bracket is still the first failing SRF in
+    list order, src/slope.F90:441, and trials above a known failure are
+    abandoned).  A column freed by an ended trial takes the next SRF of an
+    admitted analysis, else the next waiting analysis.  `arrivals` (seconds
+    from the start, one per analysis) admits a request stream: while
+    requests are still to come the engine returns every `poll_steps` CG
+    steps so they are admitted as soon as a column is free.  Until the
+    engine holds the elastic solution (the first PCG call of any trial) only
+    one trial runs.  Returns the engine's statistics."""
     spec = engine.dmesh.spec
     ncol = engine.ncol
-    cols = [_Col() for _ in range(ncol)]
+    cols: List[Optional[tuple]] = [None] * ncol  # (analysis, srf index)
     queue = list(range(len(analyses)))
+    admitted: List[Analysis] = []
     t0 = clock()
     for i, a in enumerate(analyses):
         a.t_arrival = t0 + (arrivals[i] if arrivals is not None else 0.0)
-    x_el = None
-    cg_el = 0
-    elastic_pending: List[int] = []
-    admitted: List[Analysis] = []  # in admission order
+    memo_ready = False
 
-    def admit_ready(now):
-        return [i for i in queue if arrivals is None or arrivals[i] <= now - t0]
+    def running_of(a):
+        return sum(1 for c in cols if c is not None and c[0] is a)
 
     def start_trial(j, a, k):
         s = a.srf[k]
         cohf, nuf, phif, psif, _ = strength_reduction(s, False, a.coh, nu, a.phi,
                                                       a.psi)
-        dt = dt_viscoplas(nuf, phif, ym)
-        engine.begin(j, ym, nuf, phif, cohf, psif, dt,
-                     x0=x_el if x_el is not None else None, x0_cg=cg_el)
-        cols[j] = _Col(analysis=a, srf_idx=k)
+        engine.begin(j, ym, nuf, phif, cohf, psif, dt_viscoplas(nuf, phif, ym))
+        cols[j] = (a, k)
         if a.t_start is None:
             a.t_start = clock()
 
-    def fill_free_columns():
-        if x_el is None and elastic_pending:
-            return  # one column solves the elastic system before the others start
+    def fill():
         now = clock()
         for j in range(ncol):
-            if cols[j].analysis is not None:
+            if cols[j] is not None:
                 continue
-            # the next SRF of an admitted analysis (up to srf_parallel at once)
+            if not memo_ready and any(c is not None for c in cols):
+                return  # one trial computes the elastic solution first
             placed = False
-            for a in admitted:
-                if a.done:
-                    continue
-                running = sum(1 for cc in cols if cc.analysis is a)
-                if (running < srf_parallel and a.next_srf < len(a.srf)
+            for a in admitted:  # next SRF of an admitted analysis
+                if (not a.done and running_of(a) < srf_parallel
+                        and a.next_srf < len(a.srf)
                         and (a.fail_idx is None or a.next_srf < a.fail_idx)):
                     start_trial(j, a, a.next_srf)
                     a.next_srf += 1
@@ -184,80 +196,72 @@ def run_analyses(engine: BatchEngine, analyses: List[Analysis], *, ym, nu,
                     break
             if placed:
                 continue
-            ready = admit_ready(now)
+            ready = [i for i in queue if arrivals is None or arrivals[i] <= now - t0]
             if not ready:
                 continue
-            i = ready[0]
-            queue.remove(i)
-            a = analyses[i]
+            queue.remove(ready[0])
+            a = analyses[ready[0]]
             admitted.append(a)
             start_trial(j, a, a.next_srf)
             a.next_srf += 1
-            if x_el is None:
-                elastic_pending.append(j)
-                return
 
-    def finish_analysis_if_possible(a):
+    def finish_if_possible(a):
         if a.done:
             return
-        # the bracket is known when every SRF before the first failure has
-        # converged (src/slope.F90:441: the sweep stops at the first failure)
+        # the bracket is known once every SRF before the first failure has
+        # converged (the reference's sweep stops at the first failure)
         if a.fail_idx is not None and all(k in a.rows for k in range(a.fail_idx)):
-            prev = a.srf[a.fail_idx - 1] if a.fail_idx > 0 else None
-            a.bracket = (prev, a.srf[a.fail_idx])
+            a.bracket = (a.srf[a.fail_idx - 1] if a.fail_idx > 0 else None,
+                         a.srf[a.fail_idx])
             a.done = True
         elif a.fail_idx is None and len(a.rows) == len(a.srf):
-            a.bracket = None  # no failure in the list
-            a.done = True
-        if a.done:
-            a.t_end = clock()
-            if a.fail_idx is not None:  # rows the reference would not print
-                a.rows = {k: v for k, v in a.rows.items() if k <= a.fail_idx}
-            for j, c in enumerate(cols):  # abandon speculative trials
-                if c.analysis is a:
-                    engine.release(j)
-                    cols[j] = _Col()
-            if on_done:
-                on_done(a)
-
-    fill_free_columns()
-    while True:
-        active = [j for j in range(ncol) if cols[j].analysis is not None]
-        if not active:
-            if queue:
-                fill_free_columns()
-                if not any(c.analysis for c in cols):
-                    time.sleep(0.001)
-                continue
-            break
-        is_last = np.array([1 if (c.analysis is not None and c.nl_iter + 1 == nl_maxiter)
-                            else 0 for c in cols], np.int32)
-        r = engine.plastic_iter(is_last, cg_tol, cg_maxiter, inner_tol, nl_tol)
-        for j in active:
-            c = cols[j]
-            c.nl_iter += 1
-            if r["errcode"][j] != 0:
-                raise RuntimeError("ERROR: PCG solver doesn't converge!")
-            c.cg_tot += int(r["cg_iter"][j])
-            c.fmax = float(r["fmax"][j])
-            if x_el is None and j in elastic_pending and c.nl_iter == 1:
-                x_el, _ = engine.download(j)
-                cg_el = int(r["cg_iter"][j])
-                elastic_pending.clear()
-            if r["nl_isconv"][j] or c.nl_iter == nl_maxiter:
-                a, k = c.analysis, c.srf_idx
-                x, flags = engine.download(j)
-                ux, um = uxmax_umax(spec, x)
-                a.rows[k] = dict(srf=float(a.srf[k]), cg_tot=c.cg_tot,
-                                 nl_tot=c.nl_iter, uxmax=ux, umax=um, fmax=c.fmax)
-                if keep_state:
-                    a.flags_last, a.x_last = flags, x
-                if c.nl_iter == nl_maxiter and (a.fail_idx is None or k < a.fail_idx):
-                    a.fail_idx = k
+            a.done = True  # no failure in the list
+        if not a.done:
+            return
+        a.t_end = clock()
+        if a.fail_idx is not None:  # rows the reference would not print
+            a.rows = {k: v for k, v in a.rows.items() if k <= a.fail_idx}
+        for j, c in enumerate(cols):  # abandon speculative trials
+            if c is not None and c[0] is a:
                 engine.release(j)
-                cols[j] = _Col()
-                finish_analysis_if_possible(a)
-        fill_free_columns()
+                cols[j] = None
+        if on_done:
+            on_done(a)
+
+    fill()
+    while True:
+        if all(c is None for c in cols):
+            if not queue:
+                break
+            nxt = min(arrivals[i] for i in queue) if arrivals is not None else 0.0
+            wait = t0 + nxt - clock()
+            if wait > 0:
+                time.sleep(min(wait, 0.05))
+            fill()
+            continue
+        pending = bool(queue) and arrivals is not None
+        r = engine.advance(cg_tol, cg_maxiter, inner_tol, nl_tol, nl_maxiter,
+                           max_steps=poll_steps if pending else 1 << 40)
+        memo_ready = r["memo_ready"]
+        for j in np.nonzero(r["ended"])[0]:
+            if cols[j] is None:  # abandoned above: its analysis already ended
+                continue
+            a, k = cols[j]
+            if r["errcode"][j] != 0:
+                raise PCGNotConverged("ERROR: PCG solver doesn't converge!")
+            x, flags = engine.download(j)
+            ux, um = uxmax_umax(spec, x)
+            a.rows[k] = dict(srf=float(a.srf[k]), cg_tot=int(r["cg_tot"][j]),
+                             nl_tot=int(r["nl_tot"][j]), uxmax=ux, umax=um,
+                             fmax=float(r["fmax"][j]))
+            if keep_state:
+                a.flags_last, a.x_last = flags, x
+            if r["nl_tot"][j] >= nl_maxiter and (a.fail_idx is None or k < a.fail_idx):
+                a.fail_idx = k
+            engine.release(j)
+            cols[j] = None
+            finish_if_possible(a)
+        fill()
     return engine.stats()
 
 

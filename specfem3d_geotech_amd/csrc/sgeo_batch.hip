@@ -181,6 +181,9 @@ void k_bmv_mfma(int32_t nelmt, int B,
 // each summed over c = 0..80 in order.  Columns col0 .. col0+B-1 of arrays
 // with column stride Bs; columns not in want_mask read zero, write nothing.
 // --------------------------------------------------------------------------
+// Persistent: a few blocks per CU walk the elements, so a launch in which no
+// column is wanted (the common case: the kernel is launched every step and
+// only starting or replacing columns need it) costs a handful of blocks.
 __global__ __launch_bounds__(256) void k_bmv64(int32_t nelmt, int B, int Bs, int col0,
                                                const double* __restrict__ KP,
                                                const int32_t* __restrict__ gdof,
@@ -192,28 +195,31 @@ __global__ __launch_bounds__(256) void k_bmv64(int32_t nelmt, int B, int Bs, int
   double* u = reinterpret_cast<double*>(smem);
   double* pe = u + SGEO_KPACK;
   const int t = threadIdx.x, nt = blockDim.x;
-  // skip the element entirely when no column of this launch is wanted
   bool anyw = false;
   for (int q = 0; q < B; q++) anyw = anyw || wanted(cst, col0 + q, want_mask);
-  if (!anyw) return;  // uniform per block
-  const int64_t e = blockIdx.x;
-  const double* src = KP + e * SGEO_KPACK;
-  for (int q = t; q < SGEO_KPACK; q += nt) u[q] = src[q];
-  for (int q = t; q < 81 * B; q += nt) {
-    const int c = q / B, j = q - c * B;
-    pe[q] = wanted(cst, col0 + j, want_mask)
-                ? X[(int64_t)gdof[e * 81 + c] * Bs + col0 + j] : 0.0;
-  }
-  __syncthreads();
+  if (!anyw) return;  // uniform per launch
   const int j = t % B;
   const int nrg = nt / B, rg = t / B;
-  if (!wanted(cst, col0 + j, want_mask)) return;
-  for (int r = rg; r < 81; r += nrg) {
-    const int rb = r * (r + 1) / 2;
-    double a = 0.0;
-    for (int c = 0; c < r; c++) a = a + u[rb + c] * pe[c * B + j];
-    for (int c = r; c < 81; c++) a = a + u[c * (c + 1) / 2 + r] * pe[c * B + j];
-    Y[(e * 81 + r) * Bs + col0 + j] = a;
+  const bool mine = wanted(cst, col0 + j, want_mask);
+  for (int64_t e = blockIdx.x; e < nelmt; e += gridDim.x) {
+    const double* src = KP + e * SGEO_KPACK;
+    for (int q = t; q < SGEO_KPACK; q += nt) u[q] = src[q];
+    for (int q = t; q < 81 * B; q += nt) {
+      const int c = q / B, jj = q - c * B;
+      pe[q] = wanted(cst, col0 + jj, want_mask)
+                  ? X[(int64_t)gdof[e * 81 + c] * Bs + col0 + jj] : 0.0;
+    }
+    __syncthreads();
+    if (mine) {
+      for (int r = rg; r < 81; r += nrg) {
+        const int rb = r * (r + 1) / 2;
+        double a = 0.0;
+        for (int c = 0; c < r; c++) a = a + u[rb + c] * pe[c * B + j];
+        for (int c = r; c < 81; c++) a = a + u[c * (c + 1) / 2 + r] * pe[c * B + j];
+        Y[(e * 81 + r) * Bs + col0 + j] = a;
+      }
+    }
+    __syncthreads();  // LDS reused by the next element
   }
 }
 
@@ -432,7 +438,6 @@ __global__ __launch_bounds__(256) void k_bfinalize(int64_t nch, int B,
   switch (what) {
     case BFIN_RESID:  // v0 = r.z of the FP64 residual
       c.nrep += 1;
-      c.req = 0;
       if (st == BCOL_START) {
         c.rz = v0;
         c.rz0 = v0;
@@ -461,10 +466,13 @@ __global__ __launch_bounds__(256) void k_bfinalize(int64_t nch, int B,
         cst[j] = BCOL_FAIL;
       } else {
         c.rz_prev = c.rz;
-        c.beta = v1 / c.rz;
-        c.rz = v1;
-        c.pending = 1;
-        if (v1 <= c.inner_tol * c.rz0) c.req = 1;
+        if (v1 <= c.inner_tol * c.rz0) {
+          cst[j] = BCOL_REPLACE;  // FP64 residual at the column's next step
+        } else {
+          c.beta = v1 / c.rz;
+          c.rz = v1;
+          c.pending = 1;
+        }
       }
       break;
     }
@@ -626,15 +634,19 @@ void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu,
                        Y, cst, want_mask);
 }
 
-// FP64: columns in chunks of at most 32 (LDS <= 48 KB per block)
-void launch_bmv_f64(hipStream_t s, int32_t nelmt, int B, const double* KP,
+// FP64: columns in chunks of at most 32 (LDS <= 48 KB per block), 3 blocks
+// per CU
+void launch_bmv_f64(hipStream_t s, int32_t nelmt, int B, int n_cu, const double* KP,
                     const int32_t* gdof, const double* X, double* Y,
                     const int32_t* cst, int want_mask) {
   const int Bc = B < 32 ? B : 32;
   const size_t lds = SGEO_KPACK * sizeof(double) + (size_t)81 * Bc * sizeof(double);
+  int64_t g = (int64_t)n_cu * 3;
+  if (g > nelmt) g = nelmt;
+  if (g < 1) g = 1;
   for (int c0 = 0; c0 < B; c0 += Bc)
-    hipLaunchKernelGGL(k_bmv64, dim3(nelmt), dim3(bmv_threads(Bc)), lds, s, nelmt,
-                       Bc, B, c0, KP, gdof, X, Y, cst, want_mask);
+    hipLaunchKernelGGL(k_bmv64, dim3((unsigned)g), dim3(bmv_threads(Bc)), lds, s,
+                       nelmt, Bc, B, c0, KP, gdof, X, Y, cst, want_mask);
 }
 
 void launch_bassemble_dot(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,

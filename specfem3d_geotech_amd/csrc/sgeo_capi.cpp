@@ -841,12 +841,20 @@ extern "C" int sgeo_slope_download(sgeo_slope* t, double* x, double* bodyload,
 // --------------------------------------------------------------------------
 // multi-trial mixed-precision engine
 // --------------------------------------------------------------------------
+// Host bookkeeping per column: a trial is a sequence of plastic iterations
+// (src/slope.F90:285-392); each iteration is a PCG call (device state
+// machine, BColState) followed by the element x GLL body.  Columns advance
+// independently: a column whose PCG finished gets its body pass at the next
+// host synchronisation and starts its next PCG call, while the others keep
+// iterating.
+enum BatchPhase { BPH_IDLE = 0, BPH_RUN = 1, BPH_ENDED = 2 };
+
 struct sgeo_batch {
   sgeo_mesh* mesh = nullptr;
   int B = 0;
   int64_t n1 = 0;
   double *d_x = nullptr, *d_oldx = nullptr, *d_load = nullptr, *d_body = nullptr;
-  double *d_Y64 = nullptr, *d_ext = nullptr;
+  double *d_Y64 = nullptr, *d_ext = nullptr, *d_xel = nullptr;
   float *d_r = nullptr, *d_p = nullptr, *d_kp = nullptr;
   float *d_Y32 = nullptr, *d_dinv32 = nullptr;
   double *d_evpt = nullptr, *d_stress = nullptr, *d_BL = nullptr;
@@ -855,10 +863,18 @@ struct sgeo_batch {
   PlasticScalars *d_ps = nullptr, *h_ps = nullptr;
   BCol *d_bc = nullptr, *h_bc = nullptr;
   int32_t *d_cst = nullptr, *h_cst = nullptr;
-  int32_t *d_active = nullptr, *d_islast = nullptr, *d_cols = nullptr;
+  int32_t *d_mask = nullptr, *d_islast = nullptr, *d_cols = nullptr;
   double* d_part = nullptr;
-  std::vector<int32_t> active, memo_cg, memo_pending;
+  // per column
+  std::vector<int32_t> phase, nl_iter, cg_tot, need_start, seeded, seed_cg, err;
+  std::vector<int32_t> last_it, last_rep;  // counters at the last synchronisation
+  std::vector<int32_t> cst;  // host copy of the device states
+  std::vector<double> fmax, uerr;
   std::vector<MatParams> h_mp;
+  // elastic memo: the first PCG call of every trial solves K x = extload
+  // from x = 0 (bodyload is zero), whatever the trial's strengths
+  bool memo_valid = false;
+  int32_t memo_cg = 0;
   int64_t stat_inner = 0, stat_outer = 0, stat_steps = 0;
   int k_ver = -1;  // mesh stiffness version the FP32 diagonal was made from
   size_t bytes = 0;
@@ -883,6 +899,7 @@ static int batch_sync_operator(sgeo_batch* b) {
   if (b->k_ver != m->k_ver) {
     launch_to_f32(m->ctx->stream, b->n1, m->d_dinv, b->d_dinv32);
     b->k_ver = m->k_ver;
+    b->memo_valid = false;
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -911,6 +928,7 @@ extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extlo
   TRY(dalloc(&b->d_body, nv, acc));
   TRY(dalloc(&b->d_Y64, ny, acc));
   TRY(dalloc(&b->d_ext, (size_t)b->n1, acc));
+  TRY(dalloc(&b->d_xel, (size_t)b->n1, acc));
   TRY(dalloc(&b->d_r, nv, acc));
   TRY(dalloc(&b->d_p, nv, acc));
   TRY(dalloc(&b->d_kp, nv, acc));
@@ -924,21 +942,32 @@ extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extlo
   TRY(dalloc(&b->d_ps, (size_t)ncol, acc));
   TRY(dalloc(&b->d_bc, (size_t)ncol, acc));
   TRY(dalloc(&b->d_cst, (size_t)ncol, acc));
-  TRY(dalloc(&b->d_active, (size_t)ncol, acc));
+  TRY(dalloc(&b->d_mask, (size_t)ncol, acc));
   TRY(dalloc(&b->d_islast, (size_t)ncol, acc));
   TRY(dalloc(&b->d_cols, (size_t)ncol, acc));
   TRY(dalloc(&b->d_part, (size_t)batch_partials(b->n1) * ncol, acc));
   HIPCHK(hipHostMalloc((void**)&b->h_ps, sizeof(PlasticScalars) * ncol));
   HIPCHK(hipHostMalloc((void**)&b->h_bc, sizeof(BCol) * ncol));
   HIPCHK(hipHostMalloc((void**)&b->h_cst, sizeof(int32_t) * ncol));
-  b->active.assign(ncol, 0);
-  b->memo_cg.assign(ncol, 0);
-  b->memo_pending.assign(ncol, 0);
+  b->phase.assign(ncol, BPH_IDLE);
+  b->nl_iter.assign(ncol, 0);
+  b->cg_tot.assign(ncol, 0);
+  b->need_start.assign(ncol, 0);
+  b->seeded.assign(ncol, 0);
+  b->seed_cg.assign(ncol, 0);
+  b->err.assign(ncol, 0);
+  b->last_it.assign(ncol, 0);
+  b->last_rep.assign(ncol, 0);
+  b->cst.assign(ncol, BCOL_OFF);
+  b->fmax.assign(ncol, 0.0);
+  b->uerr.assign(ncol, 0.0);
+  memset(b->h_bc, 0, sizeof(BCol) * ncol);
   b->h_mp.assign((size_t)std::max(1, m->nmat) * ncol, MatParams{});
   HIPCHK(hipMemcpyAsync(b->d_ext, extload, b->n1 * 8, hipMemcpyHostToDevice, s));
   TRY(batch_sync_operator(b));
-  HIPCHK(hipMemsetAsync(b->d_active, 0, ncol * 4, s));
+  HIPCHK(hipMemsetAsync(b->d_mask, 0, ncol * 4, s));
   HIPCHK(hipMemsetAsync(b->d_cst, 0, ncol * 4, s));
+  HIPCHK(hipMemsetAsync(b->d_bc, 0, ncol * sizeof(BCol), s));
   HIPCHK(hipMemsetAsync(b->d_ps, 0, ncol * sizeof(PlasticScalars), s));
   HIPCHK(hipMemsetAsync(b->d_x, 0, nv * 8, s));
   HIPCHK(hipGetLastError());
@@ -951,11 +980,11 @@ extern "C" int sgeo_batch_destroy(sgeo_batch* b) {
   if (!b) return 0;
   hipSetDevice(b->mesh->ctx->device);
   hipStreamSynchronize(b->mesh->ctx->stream);
-  void* ptrs[] = {b->d_x, b->d_oldx, b->d_load, b->d_body, b->d_Y64, b->d_ext,
-                  b->d_r, b->d_p, b->d_kp, b->d_Y32,
-                  b->d_dinv32, b->d_evpt, b->d_stress, b->d_BL, b->d_flags,
-                  b->d_mp, b->d_ps, b->d_bc, b->d_cst, b->d_active, b->d_islast,
-                  b->d_cols, b->d_part};
+  void* ptrs[] = {b->d_x,     b->d_oldx,   b->d_load,   b->d_body,  b->d_Y64,
+                  b->d_ext,   b->d_xel,    b->d_r,      b->d_p,     b->d_kp,
+                  b->d_Y32,   b->d_dinv32, b->d_evpt,   b->d_stress, b->d_BL,
+                  b->d_flags, b->d_mp,     b->d_ps,     b->d_bc,    b->d_cst,
+                  b->d_mask,  b->d_islast, b->d_cols,   b->d_part};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (b->h_ps) hipHostFree(b->h_ps);
@@ -981,6 +1010,7 @@ extern "C" int sgeo_batch_begin(sgeo_batch* b, int32_t col, const double* ym,
   sgeo_ctx* c = m->ctx;
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
+  TRY(batch_sync_operator(b));
   const int nmat = std::max(1, m->nmat);
   for (int i = 0; i < m->nmat; i++)
     b->h_mp[(size_t)col * nmat + i] =
@@ -988,8 +1018,8 @@ extern "C" int sgeo_batch_begin(sgeo_batch* b, int32_t col, const double* ym,
   HIPCHK(hipMemcpyAsync(b->d_mp + (size_t)col * nmat, &b->h_mp[(size_t)col * nmat],
                         nmat * sizeof(MatParams), hipMemcpyHostToDevice, s));
   const size_t ns = (size_t)m->nelmt * 27 * 6;
-  // stress_elmt=0, evpt=0, bodyload=0, x=0 or x0, oldx=0
-  // (src/slope.F90:249, 281-282)
+  // stress_elmt=0, evpt=0, bodyload=0, oldx=0, and x = 0 or the elastic
+  // solution (src/slope.F90:249, 281-282)
   HIPCHK(hipMemsetAsync(b->d_evpt + col * ns, 0, ns * 8, s));
   HIPCHK(hipMemsetAsync(b->d_stress + col * ns, 0, ns * 8, s));
   HIPCHK(hipMemsetAsync(b->d_flags + (size_t)col * m->nelmt * 27, 0,
@@ -997,117 +1027,162 @@ extern "C" int sgeo_batch_begin(sgeo_batch* b, int32_t col, const double* ym,
   launch_col_io(s, b->n1, b->B, col, b->d_body, nullptr, nullptr);
   launch_col_io(s, b->n1, b->B, col, b->d_oldx, nullptr, nullptr);
   double* d_x0 = nullptr;
+  int seeded = 0, seed_cg = 0;
   if (x0) {
     TRY(dalloc(&d_x0, (size_t)b->n1));
     HIPCHK(hipMemcpyAsync(d_x0, x0, b->n1 * 8, hipMemcpyHostToDevice, s));
+    launch_col_io(s, b->n1, b->B, col, b->d_x, d_x0, nullptr);
+    seeded = 1;
+    seed_cg = x0_cg_iter;
+  } else if (b->memo_valid) {
+    launch_col_io(s, b->n1, b->B, col, b->d_x, b->d_xel, nullptr);
+    seeded = 1;
+    seed_cg = b->memo_cg;
+  } else {
+    launch_col_io(s, b->n1, b->B, col, b->d_x, nullptr, nullptr);
   }
-  launch_col_io(s, b->n1, b->B, col, b->d_x, d_x0, nullptr);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s));
   if (d_x0) hipFree(d_x0);
-  b->active[col] = 1;
-  b->memo_pending[col] = x0 ? 1 : 0;
-  b->memo_cg[col] = x0_cg_iter;
+  b->phase[col] = BPH_RUN;
+  b->need_start[col] = 1;
+  b->nl_iter[col] = 0;
+  b->cg_tot[col] = 0;
+  b->err[col] = 0;
+  b->fmax[col] = 0.0;
+  b->uerr[col] = 0.0;
+  b->seeded[col] = seeded;
+  b->seed_cg[col] = seed_cg;
+  b->cst[col] = BCOL_OFF;
   return 0;
 }
 
 extern "C" int sgeo_batch_release(sgeo_batch* b, int32_t col) {
   if (!b || col < 0 || col >= b->B) return fail(SGEO_EARG, "bad column");
-  b->active[col] = 0;
-  b->memo_pending[col] = 0;
+  b->phase[col] = BPH_IDLE;
+  b->need_start[col] = 0;
+  b->cst[col] = BCOL_OFF;
   return 0;
 }
 
-// One plastic iteration of every active column in lockstep: load assembly,
-// mixed-precision PCG (FP32 inner CG, FP64 residual replacement), uerr,
-// element x GLL body and body-load assembly (src/slope.F90:286-391).
-extern "C" int sgeo_batch_plastic_iter(sgeo_batch* b, const int32_t* is_last,
-                                       double cg_tol, int32_t cg_maxiter,
-                                       double inner_tol, double nl_tol,
-                                       int32_t* cg_iter, int32_t* errcode,
-                                       double* uerr, int32_t* nl_isconv,
-                                       double* fmax) {
-  if (!b || !is_last) return fail(SGEO_EARG, "sgeo_batch_plastic_iter: bad arguments");
-  sgeo_mesh* m = b->mesh;
-  sgeo_ctx* c = m->ctx;
-  HIPCHK(hipSetDevice(c->device));
-  hipStream_t s = c->stream;
-  const int B = b->B;
-  const int64_t n1 = b->n1;
-  std::vector<int32_t> cols, cst(B, BCOL_OFF), islast(B, 0);
-  for (int j = 0; j < B; j++) {
-    if (!b->active[j]) continue;
-    cols.push_back(j);
-    islast[j] = is_last[j] ? 1 : 0;
-    cst[j] = b->memo_pending[j] ? BCOL_DONE : BCOL_START;
+// Small per-call uploads (column masks, lists, flags) come from host vectors
+// that die with the call: each is followed by a stream synchronisation.
+static int upload_sync(sgeo_batch* b, void* dst, const void* src, size_t bytes) {
+  hipStream_t s = b->mesh->ctx->stream;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+static int upload_mask(sgeo_batch* b, const std::vector<int32_t>& cols) {
+  std::vector<int32_t> mask(b->B, 0);
+  for (int j : cols) mask[j] = 1;
+  return upload_sync(b, b->d_mask, mask.data(), b->B * 4);
+}
+
+// Starts a PCG call for `cols`: load = extload + bodyload, fresh scalars;
+// a column seeded with the elastic solution skips its first call.
+static int batch_start(sgeo_batch* b, const std::vector<int32_t>& cols, double cg_tol,
+                       int32_t cg_maxiter, double inner_tol) {
+  hipStream_t s = b->mesh->ctx->stream;
+  TRY(upload_mask(b, cols));
+  launch_bload(s, b->n1, b->B, b->d_ext, b->d_body, b->d_load, b->d_mask);
+  for (int j : cols) {
+    const bool skip = b->seeded[j] && b->nl_iter[j] == 0;
+    b->cst[j] = skip ? BCOL_DONE : BCOL_START;
     BCol& bc = b->h_bc[j];
     bc = BCol{};
     bc.tol = cg_tol;
     bc.inner_tol = inner_tol;
     bc.maxiter = cg_maxiter;
-    bc.it = b->memo_pending[j] ? b->memo_cg[j] : 0;
+    bc.it = skip ? b->seed_cg[j] : 0;
+    b->need_start[j] = 0;
+    b->last_it[j] = bc.it;
+    b->last_rep[j] = 0;
   }
-  if (cols.empty()) return fail(SGEO_ESTATE, "no active column");
-  TRY(batch_sync_operator(b));
-  HIPCHK(hipMemcpyAsync(b->d_active, b->active.data(), B * 4, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(b->d_islast, islast.data(), B * 4, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(b->d_cols, cols.data(), cols.size() * 4,
-                        hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(b->d_cst, cst.data(), B * 4, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(b->d_bc, b->h_bc, B * sizeof(BCol), hipMemcpyHostToDevice, s));
-  launch_bload(s, n1, B, b->d_ext, b->d_body, b->d_load, b->d_active);
-  // Mixed-precision PCG of every active column, in lockstep steps; each
-  // column advances its own state (BColState), so finished columns cost no
-  // vector traffic.  The FP64 residual runs at the start and, for columns
-  // that asked for a replacement, at host synchronisation points.
-  const int64_t max_steps = 2 * (int64_t)cg_maxiter + 64;
-  int64_t steps = 0;
-  const int resid_mask = BCOL_BIT(BCOL_START) | BCOL_BIT(BCOL_REPLACE);
-  bool resid = true;  // START columns
-  for (int chunk = 8;; chunk = std::min(chunk * 2, 32)) {
-    if (resid) {
-      launch_bmv_f64(s, m->nelmt, B, m->d_KP64, m->d_gdof, b->d_x, b->d_Y64,
-                     b->d_cst, resid_mask);
-      launch_bresid(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y64, b->d_load,
-                    b->d_dinv32, b->d_r, b->d_p, b->d_part, b->d_cst);
-      launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_RESID);
-    }
-    for (int it = 0; it < chunk; it++) {
-      launch_bpupdate(s, n1, B, b->d_r, b->d_dinv32, b->d_p, b->d_cst, b->d_bc);
-      launch_bmv_mfma_f32(s, m->nelmt, B, c->n_cu, m->d_KP32, m->d_gdof, b->d_p,
-                          b->d_Y32, b->d_cst, BCOL_BIT(BCOL_INNER));
-      launch_bassemble_dot(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y32, b->d_p,
-                           b->d_kp, b->d_part, b->d_cst);
-      launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_ALPHA);
-      launch_bupdate(s, n1, B, b->d_p, b->d_kp, b->d_dinv32, b->d_x, b->d_r,
-                     b->d_part, b->d_cst, b->d_bc);
-      launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_INNER);
-    }
-    steps += chunk;
-    b->stat_steps += chunk;
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(b->h_cst, b->d_cst, B * 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(b->h_bc, b->d_bc, B * sizeof(BCol), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    bool running = false;
-    resid = false;
-    for (int j : cols) {
-      cst[j] = b->h_cst[j];
-      if (cst[j] == BCOL_INNER && b->h_bc[j].req) {
-        cst[j] = BCOL_REPLACE;
-        resid = true;
-      }
-      running = running || cst[j] == BCOL_INNER || cst[j] == BCOL_REPLACE;
-    }
-    if (!running) break;
-    if (resid)
-      HIPCHK(hipMemcpyAsync(b->d_cst, cst.data(), B * 4, hipMemcpyHostToDevice, s));
-    if (steps > max_steps)  // every step advances each running column
-      return fail(SGEO_ESTATE, "batch PCG did not terminate in %lld steps",
-                  (long long)steps);
+  TRY(upload_sync(b, b->d_cst, b->cst.data(), b->B * 4));
+  HIPCHK(hipMemcpyAsync(b->d_bc, b->h_bc, b->B * sizeof(BCol), hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+static bool pcg_running(int st) {
+  return st == BCOL_START || st == BCOL_INNER || st == BCOL_REPLACE;
+}
+
+// `chunk` lockstep CG steps of every running column, then reads the states
+// back.  Each step first forms the FP64 residual of the columns that start
+// a call or replace their residual (a no-op launch when there are none), so
+// a column's arithmetic depends only on its own iterations.  h_bc holds the
+// device scalars on return.
+static int batch_steps(sgeo_batch* b, int chunk, bool* any_running) {
+  sgeo_mesh* m = b->mesh;
+  sgeo_ctx* c = m->ctx;
+  hipStream_t s = c->stream;
+  const int B = b->B;
+  const int64_t n1 = b->n1;
+  for (int it = 0; it < chunk; it++) {
+    launch_bmv_f64(s, m->nelmt, B, c->n_cu, m->d_KP64, m->d_gdof, b->d_x, b->d_Y64,
+                   b->d_cst, BCOL_BIT(BCOL_START) | BCOL_BIT(BCOL_REPLACE));
+    launch_bresid(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y64, b->d_load, b->d_dinv32,
+                  b->d_r, b->d_p, b->d_part, b->d_cst);
+    launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_RESID);
+    launch_bpupdate(s, n1, B, b->d_r, b->d_dinv32, b->d_p, b->d_cst, b->d_bc);
+    TRY(prof_mark(c));
+    launch_bmv_mfma_f32(s, m->nelmt, B, c->n_cu, m->d_KP32, m->d_gdof, b->d_p,
+                        b->d_Y32, b->d_cst, BCOL_BIT(BCOL_INNER));
+    TRY(prof_mark(c));
+    launch_bassemble_dot(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y32, b->d_p, b->d_kp,
+                         b->d_part, b->d_cst);
+    launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_ALPHA);
+    launch_bupdate(s, n1, B, b->d_p, b->d_kp, b->d_dinv32, b->d_x, b->d_r, b->d_part,
+                   b->d_cst, b->d_bc);
+    launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_INNER);
   }
-  // uerr, body, body load for every active column
-  launch_buerr(s, n1, B, b->d_x, b->d_oldx, b->d_part, b->d_ps, b->d_active, nl_tol);
+  b->stat_steps += chunk;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(b->h_cst, b->d_cst, B * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(b->h_bc, b->d_bc, B * sizeof(BCol), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (c->prof) TRY(prof_flush(c));
+  bool running = false;
+  for (int j = 0; j < B; j++) {
+    if (b->phase[j] != BPH_RUN) continue;
+    if (b->cst[j] == BCOL_OFF) continue;  // waiting for its start
+    b->stat_inner += b->h_bc[j].it - b->last_it[j];
+    b->stat_outer += b->h_bc[j].nrep - b->last_rep[j];
+    b->last_it[j] = b->h_bc[j].it;
+    b->last_rep[j] = b->h_bc[j].nrep;
+    b->cst[j] = b->h_cst[j];
+    running = running || pcg_running(b->cst[j]);
+  }
+  *any_running = running;
+  return 0;
+}
+
+// PCG finished for `cols` (DONE or FAIL): keeps the elastic memo, then
+// x(0)=0, uerr and nl_isconv, the element x GLL body and the body load
+// (src/slope.F90:297-391) for those columns.  Updates nl_iter, cg_tot,
+// fmax, uerr and err; the columns are left with cst OFF.
+static int batch_body(sgeo_batch* b, const std::vector<int32_t>& cols, double nl_tol,
+                      int32_t nl_maxiter, const int32_t* is_last_override) {
+  sgeo_mesh* m = b->mesh;
+  hipStream_t s = m->ctx->stream;
+  const int B = b->B;
+  for (int j : cols) {  // first PCG call of a trial from x = 0: the elastic memo
+    if (!b->memo_valid && b->nl_iter[j] == 0 && !b->seeded[j] && b->cst[j] == BCOL_DONE) {
+      launch_col_io(s, b->n1, B, j, b->d_x, nullptr, b->d_xel);
+      b->memo_valid = true;
+      b->memo_cg = b->h_bc[j].it;
+    }
+  }
+  std::vector<int32_t> islast(B, 0);
+  for (int j : cols)
+    islast[j] = is_last_override ? (is_last_override[j] ? 1 : 0)
+                                 : (b->nl_iter[j] + 1 == nl_maxiter ? 1 : 0);
+  TRY(upload_mask(b, cols));
+  TRY(upload_sync(b, b->d_islast, islast.data(), B * 4));
+  TRY(upload_sync(b, b->d_cols, cols.data(), cols.size() * 4));
+  launch_buerr(s, b->n1, B, b->d_x, b->d_oldx, b->d_part, b->d_ps, b->d_mask, nl_tol);
   BodyArgs ba;
   ba.nel = m->nelmt;
   ba.corners = m->d_corners;
@@ -1126,29 +1201,132 @@ extern "C" int sgeo_batch_plastic_iter(sgeo_batch* b, const int32_t* is_last,
   ba.cols = b->d_cols;
   ba.ncols = (int)cols.size();
   launch_plastic_body(s, ba);
-  launch_bbody_assemble(s, n1, B, m->nelmt, m->d_rowptr, m->d_slots, b->d_BL,
-                        b->d_body, b->d_active, b->d_islast, b->d_ps);
+  launch_bbody_assemble(s, b->n1, B, m->nelmt, m->d_rowptr, m->d_slots, b->d_BL,
+                        b->d_body, b->d_mask, b->d_islast, b->d_ps);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(b->h_ps, b->d_ps, B * sizeof(PlasticScalars),
                         hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(b->h_bc, b->d_bc, B * sizeof(BCol), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  for (int j = 0; j < B; j++) {
-    if (!b->active[j]) continue;
-    if (!b->memo_pending[j]) {
-      b->stat_inner += b->h_bc[j].it;
-      b->stat_outer += b->h_bc[j].nrep;
-    }
-    if (cg_iter) cg_iter[j] = b->h_bc[j].it;
-    if (errcode) errcode[j] = cst[j] == BCOL_DONE ? 0 : -1;
-    if (uerr) uerr[j] = b->h_ps[j].uerr;
-    if (nl_isconv) nl_isconv[j] = b->h_ps[j].nl_isconv;
-    if (fmax) {
-      unsigned long long bits = b->h_ps[j].fmax_bits;
-      memcpy(&fmax[j], &bits, 8);
-    }
-    b->memo_pending[j] = 0;
+  for (int j : cols) {
+    b->nl_iter[j] += 1;
+    b->cg_tot[j] += b->h_bc[j].it;
+    if (b->cst[j] != BCOL_DONE) b->err[j] = -1;
+    b->uerr[j] = b->h_ps[j].uerr;
+    unsigned long long bits = b->h_ps[j].fmax_bits;
+    memcpy(&b->fmax[j], &bits, 8);
+    b->cst[j] = BCOL_OFF;
   }
+  return 0;
+}
+
+// One plastic iteration of every running column in lockstep
+// (src/slope.F90:286-391).
+extern "C" int sgeo_batch_plastic_iter(sgeo_batch* b, const int32_t* is_last,
+                                       double cg_tol, int32_t cg_maxiter,
+                                       double inner_tol, double nl_tol,
+                                       int32_t* cg_iter, int32_t* errcode,
+                                       double* uerr, int32_t* nl_isconv,
+                                       double* fmax) {
+  if (!b || !is_last) return fail(SGEO_EARG, "sgeo_batch_plastic_iter: bad arguments");
+  sgeo_ctx* c = b->mesh->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  const int B = b->B;
+  std::vector<int32_t> cols;
+  for (int j = 0; j < B; j++)
+    if (b->phase[j] == BPH_RUN) cols.push_back(j);
+  if (cols.empty()) return fail(SGEO_ESTATE, "no active column");
+  TRY(batch_sync_operator(b));
+  std::vector<int32_t> cg0(B, 0);
+  for (int j : cols) cg0[j] = b->cg_tot[j];
+  TRY(batch_start(b, cols, cg_tol, cg_maxiter, inner_tol));
+  const int64_t max_steps = 2 * (int64_t)cg_maxiter + 64;
+  int64_t steps = 0;
+  bool running = false;
+  for (int j : cols) running = running || pcg_running(b->cst[j]);
+  for (int chunk = 8; running; chunk = std::min(chunk * 2, 32)) {
+    TRY(batch_steps(b, chunk, &running));
+    steps += chunk;
+    if (running && steps > max_steps)  // every step advances each running column
+      return fail(SGEO_ESTATE, "batch PCG did not terminate in %lld steps",
+                  (long long)steps);
+  }
+  TRY(batch_body(b, cols, nl_tol, 0, is_last));
+  for (int j : cols) {
+    if (cg_iter) cg_iter[j] = b->cg_tot[j] - cg0[j];
+    if (errcode) errcode[j] = b->err[j];
+    if (uerr) uerr[j] = b->uerr[j];
+    if (nl_isconv) nl_isconv[j] = b->h_ps[j].nl_isconv;
+    if (fmax) fmax[j] = b->fmax[j];
+    b->err[j] = 0;
+  }
+  return 0;
+}
+
+// Runs every running column through its own sequence of plastic iterations
+// until at least one trial ends (nl_isconv, nl_maxiter reached, or a PCG
+// call that did not converge), the elastic memo becomes available, or
+// max_steps CG steps have run.
+extern "C" int sgeo_batch_advance(sgeo_batch* b, double cg_tol, int32_t cg_maxiter,
+                                  double inner_tol, double nl_tol, int32_t nl_maxiter,
+                                  int64_t max_steps, int32_t* ended, int32_t* nl_tot,
+                                  int32_t* cg_tot, double* fmax, int32_t* errcode,
+                                  int32_t* memo_ready) {
+  if (!b || !ended || nl_maxiter < 1)
+    return fail(SGEO_EARG, "sgeo_batch_advance: bad arguments");
+  sgeo_ctx* c = b->mesh->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(batch_sync_operator(b));
+  const int B = b->B;
+  for (int j = 0; j < B; j++) ended[j] = 0;
+  const bool memo0 = b->memo_valid;
+  int64_t steps = 0;
+  int chunk = 8;
+  for (;;) {
+    // columns that need a new PCG call
+    std::vector<int32_t> starts;
+    for (int j = 0; j < B; j++)
+      if (b->phase[j] == BPH_RUN && b->need_start[j]) starts.push_back(j);
+    if (!starts.empty()) {
+      TRY(batch_start(b, starts, cg_tol, cg_maxiter, inner_tol));
+      chunk = 8;
+    }
+    bool running = false;
+    for (int j = 0; j < B; j++)
+      running = running || (b->phase[j] == BPH_RUN && pcg_running(b->cst[j]));
+    if (running) {
+      TRY(batch_steps(b, chunk, &running));
+      steps += chunk;
+      chunk = std::min(chunk * 2, 32);
+    }
+    // columns whose PCG call finished: body pass
+    std::vector<int32_t> fin;
+    for (int j = 0; j < B; j++)
+      if (b->phase[j] == BPH_RUN && (b->cst[j] == BCOL_DONE || b->cst[j] == BCOL_FAIL))
+        fin.push_back(j);
+    bool any_end = false;
+    if (!fin.empty()) {
+      TRY(batch_body(b, fin, nl_tol, nl_maxiter, nullptr));
+      for (int j : fin) {
+        const bool end = b->err[j] != 0 || b->h_ps[j].nl_isconv ||
+                         b->nl_iter[j] >= nl_maxiter;
+        if (end) {
+          b->phase[j] = BPH_ENDED;
+          ended[j] = 1;
+          any_end = true;
+          if (nl_tot) nl_tot[j] = b->nl_iter[j];
+          if (cg_tot) cg_tot[j] = b->cg_tot[j];
+          if (fmax) fmax[j] = b->fmax[j];
+          if (errcode) errcode[j] = b->err[j];
+        } else {
+          b->need_start[j] = 1;
+        }
+      }
+    }
+    bool busy = false;
+    for (int j = 0; j < B; j++) busy = busy || b->phase[j] == BPH_RUN;
+    if (any_end || !busy || (b->memo_valid && !memo0) || steps >= max_steps) break;
+  }
+  if (memo_ready) *memo_ready = b->memo_valid ? 1 : 0;
   return 0;
 }
 

@@ -98,10 +98,11 @@ void launch_plastic_body(hipStream_t s, const BodyArgs& b);
 // ---- multi-trial mixed-precision engine (sgeo_batch.hip) -----------------
 // Per-column states of the mixed-precision PCG.  The inner iteration runs in
 // FP32 (operator, r, p, kp) with FP64 dot products and an FP64 solution x;
-// the FP64 true residual f - K x (FP64 operator) starts each solve and
-// replaces the recursive FP32 residual once r.z has dropped by inner_tol
-// since the last replacement (requested by the column, carried out at the
-// next host synchronisation), keeping the search direction.
+// the FP64 true residual f - K x (FP64 operator) starts each solve and, at
+// the column's next step, replaces the recursive FP32 residual once r.z has
+// dropped by inner_tol since the last replacement, keeping the search
+// direction.  Everything a column computes depends only on its own
+// iteration count, never on when the host synchronises.
 enum BColState {
   BCOL_OFF = 0,      // column not in this solve
   BCOL_START = 1,    // next: FP64 residual, p = D r
@@ -122,15 +123,13 @@ struct BCol {  // per-column scalars, device resident
   int32_t it, maxiter;  // CG iterations of this solve, limit
   int32_t pending;      // 1: p = D r + beta p before the next product
   int32_t nrep;         // FP64 residuals of this solve
-  int32_t req;          // 1: the column asks for an FP64 residual
-  int32_t pad_;
 };
 
 int64_t batch_partials(int64_t n1);
 void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu,
                          const float* KP, const int32_t* gdof, const float* P,
                          float* Y, const int32_t* cst, int want_mask);
-void launch_bmv_f64(hipStream_t s, int32_t nelmt, int B, const double* KP,
+void launch_bmv_f64(hipStream_t s, int32_t nelmt, int B, int n_cu, const double* KP,
                     const int32_t* gdof, const double* X, double* Y,
                     const int32_t* cst, int want_mask);
 void launch_bassemble_dot(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,

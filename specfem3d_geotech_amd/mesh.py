@@ -18,6 +18,8 @@ reference's driver holds before the hot path starts.
 """
 from __future__ import annotations
 
+import dataclasses
+
 import math
 import os
 from dataclasses import dataclass, field
@@ -60,6 +62,13 @@ class HexMesh:
         m = self.materials
         return tuple(np.array([getattr(b, k) for b in m], dtype=np.float64)
                      for k in ("gam", "ym", "nu", "phi", "coh", "psi"))
+
+    def with_strengths(self, coh, phi) -> "HexMesh":
+        """The same mesh with the materials' cohesion and friction angle
+        replaced (per material block)."""
+        mats = [dataclasses.replace(b, coh=float(coh[i]), phi=float(phi[i]))
+                for i, b in enumerate(self.materials)]
+        return dataclasses.replace(self, materials=mats)
 
 
 def slope_surface(x: np.ndarray, *, z_crest: float, z_toe: float,

@@ -98,12 +98,14 @@ def uxmax_umax(spec: SpecMesh, x: np.ndarray):
 def slope3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
             nl_tol: float, nl_maxiter: int, srf: Sequence[float],
             phinu: bool = False, kprec: int = K_F64, kfmt: int = 0,
-            spec: Optional[SpecMesh] = None, progress=None) -> SlopeResult:
+            spec: Optional[SpecMesh] = None, progress=None,
+            iter_progress=None) -> SlopeResult:
     """Strength-reduction sweep of src/slope.F90:244-443 on the GPU.
 
     kprec/kfmt select the operator the PCG streams (FP64 full reproduces
     the reference's arithmetic; packed halves the bytes).  `progress`, if
-    given, is called with each summary row."""
+    given, is called with each summary row, `iter_progress` with (nl_iter,
+    cg_iter) after each plastic iteration."""
     spec = spec or hex2spec(mesh)
     gam, ym, nu, phi, coh, psi = mesh.material_arrays()
     dm = DeviceMesh(ctx, spec, len(gam))
@@ -132,6 +134,8 @@ def slope3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
                 r = st.plastic_iter(cg_tol, cg_maxiter, nl_tol,
                                     nl_iter == nl_maxiter)
                 cg_log.append(r["cg_iter"])
+                if iter_progress:
+                    iter_progress(nl_iter, r["cg_iter"])
                 if r["errcode"] != 0:
                     raise PCGNotConverged("ERROR: PCG solver doesn't converge!")
                 cg_tot += r["cg_iter"]

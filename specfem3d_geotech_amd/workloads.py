@@ -53,3 +53,28 @@ WORKLOADS = {
 
 def get(name: str) -> Workload:
     return WORKLOADS[name]
+
+
+def form_a(n: int, nmat: int, seed: int = 3, c_mean: float = 10.0, c_cov: float = 0.3,
+           phi_mean_deg: float = 20.0, tan_cov: float = 0.2):
+    """Strength realisations of the batched configuration, form A: lognormal
+    c (mean 10 kPa, CoV 0.3) and lognormal tan(phi) (mean tan 20 deg, CoV
+    0.2), spatially uniform per trial (every material of the mesh takes the
+    trial's c and phi).  Returns [(coh[nmat], phi_deg[nmat])] * n."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+
+    def lognormal(mean, cov, size):
+        s2 = np.log1p(cov * cov)
+        return rng.lognormal(np.log(mean) - 0.5 * s2, np.sqrt(s2), size)
+
+    c = lognormal(c_mean, c_cov, n)
+    t = lognormal(np.tan(np.radians(phi_mean_deg)), tan_cov, n)
+    phi = np.degrees(np.arctan(t))
+    return [(np.full(nmat, c[i]), np.full(nmat, phi[i])) for i in range(n)]
+
+
+def form_b(n: int = 64, lo: float = 1.0, hi: float = 2.0):
+    """Form B: n SRF values on [lo, hi] for one strength set."""
+    import numpy as np
+    return [float(s) for s in np.linspace(lo, hi, n)]

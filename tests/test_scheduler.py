@@ -17,37 +17,58 @@ NL_MAX = 12
 
 
 class StandIn:
+    """Engine stand-in: every running column does one plastic iteration per
+    round; a round ends the call when a trial ends or the elastic memo
+    appears (as sgeo_batch_advance does)."""
+
     def __init__(self, ncol):
         self.ncol = ncol
         self.dmesh = SimpleNamespace(spec=SimpleNamespace(gdof=np.array([[1, 2, 3]])))
         self.col = [None] * ncol
         self.begins = []
+        self.active_per_call = []
+        self.memo = False
         self.steps = 0
 
     def begin(self, col, ym, nuf, phif, cohf, psif, dt, x0=None, x0_cg=0):
         assert self.col[col] is None, "column reused before release"
         c = float(cohf[0])
         need = NL_MAX if c < 5.0 else 1 + int(round(20.0 / c))
-        self.col[col] = dict(c=c, need=need, it=0, memo=x0 is not None, x0_cg=x0_cg)
-        self.begins.append((col, c, x0 is not None))
+        self.col[col] = dict(c=c, need=need, it=0, cg=0, seeded=self.memo, ended=False)
+        self.begins.append((col, c, self.memo))
 
     def release(self, col):
         self.col[col] = None
 
-    def plastic_iter(self, is_last, cg_tol, cg_maxiter, inner_tol, nl_tol):
-        self.steps += 1
+    def advance(self, cg_tol, cg_maxiter, inner_tol, nl_tol, nl_maxiter,
+                max_steps=1 << 40):
         n = self.ncol
-        out = dict(cg_iter=np.zeros(n, np.int32), errcode=np.zeros(n, np.int32),
-                   uerr=np.zeros(n), nl_isconv=np.zeros(n, np.int32), fmax=np.zeros(n))
-        for j, s in enumerate(self.col):
-            if s is None:
-                continue
-            s["it"] += 1
-            first = s["it"] == 1
-            out["cg_iter"][j] = s["x0_cg"] if (first and s["memo"]) else 10 + s["it"]
-            out["nl_isconv"][j] = 1 if s["it"] >= s["need"] and s["need"] < NL_MAX else 0
-            out["fmax"][j] = s["c"]
-            assert bool(is_last[j]) == (s["it"] == NL_MAX)
+        out = dict(ended=np.zeros(n, np.int32), nl_tot=np.zeros(n, np.int32),
+                   cg_tot=np.zeros(n, np.int32), fmax=np.zeros(n),
+                   errcode=np.zeros(n, np.int32), memo_ready=self.memo)
+        self.active_per_call.append(sum(1 for s in self.col
+                                        if s is not None and not s["ended"]))
+        memo0 = self.memo
+        while True:
+            any_end = False
+            for j, s in enumerate(self.col):
+                if s is None or s["ended"]:
+                    continue
+                s["it"] += 1
+                s["cg"] += 11 if (s["it"] == 1) else 10 + s["it"]
+                if s["it"] == 1 and not s["seeded"]:
+                    self.memo = True
+                if (s["it"] >= s["need"] and s["need"] < NL_MAX) or s["it"] >= nl_maxiter:
+                    s["ended"] = True
+                    any_end = True
+                    out["ended"][j] = 1
+                    out["nl_tot"][j] = s["it"]
+                    out["cg_tot"][j] = s["cg"]
+                    out["fmax"][j] = s["c"]
+            self.steps += 1
+            if any_end or (self.memo and not memo0) or self.steps % max(1, max_steps) == 0:
+                break
+        out["memo_ready"] = self.memo
         return out
 
     def download(self, col):
@@ -102,12 +123,12 @@ def test_brackets_and_rows(ncol, srf_parallel):
 
 def test_elastic_memo_used_after_first_trial():
     eng, an = _run(4, COHS, SRF)
-    # the first trial solves the elastic system itself; later ones reuse it
+    # the first trial solves the elastic system alone; later ones reuse it
     assert eng.begins[0][2] is False
     assert all(m for (_, _, m) in eng.begins[1:])
-    # memo reports the first solve's iteration count (11) for every trial
+    assert eng.active_per_call[0] == 1 and max(eng.active_per_call) == 4
     r0 = an[0].rows[0]
-    assert r0["cg_tot"] == sum(10 + i for i in range(1, r0["nl_tot"] + 1))
+    assert r0["cg_tot"] == 11 + sum(10 + i for i in range(2, r0["nl_tot"] + 1))
 
 
 def test_arrivals_delay_admission():
