@@ -99,12 +99,23 @@ def single_trial_pcg(ctx, dm, ext, ym, nu, phi, coh, psi, steps, warm):
     return dt_s, n, ms
 
 
-def roofline(kernel, bytes_per_launch, n, ms, extra=None):
+def measured_traffic(kernel_key):
+    """HBM bytes per launch of a kernel from the committed PMC summary
+    (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE runs scaled by what
+    the counters read for a copy of known size), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
+            return json.load(fh)[kernel_key]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def roofline(kernel, bytes_per_launch, n, ms, extra=None, traffic=None):
     avg = ms / max(n, 1)
     ach = bytes_per_launch / (avg * 1e-3) / 1e9 if n else 0.0
     out = dict(kernel=kernel, bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS,
                unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4), avg_launch_ms=round(avg, 4),
-               launches=n, bytes_per_launch=int(bytes_per_launch), traffic=None)
+               launches=n, bytes_per_launch=int(bytes_per_launch), traffic=traffic)
     if extra:
         out.update(extra)
     return out
@@ -218,7 +229,11 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
         roofline=roofline("k_bmv_mfma<4> (v_mfma_f32_16x16x4f32)", kbytes, nl, kms,
                           dict(bytes_model="E*(3328*4 + 81*4 + 2*81*4*b_eff), "
                                "b_eff = mean trials iterating per step",
-                               b_eff=round(b_eff, 2))))
+                               b_eff=round(b_eff, 2),
+                               traffic_source="profiles/bmv_mfma_c1_b64_traffic.md "
+                               "(c1, 64 trials iterating)"),
+                          traffic=measured_traffic("k_bmv_mfma<4>")
+                          if (a.workload == "c1" and B == 64) else None))
     return res
 
 

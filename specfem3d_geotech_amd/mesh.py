@@ -323,6 +323,16 @@ def read_reference_input(sem_path: str) -> tuple:
                    nl_maxiter=int(ctl.get("nl_maxiter", 100)),
                    srf=[float(s) for s in ctl.get("srf", "1.0").split()],
                    phinu=ctl.get("phinu", "0").strip() == "1")
+    # excavation keys (src/read_input.f90:351-373): nexcav stages, stage i
+    # removes the nexcavid(i) material ids that follow in excavid
+    nexcav = int(ctl.get("nexcav", "0"))
+    if nexcav > 0:
+        nid = [int(v) for v in ctl.get("nexcavid", "").split()] or [1] * nexcav
+        control.update(nexcav=nexcav, nexcavid=nid,
+                       excavid=[int(v) for v in ctl["excavid"].split()][:sum(nid)],
+                       ninc=int(ctl.get("ninc", "1")))
+    st0 = kv.get("stress0", {})
+    control["s0_type"] = int(float(st0.get("type", "0")))
     mesh = HexMesh(coord=coord, conn=conn, mat_id=mat_id, materials=mats, bc=bc,
                    meta=dict(kind="file", path=sem_path))
     return mesh, control

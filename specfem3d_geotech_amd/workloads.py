@@ -62,14 +62,16 @@ def form_a(n: int, nmat: int, seed: int = 3, c_mean: float = 10.0, c_cov: float 
     0.2), spatially uniform per trial (every material of the mesh takes the
     trial's c and phi).  Returns [(coh[nmat], phi_deg[nmat])] * n."""
     import numpy as np
-    rng = np.random.default_rng(seed)
+    # one row of two standard normals per trial: trial i is the same for
+    # every n
+    z = np.random.default_rng(seed).standard_normal((n, 2))
 
-    def lognormal(mean, cov, size):
+    def lognormal(mean, cov, zz):
         s2 = np.log1p(cov * cov)
-        return rng.lognormal(np.log(mean) - 0.5 * s2, np.sqrt(s2), size)
+        return np.exp(np.log(mean) - 0.5 * s2 + np.sqrt(s2) * zz)
 
-    c = lognormal(c_mean, c_cov, n)
-    t = lognormal(np.tan(np.radians(phi_mean_deg)), tan_cov, n)
+    c = lognormal(c_mean, c_cov, z[:, 0])
+    t = lognormal(np.tan(np.radians(phi_mean_deg)), tan_cov, z[:, 1])
     phi = np.degrees(np.arctan(t))
     return [(np.full(nmat, c[i]), np.full(nmat, phi[i])) for i in range(n)]
 

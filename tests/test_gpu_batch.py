@@ -140,3 +140,43 @@ def test_refined_solve_matches_fp64(ctx):
           f"fp64 cg_tol 1e-8: {e_fp64:.3e}; tight solve its {it_t}")
     assert ec_t == 0
     assert e_mixed <= 2.0 * max(1e-8, e_fp64)
+
+
+def test_batched_realisations_match_oracle_brackets(ctx):
+    """Form-A strength realisations through the continuous-batching scheduler
+    (more analyses than columns, so columns are reused) against the oracle's
+    FP64 sweep of each realisation: identical FOS brackets; plastic iteration
+    counts of converged SRFs equal or one apart (uerr <= nl_tol decided on
+    solutions that agree to the PCG tolerance)."""
+    from specfem3d_geotech_amd import workloads
+    z = golden("slope_8x4x6_layered")
+    mesh = mesh_from_golden(z)
+    spec = hex2spec(mesh)
+    ctl = control_from_golden(z)
+    ctl["nl_maxiter"] = 60
+    gam, ym, nu, phi, coh, psi = mesh.material_arrays()
+    real = workloads.form_a(6, len(gam))
+    an = [Analysis(coh=c, phi=p, psi=psi.copy(), srf=list(ctl["srf"]), tag=i)
+          for i, (c, p) in enumerate(real)]
+    dm, ext = device_mesh(ctx, mesh, spec)
+    try:
+        eng = BatchEngine(dm, ext, 4)
+        try:
+            run_analyses(eng, an, ym=ym, nu=nu, cg_tol=ctl["cg_tol"],
+                         cg_maxiter=ctl["cg_maxiter"], nl_tol=ctl["nl_tol"],
+                         nl_maxiter=ctl["nl_maxiter"])
+        finally:
+            eng.close()
+    finally:
+        dm.close()
+    for a, (c, p) in zip(an, real):
+        o = O.slope3d(spec, mesh.with_strengths(c, p), **ctl)
+        rows = o["rows"]
+        br = None
+        if rows[-1]["nl_tot"] == ctl["nl_maxiter"]:
+            br = (rows[-2]["srf"] if len(rows) > 1 else None, rows[-1]["srf"])
+        print(f"c={c[0]:.3f} phi={p[0]:.3f}: batched {a.bracket}, oracle {br}")
+        assert a.bracket == br
+        for k, r in enumerate(rows):
+            if r["nl_tot"] < ctl["nl_maxiter"]:
+                assert abs(a.rows[k]["nl_tot"] - r["nl_tot"]) <= 1
