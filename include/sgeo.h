@@ -131,6 +131,24 @@ int sgeo_mesh_download_stiffness(sgeo_mesh *mesh, double *storekm);
 int sgeo_mesh_matvec(sgeo_mesh *mesh, const double *p, double *kp);
 /* HBM bytes held by the mesh. */
 int sgeo_mesh_bytes(sgeo_mesh *mesh, size_t *bytes);
+/* pcg_solver (src/solver.f90:14-79) on a resident mesh: its selected
+ * operator and inverted diagonal; u (host, neq+1) warm start, overwritten;
+ * f (host, neq+1).  cg_iter and errcode as sgeo_pcg_solver. */
+int sgeo_mesh_solve(sgeo_mesh *mesh, double *u, const double *f, double cg_tol,
+                    int32_t cg_maxiter, int32_t *cg_iter, int32_t *errcode);
+/* Elastic stress at every GLL point of the mesh's elements from the
+ * displacement x (host, neq+1): replaces elastic_stress
+ * (src/postprocess.f90:40-79) with E = ym[m], nu = nu[m]; stress (host)
+ * in the reference layout (6,27,nelmt).  The excavation's in-situ stress. */
+int sgeo_mesh_elastic_stress(sgeo_mesh *mesh, const double *ym,
+                             const double *nu, const double *x, double *stress);
+/* Excavation load of the mesh's elements (the elements a stage removes,
+ * numbered with the stage's equations): replaces excavation_load_nodal and
+ * the nodal-to-equation copy (src/excavation_library.f90:186-236,
+ * src/serial_library.f90:110-130).  stress: host (6,27,nelmt) of these
+ * elements; load: host (neq+1), load[0] = 0. */
+int sgeo_mesh_excavation_load(sgeo_mesh *mesh, const double *gam,
+                              const double *stress, double *load);
 
 /* ---- Seam 2: slope plastic iteration on a resident mesh -----------------
  * State of one strength-reduction trial (src/slope.F90:244-443): x, oldx,
@@ -144,6 +162,18 @@ int sgeo_slope_destroy(sgeo_slope *slope);
 int sgeo_slope_begin_srf(sgeo_slope *slope, const double *ym,
                          const double *nuf, const double *phif,
                          const double *cohf, const double *psif, double dt);
+/* Load increment of an excavation stage (src/excavation.F90:509-512):
+ * bodyload, evpt, x and oldx reset; the stress state is kept, or replaced
+ * by stress0 (host, (6,27,nelmt) reference layout) when given -- the
+ * in-situ stress of the stage's intact elements.  Later plastic iterations
+ * also record the von Mises plastic strain of every point at the final
+ * pass (src/excavation.F90:601-602, sgeo_slope_download_vmeps). */
+int sgeo_slope_begin_increment(sgeo_slope *slope, const double *ym,
+                               const double *nuf, const double *phif,
+                               const double *cohf, const double *psif,
+                               double dt, const double *stress0);
+/* sqrt(2/3 evpt.evpt) per GLL point at the last final pass, (27,nelmt). */
+int sgeo_slope_download_vmeps(sgeo_slope *slope, double *vm_pt);
 /* One plastic iteration (src/slope.F90:286-391): load = extload+bodyload,
  * PCG (warm start), x(0)=0, uerr, nl_isconv, then the element x GLL body.
  * is_last = (nl_iter == nl_maxiter).  Outputs: cg_iter, errcode of the

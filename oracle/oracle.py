@@ -55,6 +55,11 @@ def lib():
                                                   i32, i32, d, d, d, d, i32,
                                                   C.c_int32, i32])
         _LIB.orc_slope3d.restype = C.c_int
+        _LIB.orc_elastic_stress.argtypes = [C.c_int32, i32, d, i32, i32, d, d, d, d]
+        _LIB.orc_excavation_load_nodal.argtypes = [C.c_int32, i32, d, i32, i32, d, d,
+                                                   C.c_int32, d]
+        _LIB.orc_dt_viscoplas_mask.argtypes = [C.c_int, d, d, d, u8]
+        _LIB.orc_dt_viscoplas_mask.restype = C.c_double
     return _LIB
 
 
@@ -198,3 +203,153 @@ def slope3d(spec, mesh, *, cg_tol, cg_maxiter, nl_tol, nl_maxiter, srf,
                  uxmax=float(ux[s]), umax=float(um[s]), fmax=float(fm[s]))
             for s in range(rc)]
     return dict(rows=rows, x=x, cg_log=log[:min(nlog.value, cg_log_cap)].copy())
+
+
+# ----------------------------------------------------------------------------
+# excavation (src/excavation.F90:180-232 initial stress, :333-700 stages)
+# ----------------------------------------------------------------------------
+
+def elastic_stress(spec, ym, nu, x, elmt=None):
+    """stress (E,27,6) = C B u_e at every GLL point (src/postprocess.f90:40-79)."""
+    st = np.zeros((spec.nelmt, 27, 6))
+    el = _i32(elmt) if elmt is not None else None
+    lib().orc_elastic_stress(len(el) if el is not None else spec.nelmt,
+                             _p(el, C.c_int32), _p(_f64(spec.corners)),
+                             _p(_i32(spec.gdof_elmt), C.c_int32),
+                             _p(_i32(spec.mat_id), C.c_int32), _p(_f64(ym)),
+                             _p(_f64(nu)), _p(_f64(x)), _p(st))
+    return st
+
+
+def excavation_load_nodal(spec, gam, stress, elmt_void):
+    """excavload (nnode, 3) of the void elements (src/excavation_library.f90:
+    186-236)."""
+    out = np.zeros((spec.nnode, 3))
+    ev = _i32(elmt_void)
+    lib().orc_excavation_load_nodal(len(ev), _p(ev, C.c_int32),
+                                    _p(_f64(spec.corners)),
+                                    _p(_i32(spec.g_num), C.c_int32),
+                                    _p(_i32(spec.mat_id), C.c_int32), _p(_f64(gam)),
+                                    _p(_f64(stress)), spec.nnode, _p(out))
+    return out
+
+
+def dt_viscoplas_mask(nuf, phif, ym, ismat):
+    m = np.ascontiguousarray(ismat, dtype=np.uint8)
+    nuf, phif, ym = _f64(nuf), _f64(phif), _f64(ym)
+    return lib().orc_dt_viscoplas_mask(len(nuf), _p(nuf), _p(phif), _p(ym),
+                                       _p(m, C.c_uint8))
+
+
+def _stiffness_subset(corners, gdof, mat_id, neq, gam, nu, ym):
+    E = len(mat_id)
+    km = np.zeros((E, 81, 81)); dp = np.zeros(neq + 1)
+    lib().orc_stiffness_bodyload(E, neq, _p(_f64(corners)), _p(_i32(gdof), C.c_int32),
+                                 _p(_i32(mat_id), C.c_int32), _p(_f64(gam)),
+                                 _p(_f64(nu)), _p(_f64(ym)), _p(km), _p(dp), None, None)
+    return km, dp
+
+
+def renumber_gdof(gdof_node, node_alive):
+    """modify_gdof (src/excavation_library.f90:90-111): dead nodes fixed,
+    then every free dof renumbered node-major, component fastest."""
+    g = np.where(node_alive[:, None], gdof_node, 0)
+    free = g.reshape(-1) != 0
+    out = np.zeros(g.size, np.int64)
+    out[free] = np.arange(1, int(free.sum()) + 1)
+    return out.reshape(g.shape), int(free.sum())
+
+
+def excavation3d(spec, mesh, *, cg_tol, cg_maxiter, nl_tol, nl_maxiter, srf,
+                 nexcav, nexcavid, excavid, ninc, phinu=False, s0_type=0,
+                 on_pcg=None):
+    """The reference's staged excavation (src/excavation.F90:13-700), one SRF
+    (the reference supports only nsrf = 1 there).  Returns dict(rows=[(stage,
+    cg_tot, nl_tot, uxmax, umax, fmax)], stress0, nodalu, cg_log)."""
+    if s0_type != 0:
+        raise NotImplementedError("s0_type 1 (overburden) not restated")
+    gam, ym, nu, phi, coh, psi = mesh.material_arrays()
+    nmat, E = len(gam), spec.nelmt
+    g_num = np.asarray(spec.g_num)
+    mat = np.asarray(spec.mat_id)
+    gdof_node = np.asarray(spec.gdof).copy()
+    cg_log = []
+    # initial stress: elastic solve under gravity on the whole mesh
+    km, dp, ext = stiffness_bodyload(spec, gam, nu, ym)
+    dinv = np.zeros_like(dp); dinv[1:] = 1.0 / dp[1:]
+    x, it, ec = pcg_solver(spec.neq, km, np.zeros(spec.neq + 1), ext, dinv,
+                           spec.gdof_elmt, cg_tol, cg_maxiter)
+    if ec != 0:
+        raise RuntimeError("ERROR: PCG solver doesn't converge!")
+    cg_log.append(it)
+    x[0] = 0.0
+    stress = elastic_stress(spec, ym, nu, x)
+    stress0 = stress.copy()
+    del km
+    s = float(srf[0])
+    nodalu = np.zeros((spec.nnode, 3))
+    ismat = np.ones(nmat, bool)
+    evpt = np.zeros((E, 27, 6))
+    rows = []
+    for i_ex in range(nexcav):
+        id0 = i_ex * nexcavid[i_ex]
+        ids = np.asarray(excavid[id0:id0 + nexcavid[i_ex]])
+        ismat[ids - 1] = False
+        off = np.zeros(nmat, bool); off[ids - 1] = True
+        void = np.nonzero(off[mat - 1])[0].astype(np.int32)
+        intact = np.nonzero(~off[mat - 1] & ismat[mat - 1])[0].astype(np.int32)
+        if intact.size == 0:
+            break
+        alive = np.zeros(spec.nnode, bool); alive[g_num[intact].reshape(-1)] = True
+        gdof_node, neq = renumber_gdof(gdof_node, alive)
+        gdof_elmt = gdof_node[g_num].reshape(E, 81).astype(np.int32)
+        exl = excavation_load_nodal(spec, gam, stress, void)
+        extload = np.zeros(neq + 1)
+        extload[gdof_node.reshape(-1)] = exl.reshape(-1)   # distribute2ghosts
+        extload[0] = 0.0
+        cohf, nuf, phif, psif, _ = strength_reduction(s, phinu, coh, nu, phi, psi)
+        dt = dt_viscoplas_mask(nuf, phif, ym, ismat)
+        ge = gdof_elmt[intact]
+        km, dp = _stiffness_subset(spec.corners[intact], ge, mat[intact], neq, gam,
+                                   nuf, ym)
+        dinv = np.zeros_like(dp); dinv[1:] = 1.0 / dp[1:]
+        extload = extload / ninc
+        cg_tot = nl_tot = 0
+        fmax = 0.0
+        nl_iter = 0
+        for _ in range(ninc):
+            bodyload = np.zeros(neq + 1)
+            evpt[:] = 0.0
+            x = np.zeros(neq + 1); oldx = np.zeros(neq + 1)
+            for nl_iter in range(1, nl_maxiter + 1):
+                load = extload + bodyload
+                load[0] = 0.0
+                x, it, ec = pcg_solver(neq, km, x, load, dinv, ge, cg_tol, cg_maxiter)
+                if ec != 0:
+                    raise RuntimeError("ERROR: PCG solver doesn't converge!")
+                cg_log.append(it)
+                if on_pcg:
+                    on_pcg(i_ex, it)
+                cg_tot += it
+                x[0] = 0.0
+                uerr = np.abs(x - oldx).max() / np.abs(x).max()
+                oldx = x.copy()
+                conv = uerr <= nl_tol
+                last = conv or nl_iter == nl_maxiter
+                fmax = lib().orc_plastic_body_excav(
+                    neq, len(intact), _p(intact, C.c_int32), _p(_f64(spec.corners)),
+                    _p(ge, C.c_int32), _p(_i32(mat), C.c_int32), _p(_f64(ym)),
+                    _p(_f64(nuf)), _p(_f64(phif)), _p(_f64(cohf)), _p(_f64(psif)), dt,
+                    _p(x), _p(evpt), _p(stress), _p(bodyload), int(last), None, None)
+                if last:
+                    break
+            nl_tot += nl_iter
+            live = gdof_node != 0
+            nodalu[live] += x[gdof_node[live]]
+        un = nodalu[alive]
+        rows.append((i_ex + 1, cg_tot, nl_tot, float(np.abs(un).max()),
+                     float(np.sqrt((un * un).sum(axis=1)).max()), float(fmax)))
+        if nl_iter == nl_maxiter:
+            break
+    return dict(rows=rows, stress0=stress0, nodalu=nodalu, cg_log=cg_log,
+                stress=stress)

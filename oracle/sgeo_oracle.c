@@ -836,3 +836,86 @@ done:
   free(phif); free(psif);
   return rc < 0 ? rc : nrun;
 }
+
+/* ------------------------------------------------------------------------ */
+/* excavation helpers (src/postprocess.f90:40-79;                           */
+/*                     src/excavation_library.f90:186-236)                  */
+/* ------------------------------------------------------------------------ */
+
+void orc_elastic_stress(int32_t nel, const int32_t *elmt, const double *xc,
+                        const int32_t *gdof, const int32_t *mat_id,
+                        const double *ym, const double *nu, const double *x,
+                        double *stress) {
+  const tables_t *T = tables();
+#pragma omp parallel for schedule(static)
+  for (int32_t a = 0; a < nel; a++) {
+    const int32_t e = elmt ? elmt[a] : a;
+    const int m = mat_id[e] - 1;
+    double cmat[36], deriv[3 * NG], b[NST * NED], eld[NED];
+    orc_compute_cmat(ym[m], nu[m], cmat);
+    const int32_t *g = gdof + (size_t)NED * e;
+    for (int j = 0; j < NED; j++) eld[j] = x[g[j]];
+    for (int i = 0; i < NG; i++) {
+      gll_point_deriv(T, xc + 24 * (size_t)e, i, deriv);
+      compute_bmat(deriv, b);
+      double eps[NST];
+      for (int s = 0; s < NST; s++) { /* eps = matmul(bmat, eld) */
+        double acc = 0.0;
+        for (int j = 0; j < NED; j++) acc = acc + b[s + 6 * j] * eld[j];
+        eps[s] = acc;
+      }
+      double *st = stress + 6 * (i + NG * (size_t)e);
+      for (int s = 0; s < NST; s++) { /* sigma = matmul(cmat, eps) */
+        double acc = 0.0;
+        for (int t = 0; t < NST; t++) acc = acc + cmat[s + 6 * t] * eps[t];
+        st[s] = acc;
+      }
+    }
+  }
+}
+
+void orc_excavation_load_nodal(int32_t nvoid, const int32_t *elmt_void,
+                               const double *xc, const int32_t *g_num,
+                               const int32_t *mat_id, const double *gam,
+                               const double *stress, int32_t nnode,
+                               double *excavload) {
+  const tables_t *T = tables();
+  memset(excavload, 0, sizeof(double) * 3 * (size_t)nnode);
+  for (int32_t a = 0; a < nvoid; a++) {
+    const int32_t e = elmt_void[a];
+    double deriv[3 * NG], b[NST * NED], bload[NED], eld[NED], tload[NED];
+    for (int j = 0; j < NED; j++) { bload[j] = 0.0; eld[j] = 0.0; }
+    for (int i = 0; i < NG; i++) {
+      double detjac = gll_point_deriv(T, xc + 24 * (size_t)e, i, deriv);
+      compute_bmat(deriv, b);
+      const double *sg = stress + 6 * (i + NG * (size_t)e);
+      for (int j = 0; j < NED; j++) { /* eload = matmul(sigma, bmat) */
+        double acc = 0.0;
+        for (int s = 0; s < NST; s++) acc = acc + sg[s] * b[s + 6 * j];
+        bload[j] = bload[j] + acc * detjac * T->w[i];
+      }
+      eld[3 * i + 2] = eld[3 * i + 2] + detjac * T->w[i];
+    }
+    const double gm = gam[mat_id[e] - 1];
+    for (int j = 0; j < NED; j++) tload[j] = eld[j] * gm + bload[j];
+    const int32_t *num = g_num + (size_t)NG * e;
+    for (int c = 0; c < 3; c++)
+      for (int n = 0; n < NG; n++)
+        excavload[3 * (size_t)num[n] + c] = excavload[3 * (size_t)num[n] + c] +
+                                            tload[3 * n + c];
+  }
+}
+
+double orc_dt_viscoplas_mask(int nmat, const double *nuf, const double *phif,
+                             const double *ymf, const uint8_t *ismat) {
+  /* src/plastic_library.f90:17-39 with the optional ismat mask */
+  double dt_min = 1.0e32;
+  for (int m = 0; m < nmat; m++) {
+    if (ismat && !ismat[m]) continue;
+    double snphi = sin(phif[m] * DEG2RAD);
+    double dt = 4.0 * (1.0 + nuf[m]) * (1.0 - 2.0 * nuf[m]) /
+                (ymf[m] * (1.0 - 2.0 * nuf[m] + snphi * snphi));
+    if (dt < dt_min) dt_min = dt;
+  }
+  return dt_min;
+}

@@ -106,6 +106,28 @@ double orc_plastic_body_excav(int32_t neq, int32_t nel_act,
                               double *bodyload, int conv_or_last,
                               const int32_t *g_num, double *vmeps);
 
+/* elastic_stress (src/postprocess.f90:40-79): stress(:,i,e) = C B u_e at
+ * every GLL point of elements elmt[0..nel-1] (NULL: 0..nel-1), written at
+ * the element's place in the full stress(6,27,nelmt) array. */
+void orc_elastic_stress(int32_t nel, const int32_t *elmt, const double *xc,
+                        const int32_t *gdof, const int32_t *mat_id,
+                        const double *ym, const double *nu, const double *x,
+                        double *stress);
+
+/* excavation_load_nodal (src/excavation_library.f90:186-236): per void
+ * element, tload = eld*gam + sum_i (sigma_i B_i) detJ_i w_i, accumulated
+ * into excavload[3*node + c] in element order (g_num 0-based, 27 per
+ * element; stress indexed by element id in the full array). */
+void orc_excavation_load_nodal(int32_t nvoid, const int32_t *elmt_void,
+                               const double *xc, const int32_t *g_num,
+                               const int32_t *mat_id, const double *gam,
+                               const double *stress, int32_t nnode,
+                               double *excavload);
+
+/* dt_viscoplas with the optional ismat mask (src/plastic_library.f90:17-39) */
+double orc_dt_viscoplas_mask(int nmat, const double *nuf, const double *phif,
+                             const double *ymf, const uint8_t *ismat);
+
 /* Full strength-reduction sweep of slope3d (src/slope.F90:244-443) on a
  * prepared mesh.  Per SRF s writes sum_cg[s], sum_nl[s], uxmax[s], umax[s],
  * fmax[s]; returns the number of SRFs run (the loop stops after the first

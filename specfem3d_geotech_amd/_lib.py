@@ -48,6 +48,12 @@ SIGNATURES = {
     "sgeo_slope_begin_srf": (C.c_int, [vp, d_p, d_p, d_p, d_p, d_p, C.c_double]),
     "sgeo_slope_plastic_iter": (C.c_int, [vp, C.c_double, C.c_int32, C.c_double,
                                           C.c_int, i_p, i_p, d_p, i_p, d_p]),
+    "sgeo_slope_begin_increment": (C.c_int, [vp, d_p, d_p, d_p, d_p, d_p, C.c_double,
+                                             d_p]),
+    "sgeo_slope_download_vmeps": (C.c_int, [vp, d_p]),
+    "sgeo_mesh_elastic_stress": (C.c_int, [vp, d_p, d_p, d_p, d_p]),
+    "sgeo_mesh_solve": (C.c_int, [vp, d_p, d_p, C.c_double, C.c_int32, i_p, i_p]),
+    "sgeo_mesh_excavation_load": (C.c_int, [vp, d_p, d_p, d_p]),
     "sgeo_slope_run_srf": (C.c_int, [vp, i_p, C.c_int32, C.c_double, C.c_int32,
                                      C.c_double, C.c_int32, i_p, i_p, d_p, d_p,
                                      d_p, i_p]),

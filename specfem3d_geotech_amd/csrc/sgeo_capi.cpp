@@ -123,10 +123,12 @@ struct sgeo_slope {
   double *d_x = nullptr, *d_oldx = nullptr, *d_load = nullptr;
   double *d_body = nullptr, *d_ext = nullptr;
   double *d_evpt = nullptr, *d_stress = nullptr, *d_BL = nullptr;
+  double* d_vm = nullptr;  // excavation: per-point von Mises plastic strain
   uint8_t* d_flags = nullptr;
   MatParams* d_mp = nullptr;
   std::vector<MatParams> h_mp;
   bool begun = false;
+  int excav = 0;
 };
 
 template <typename T>
@@ -680,8 +682,9 @@ extern "C" int sgeo_slope_destroy(sgeo_slope* t) {
   if (!t) return 0;
   hipSetDevice(t->mesh->ctx->device);
   hipStreamSynchronize(t->mesh->ctx->stream);
-  void* ptrs[] = {t->d_x, t->d_oldx, t->d_load, t->d_body, t->d_ext,
-                  t->d_evpt, t->d_stress, t->d_BL, t->d_flags, t->d_mp};
+  void* ptrs[] = {t->d_x,      t->d_oldx, t->d_load,  t->d_body,
+                  t->d_ext,    t->d_evpt, t->d_stress, t->d_BL,
+                  t->d_flags,  t->d_mp,   t->d_vm};
   for (void* p : ptrs)
     if (p) hipFree(p);
   delete t;
@@ -713,6 +716,141 @@ extern "C" int sgeo_slope_begin_srf(sgeo_slope* t, const double* ym,
   HIPCHK(hipMemsetAsync(t->d_oldx, 0, n1 * 8, s));
   HIPCHK(hipMemsetAsync(t->d_flags, 0, (size_t)m->nelmt * 27, s));
   t->begun = true;
+  return 0;
+}
+
+// Load increment of an excavation stage (src/excavation.F90:509-512):
+// bodyload=0, evpt=0, x=0, oldx=0, the stress state kept (stress0 != NULL
+// replaces it, reference layout (6,27,nelmt)); the body then also records
+// the von Mises plastic strain at the final pass (src/excavation.F90:601-602).
+extern "C" int sgeo_slope_begin_increment(sgeo_slope* t, const double* ym,
+                                          const double* nuf, const double* phif,
+                                          const double* cohf, const double* psif,
+                                          double dt, const double* stress0) {
+  if (!t || !ym || !nuf || !phif || !cohf || !psif)
+    return fail(SGEO_EARG, "sgeo_slope_begin_increment: null argument");
+  sgeo_mesh* m = t->mesh;
+  sgeo_ctx* c = m->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  t->h_mp.resize((size_t)m->nmat);
+  for (int i = 0; i < m->nmat; i++)
+    t->h_mp[i] = MatParams{ym[i], nuf[i], phif[i], cohf[i], psif[i], dt};
+  const size_t n1 = (size_t)m->neq + 1;
+  const size_t ns = (size_t)m->nelmt * 27 * 6;
+  if (!t->d_vm) TRY(dalloc(&t->d_vm, (size_t)m->nelmt * 27));
+  HIPCHK(hipMemcpyAsync(t->d_mp, t->h_mp.data(), m->nmat * sizeof(MatParams),
+                        hipMemcpyHostToDevice, s));
+  if (stress0)
+    HIPCHK(hipMemcpyAsync(t->d_stress, stress0, ns * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemsetAsync(t->d_evpt, 0, ns * 8, s));
+  HIPCHK(hipMemsetAsync(t->d_body, 0, n1 * 8, s));
+  HIPCHK(hipMemsetAsync(t->d_x, 0, n1 * 8, s));
+  HIPCHK(hipMemsetAsync(t->d_oldx, 0, n1 * 8, s));
+  HIPCHK(hipMemsetAsync(t->d_flags, 0, (size_t)m->nelmt * 27, s));
+  HIPCHK(hipMemsetAsync(t->d_vm, 0, (size_t)m->nelmt * 27 * 8, s));
+  HIPCHK(hipStreamSynchronize(s));
+  t->excav = 1;
+  t->begun = true;
+  return 0;
+}
+
+extern "C" int sgeo_slope_download_vmeps(sgeo_slope* t, double* vm_pt) {
+  if (!t || !vm_pt || !t->d_vm) return fail(SGEO_ESTATE, "no excavation state");
+  sgeo_mesh* m = t->mesh;
+  HIPCHK(hipSetDevice(m->ctx->device));
+  HIPCHK(hipMemcpyAsync(vm_pt, t->d_vm, (size_t)m->nelmt * 27 * 8,
+                        hipMemcpyDeviceToHost, m->ctx->stream));
+  HIPCHK(hipStreamSynchronize(m->ctx->stream));
+  return 0;
+}
+
+// pcg_solver on a resident mesh (its operator, its inverted diagonal):
+// u (host, neq+1, warm start, overwritten), f (host, neq+1).
+extern "C" int sgeo_mesh_solve(sgeo_mesh* m, double* u, const double* f,
+                               double cg_tol, int32_t cg_maxiter, int32_t* cg_iter,
+                               int32_t* errcode) {
+  if (!m || !u || !f || !cg_iter || !errcode || m->kprec < 0)
+    return fail(SGEO_EARG, "sgeo_mesh_solve: bad arguments");
+  sgeo_ctx* c = m->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const size_t n1 = (size_t)m->neq + 1;
+  double *d_u = nullptr, *d_f = nullptr;
+  TRY(dalloc(&d_u, n1));
+  TRY(dalloc(&d_f, n1));
+  HIPCHK(hipMemcpyAsync(d_u, u, n1 * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d_f, f, n1 * 8, hipMemcpyHostToDevice, s));
+  const int rc = pcg_device(m, d_u, d_f, cg_tol, cg_maxiter, cg_iter, errcode);
+  if (rc == 0) {
+    HIPCHK(hipMemcpyAsync(u, d_u, n1 * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  hipFree(d_u);
+  hipFree(d_f);
+  return rc;
+}
+
+// Elastic stress at every GLL point from a displacement x (host, neq+1):
+// src/postprocess.f90:40-79 with E = ym[m], nu = nu[m].  stress: host,
+// (6,27,nelmt) reference layout.
+extern "C" int sgeo_mesh_elastic_stress(sgeo_mesh* m, const double* ym,
+                                        const double* nu, const double* x,
+                                        double* stress) {
+  if (!m || !ym || !nu || !x || !stress || !m->d_corners || !m->d_mat)
+    return fail(SGEO_EARG, "sgeo_mesh_elastic_stress: bad arguments");
+  sgeo_ctx* c = m->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const size_t n1 = (size_t)m->neq + 1, ns = (size_t)m->nelmt * 27 * 6;
+  const int nmat = std::max(1, m->nmat);
+  double *d_x = nullptr, *d_st = nullptr, *d_mat = nullptr;
+  TRY(dalloc(&d_x, n1));
+  TRY(dalloc(&d_st, ns));
+  TRY(dalloc(&d_mat, 2 * (size_t)nmat));
+  HIPCHK(hipMemcpyAsync(d_x, x, n1 * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d_mat, ym, nmat * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d_mat + nmat, nu, nmat * 8, hipMemcpyHostToDevice, s));
+  launch_elastic_stress(s, m->nelmt, m->d_corners, m->d_gdof, m->d_mat, d_mat,
+                        d_mat + nmat, d_x, d_st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(stress, d_st, ns * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  hipFree(d_x);
+  hipFree(d_st);
+  hipFree(d_mat);
+  return 0;
+}
+
+// Excavation load of the mesh's elements -- the removed ones of a stage,
+// numbered with the stage's equations (src/excavation_library.f90:186-236
+// and distribute2ghosts, src/serial_library.f90:110-130): load (host,
+// neq+1) = per equation the sum of its element contributions in element
+// order, load[0] = 0.  stress: host, (6,27,nelmt) of these elements.
+extern "C" int sgeo_mesh_excavation_load(sgeo_mesh* m, const double* gam,
+                                         const double* stress, double* load) {
+  if (!m || !gam || !stress || !load || !m->d_corners || !m->d_mat)
+    return fail(SGEO_EARG, "sgeo_mesh_excavation_load: bad arguments");
+  sgeo_ctx* c = m->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const int64_t n1 = (int64_t)m->neq + 1;
+  const size_t ns = (size_t)m->nelmt * 27 * 6;
+  const int nmat = std::max(1, m->nmat);
+  double *d_st = nullptr, *d_gam = nullptr, *d_out = nullptr;
+  TRY(dalloc(&d_st, ns));
+  TRY(dalloc(&d_gam, (size_t)nmat));
+  TRY(dalloc(&d_out, (size_t)n1));
+  HIPCHK(hipMemcpyAsync(d_st, stress, ns * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d_gam, gam, nmat * 8, hipMemcpyHostToDevice, s));
+  launch_excav_load(s, m->nelmt, m->d_corners, m->d_mat, d_gam, d_st, m->d_Y);
+  launch_assemble(s, n1, m->d_rowptr, m->d_slots, m->d_Y, d_out, nullptr, 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(load, d_out, n1 * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  hipFree(d_st);
+  hipFree(d_gam);
+  hipFree(d_out);
   return 0;
 }
 
@@ -748,6 +886,8 @@ extern "C" int sgeo_slope_plastic_iter(sgeo_slope* t, double cg_tol,
   b.flags = t->d_flags;
   b.ps = c->d_ps;
   b.is_last = is_last;
+  b.excav = t->excav;
+  b.vm_pt = t->d_vm;
   launch_plastic_body(s, b);
   // bodyload(egdof) += bload unless converged/last (src/slope.F90:380-384);
   // the kernel itself returns when nl_isconv is set on the device

@@ -267,6 +267,87 @@ __global__ void k_scale_by_mat(int32_t nelmt, const double* __restrict__ eld,
 }
 
 // --------------------------------------------------------------------------
+// elastic stress at the GLL points (src/postprocess.f90:40-79): thread per
+// (element, point); sigma = matmul(cmat, matmul(bmat, u_e)), cmat(E, nu)
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_elastic_stress(
+    int32_t nelmt, const double* __restrict__ corners, const int32_t* __restrict__ gdof,
+    const int32_t* __restrict__ mat_id, const double* __restrict__ ym,
+    const double* __restrict__ nu, const double* __restrict__ x,
+    double* __restrict__ stress) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= (int64_t)nelmt * 27) return;
+  const int64_t e = q / 27;
+  const int i = (int)(q - e * 27);
+  double xc[24];
+  for (int k = 0; k < 24; k++) xc[k] = corners[e * 24 + k];
+  double jinv[3][3];
+  jacobian_point(i, xc, jinv);
+  const int m = mat_id[e] - 1;
+  const Cmat cm = cmat_of(ym[m], nu[m]);
+  double eps[6] = {0, 0, 0, 0, 0, 0};
+  for (int n = 0; n < 27; n++) {  // eps = matmul(bmat, eld), zero terms skipped
+    double d[3];
+    deriv_node(i, n, jinv, d);
+    const double ux = x[gdof[e * 81 + 3 * n]], uy = x[gdof[e * 81 + 3 * n + 1]],
+                 uz = x[gdof[e * 81 + 3 * n + 2]];
+    eps[0] = eps[0] + d[0] * ux;
+    eps[1] = eps[1] + d[1] * uy;
+    eps[2] = eps[2] + d[2] * uz;
+    eps[3] = eps[3] + d[1] * ux;
+    eps[3] = eps[3] + d[0] * uy;
+    eps[4] = eps[4] + d[2] * uy;
+    eps[4] = eps[4] + d[1] * uz;
+    eps[5] = eps[5] + d[2] * ux;
+    eps[5] = eps[5] + d[0] * uz;
+  }
+  double sig[6];
+  cmat_apply(cm, eps, sig);
+  for (int s = 0; s < 6; s++) stress[q * 6 + s] = sig[s];
+}
+
+// --------------------------------------------------------------------------
+// excavation load of the removed elements (src/excavation_library.f90:
+// 186-236): Y[e][j] = eld(j)*gam + sum_i eload_i(j) detJ_i w_i, eload_i =
+// matmul(sigma_i, bmat_i), eld(3i+3) = detJ_i w_i.  One block per element:
+// the 27 points' inverse Jacobians and stresses in LDS, thread j sums its
+// dof over the points in order.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(128) void k_excav_load(
+    int32_t nelmt, const double* __restrict__ corners,
+    const int32_t* __restrict__ mat_id, const double* __restrict__ gam,
+    const double* __restrict__ stress, double* __restrict__ Y) {
+  __shared__ double s_jinv[27][9];
+  __shared__ double s_det[27];
+  __shared__ double s_sig[27][6];
+  const int64_t e = blockIdx.x;
+  const int t = threadIdx.x;
+  if (t < 27) {
+    double xc[24];
+    for (int k = 0; k < 24; k++) xc[k] = corners[e * 24 + k];
+    double jinv[3][3];
+    s_det[t] = jacobian_point(t, xc, jinv);
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) s_jinv[t][a * 3 + b] = jinv[a][b];
+    for (int c = 0; c < 6; c++) s_sig[t][c] = stress[(e * 27 + t) * 6 + c];
+  }
+  __syncthreads();
+  if (t >= 81) return;
+  const int n = t / 3, comp = t % 3;
+  double bl = 0.0;
+  for (int i = 0; i < 27; i++) {
+    double jinv[3][3];
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) jinv[a][b] = s_jinv[i][a * 3 + b];
+    double d[3];
+    deriv_node(i, n, jinv, d);
+    bl = bl + eload_dof(comp, s_sig[i], d) * s_det[i] * gll_weight(i);
+  }
+  const double eld = (comp == 2) ? s_det[n] * gll_weight(n) : 0.0;
+  Y[e * 81 + t] = eld * gam[mat_id[e] - 1] + bl;
+}
+
+// --------------------------------------------------------------------------
 // launch wrappers
 // --------------------------------------------------------------------------
 static inline unsigned nblk(int64_t n, int per) {
@@ -304,6 +385,22 @@ void launch_scale_by_mat(hipStream_t s, int32_t nelmt, const double* eld,
                          const int32_t* mat_id, const double* gam, double* Y) {
   hipLaunchKernelGGL(k_scale_by_mat, dim3(grid_stride_blocks((int64_t)nelmt * 81)),
                      dim3(SGEO_VEC_THREADS), 0, s, nelmt, eld, mat_id, gam, Y);
+}
+
+void launch_elastic_stress(hipStream_t s, int32_t nelmt, const double* corners,
+                           const int32_t* gdof, const int32_t* mat_id, const double* ym,
+                           const double* nu, const double* x, double* stress) {
+  if (nelmt <= 0) return;
+  hipLaunchKernelGGL(k_elastic_stress, dim3(nblk((int64_t)nelmt * 27, 256)), dim3(256),
+                     0, s, nelmt, corners, gdof, mat_id, ym, nu, x, stress);
+}
+
+void launch_excav_load(hipStream_t s, int32_t nelmt, const double* corners,
+                       const int32_t* mat_id, const double* gam, const double* stress,
+                       double* Y) {
+  if (nelmt <= 0) return;
+  hipLaunchKernelGGL(k_excav_load, dim3(nelmt), dim3(128), 0, s, nelmt, corners,
+                     mat_id, gam, stress, Y);
 }
 
 void launch_plastic_body(hipStream_t s, const BodyArgs& b) {

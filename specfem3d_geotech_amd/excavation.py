@@ -1,0 +1,165 @@
+"""Host mirror of `excavation3d` (src/excavation.F90:13-700) over the device
+path.
+
+The reference's stage logic stays on the host in its order: intact and void
+element sets, intact nodes, the renumbered equations (integer work,
+src/excavation_library.f90:17-111), the nodal displacement sum and the
+stage summary row.  Everything numeric runs on the GPU through libsgeo.so:
+- the initial gravity solve and the elastic stress (s0_type 0);
+- each stage's excavation load, built from the removed elements' stress;
+- each stage's stiffness, built on the intact elements with the reduced nu;
+- every load increment's plastic loop, with its PCG and the element x GLL
+  body, starting from the in-situ stress.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from .mesh import HexMesh
+from .plastic import dt_viscoplas, strength_reduction
+from .slope import DeviceSlope, PCGNotConverged
+from .solver import K_F64, KFMT_FULL, Context, DeviceMesh
+from .spec import SpecMesh, hex2spec
+
+
+def intact_void(mat_id: np.ndarray, ismat: np.ndarray, removed: Sequence[int]):
+    """intact_void_elmt (src/excavation_library.f90:17-53): elements of the
+    materials removed now are void; elements of still-active materials are
+    intact (earlier removals are neither); both in element order."""
+    off = np.zeros(ismat.size, bool)
+    off[np.asarray(removed) - 1] = True
+    m = np.asarray(mat_id) - 1
+    void = np.nonzero(off[m])[0].astype(np.int32)
+    intact = np.nonzero(~off[m] & ismat[m])[0].astype(np.int32)
+    return intact, void
+
+
+def renumber(gdof_node: np.ndarray, alive: np.ndarray):
+    """modify_gdof (src/excavation_library.f90:90-111): dofs of dead nodes
+    fixed, the free ones renumbered node-major, component fastest."""
+    g = np.where(alive[:, None], gdof_node, 0)
+    free = g.reshape(-1) != 0
+    out = np.zeros(g.size, np.int64)
+    out[free] = np.arange(1, int(free.sum()) + 1)
+    return out.reshape(g.shape), int(free.sum())
+
+
+def sub_spec(spec: SpecMesh, elmt: np.ndarray, gdof_node: np.ndarray,
+             gdof_elmt: np.ndarray, neq: int) -> SpecMesh:
+    return SpecMesh(nelmt=int(elmt.size), nnode=spec.nnode, neq=neq,
+                    g_num=np.asarray(spec.g_num)[elmt], g_coord=spec.g_coord,
+                    gdof=gdof_node, gdof_elmt=np.ascontiguousarray(gdof_elmt[elmt]),
+                    corners=np.ascontiguousarray(np.asarray(spec.corners)[elmt]),
+                    mat_id=np.ascontiguousarray(np.asarray(spec.mat_id)[elmt]))
+
+
+@dataclass
+class ExcavationResult:
+    rows: List[Dict[str, float]]      # (stage, cg_tot, nl_tot, uxmax, umax, fmax)
+    cg_log: List[int]
+    nodalu: np.ndarray                # (nnode, 3)
+    stress: np.ndarray                # (nelmt, 27, 6) after the last stage
+    extra: Dict[str, object] = field(default_factory=dict)
+
+
+def excavation3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
+                 nl_tol: float, nl_maxiter: int, srf: Sequence[float], nexcav: int,
+                 nexcavid: Sequence[int], excavid: Sequence[int], ninc: int,
+                 phinu: bool = False, s0_type: int = 0, kprec: int = K_F64,
+                 kfmt: int = KFMT_FULL, spec: Optional[SpecMesh] = None,
+                 progress=None) -> ExcavationResult:
+    """Staged excavation on the GPU for one SRF.  The reference runs
+    excavation with nsrf = 1 only (it reallocates and closes per-SRF state
+    inside its SRF loop); the first SRF is used."""
+    if s0_type != 0:
+        raise NotImplementedError("initial stress type 1 (overburden) is not built")
+    spec = spec or hex2spec(mesh)
+    gam, ym, nu, phi, coh, psi = mesh.material_arrays()
+    nmat, E = len(gam), spec.nelmt
+    g_num = np.asarray(spec.g_num)
+    mat = np.asarray(spec.mat_id)
+    cg_log: List[int] = []
+    # initial stress from the gravity solve on the whole mesh
+    # (src/excavation.F90:180-232)
+    dm = DeviceMesh(ctx, spec, nmat)
+    try:
+        ext, _ = dm.build_stiffness(gam, ym, nu, kprec=K_F64)
+        dm.set_operator(kprec, kfmt)
+        x, it, ec = dm.solve(ext, None, cg_tol, cg_maxiter)
+        if ec != 0:
+            raise PCGNotConverged("ERROR: PCG solver doesn't converge!")
+        cg_log.append(it)
+        x[0] = 0.0
+        stress = dm.elastic_stress(ym, nu, x)
+    finally:
+        dm.close()
+    s = float(srf[0])
+    gdof_node = np.asarray(spec.gdof).copy()
+    nodalu = np.zeros((spec.nnode, 3))
+    ismat = np.ones(nmat, bool)
+    rows: List[Dict[str, float]] = []
+    for i_ex in range(nexcav):
+        id0 = i_ex * nexcavid[i_ex]  # src/excavation.F90:339
+        removed = list(excavid[id0:id0 + nexcavid[i_ex]])
+        ismat[np.asarray(removed) - 1] = False
+        intact, void = intact_void(mat, ismat, removed)
+        if intact.size == 0:
+            break
+        alive = np.zeros(spec.nnode, bool)
+        alive[g_num[intact].reshape(-1)] = True
+        gdof_node, neq = renumber(gdof_node, alive)
+        gdof_elmt = gdof_node[g_num].reshape(E, 81).astype(np.int32)
+        # excavation load of the removed elements (numbered for this stage)
+        extload = np.zeros(neq + 1)
+        if void.size:
+            dv = DeviceMesh(ctx, sub_spec(spec, void, gdof_node, gdof_elmt, neq), nmat)
+            try:
+                extload = dv.excavation_load(gam, stress[void])
+            finally:
+                dv.close()
+        cohf, nuf, phif, psif, _ = strength_reduction(s, phinu, coh, nu, phi, psi)
+        dt = dt_viscoplas(nuf, phif, ym, ismat)
+        di = DeviceMesh(ctx, sub_spec(spec, intact, gdof_node, gdof_elmt, neq), nmat)
+        trial = None
+        try:
+            di.build_stiffness(gam, ym, nuf, kprec=K_F64, want_extload=False)
+            di.set_operator(kprec, kfmt)
+            trial = DeviceSlope(di, extload / ninc)
+            cg_tot = nl_tot = 0
+            fmax = 0.0
+            nl_iter = 0
+            for inc in range(ninc):
+                trial.begin_increment(ym, nuf, phif, cohf, psif, dt,
+                                      stress[intact] if inc == 0 else None)
+                for nl_iter in range(1, nl_maxiter + 1):
+                    r = trial.plastic_iter(cg_tol, cg_maxiter, nl_tol,
+                                           nl_iter == nl_maxiter)
+                    cg_log.append(r["cg_iter"])
+                    if r["errcode"] != 0:
+                        raise PCGNotConverged("ERROR: PCG solver doesn't converge!")
+                    cg_tot += r["cg_iter"]
+                    fmax = r["fmax"]
+                    if r["nl_isconv"] or nl_iter == nl_maxiter:
+                        break
+                nl_tot += nl_iter
+                st = trial.download()
+                live = gdof_node != 0
+                nodalu[live] += st["x"][gdof_node[live]]
+            stress[intact] = st["stress"]
+        finally:
+            if trial is not None:
+                trial.close()
+            di.close()
+        un = nodalu[alive]
+        rows.append(dict(stage=i_ex + 1, cg_tot=cg_tot, nl_tot=nl_tot,
+                         uxmax=float(np.abs(un).max()),
+                         umax=float(np.sqrt((un * un).sum(axis=1)).max()),
+                         fmax=float(fmax)))
+        if progress:
+            progress(rows[-1])
+        if nl_iter == nl_maxiter:  # src/excavation.F90:694
+            break
+    return ExcavationResult(rows=rows, cg_log=cg_log, nodalu=nodalu, stress=stress)

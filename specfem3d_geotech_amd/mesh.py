@@ -75,6 +75,8 @@ def slope_surface(x: np.ndarray, *, z_crest: float, z_toe: float,
                   x_crest: float, angle_deg: float) -> np.ndarray:
     """Ground height of a planar slope: crest plateau, face at `angle_deg`,
     toe plateau."""
+    if z_crest == z_toe:  # flat ground (box meshes)
+        return np.full(np.shape(x), float(z_crest))
     run = (z_crest - z_toe) / math.tan(math.radians(angle_deg))
     t = np.clip((x - x_crest) / run, 0.0, 1.0)
     return z_crest - t * (z_crest - z_toe)
@@ -178,6 +180,29 @@ def box_mesh(nx: int, ny: int, nz: int, *, lx: float, ly: float, lz: float,
         m.mat_id = np.asarray(region_of(c[:, 0], c[:, 1], c[:, 2]),
                               dtype=np.int32)
     m.meta.update(kind="box", lz=lz)
+    return m
+
+
+def pit_mesh(nx: int, ny: int, nz: int, *, lx: float, ly: float, lz: float,
+             pit: Sequence[float], nstage: int, stage_depth: float,
+             soil: Material) -> HexMesh:
+    """Box with a rectangular pit excavated from the top in `nstage` layers
+    of `stage_depth`: elements whose centroid lies in the pit footprint
+    pit = (x0, x1, y0, y1) and in layer s (counted from the top) get
+    material id s + 2; all others id 1.  Every id carries the same soil, as
+    in the reference's excavation examples (input/excavation_3d*).  Stage s
+    removes id s + 2 (control keys nexcav = nstage, excavid = 2 .. nstage+1).
+    """
+    x0, x1, y0, y1 = pit
+
+    def region(xc, yc, zc):
+        inside = (xc > x0) & (xc < x1) & (yc > y0) & (yc < y1)
+        layer = np.floor((lz - zc) / stage_depth).astype(np.int64)
+        return np.where(inside & (layer < nstage), layer + 2, 1)
+
+    m = box_mesh(nx, ny, nz, lx=lx, ly=ly, lz=lz,
+                 materials=[soil] * (nstage + 1), region_of=region)
+    m.meta.update(kind="pit", nstage=nstage)
     return m
 
 

