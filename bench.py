@@ -21,6 +21,12 @@ Modes:
             FP64 drop-in PCG on the same mesh for the per-trial speedup.
   pcg       the plain FP64 drop-in: one step = one CG iteration of the
             elastic solve (fixed count); value = DOF-iterations/s.
+  serve     a Poisson stream (--rate, default 4/s) of --requests analyses on
+            seeded meshes of --min/--max-elements (log-uniform): 70% slope FOS
+            sweeps (angle 20-45 deg, 1-3 layers; mixed multi-trial engine,
+            8 SRFs in flight), 30% staged box-pit excavations (3-6 stages);
+            admission by HBM footprint, concurrent requests on their own
+            streams.  value = analyses/hour, with p50/p99 latency.
   analysis  complete FOS analyses: --form fp64 runs the reference's sweep
             with the FP64 drop-in, one analysis after another; --form batched
             runs --analyses N form-A analyses through the continuous-batching
@@ -57,7 +63,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--mode", default="sweep", choices=["sweep", "pcg", "analysis"])
+    ap.add_argument("--mode", default="sweep",
+                    choices=["sweep", "pcg", "analysis", "serve"])
     ap.add_argument("--form", default=None,
                     choices=list(FORMS) + ["batched"])
     ap.add_argument("--workload", default="c1", choices=["c1", "c0"])
@@ -65,6 +72,10 @@ def parse():
     ap.add_argument("--analyses", type=int, default=None)
     ap.add_argument("--rate", type=float, default=None,
                     help="analysis mode: Poisson arrival rate (1/s) of requests")
+    ap.add_argument("--requests", type=int, default=32)
+    ap.add_argument("--min-elements", type=int, default=500)
+    ap.add_argument("--max-elements", type=int, default=5000)
+    ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-iters", type=int, default=30)
     return ap.parse_args()
@@ -360,6 +371,29 @@ def mode_analysis(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
     return out
 
 
+def mode_serve(a):
+    from specfem3d_geotech_amd.serving import Server, make_requests
+    reqs = make_requests(a.requests, rate=a.rate or 4.0,
+                         nelmt_range=(a.min_elements, a.max_elements), seed=4)
+    srv = Server(max_workers=a.workers, slope_columns=8)
+
+    def done(r):
+        log(f"  request {r.tag} {r.kind} {r.nelmt} el: latency {r.latency:.2f} s")
+    st = srv.serve(reqs, on_done=done)
+    kinds = [r.kind for r in reqs]
+    return dict(
+        metric=METRIC, value=st["analyses_per_hour"], unit="FOS analyses/hour",
+        n_gpus=1, steps=len(reqs), warmup=0, ms_per_step=st["wall_s"] / len(reqs) * 1e3,
+        higher_is_better=True, scaling="weak", vs_baseline=None,
+        dtype="f32 inner CG (MFMA) + f64 (slopes); f64 (excavations)", data=DATA,
+        config=dict(workload=f"serving: Poisson {a.rate or 4.0}/s, {len(reqs)} requests, "
+                    f"meshes {a.min_elements}-{a.max_elements} elements log-uniform, "
+                    f"{kinds.count('slope')} slope sweeps + {kinds.count('excavation')} "
+                    "staged excavations", workers=a.workers, mode="serve"),
+        latency_s=dict(p50=st["p50_s"], p99=st["p99_s"], max=st["max_s"]),
+        serving=st)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -370,6 +404,9 @@ def main():
     from specfem3d_geotech_amd import workloads
     from specfem3d_geotech_amd.solver import Context
 
+    if a.mode == "serve":
+        print(json.dumps(mode_serve(a)), flush=True)
+        return
     wl = workloads.get(a.workload)
     t0 = time.time()
     mesh, spec = wl.build()

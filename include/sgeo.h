@@ -250,6 +250,11 @@ int sgeo_batch_advance(sgeo_batch *batch, double cg_tol, int32_t cg_maxiter,
                        int32_t *memo_ready);
 int sgeo_batch_download(sgeo_batch *batch, int32_t col, double *x,
                         uint8_t *flags);
+/* CG steps run as captured HIP graphs of 8, 16 or 32 steps (default, fewer
+ * launch submissions for small meshes) or as separate launches (enable 0;
+ * HIP-event timing with sgeo_profile always uses separate launches).  The
+ * results are the same bits either way. */
+int sgeo_batch_set_graphs(sgeo_batch *batch, int enable);
 /* Totals since creation, as of the last host synchronisation: CG
  * iterations of all trials (seeded elastic solutions not counted), FP64
  * residuals of all trials, and batched steps (one step = one CG iteration

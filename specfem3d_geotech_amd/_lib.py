@@ -70,6 +70,7 @@ SIGNATURES = {
                                      C.c_int32, C.c_int64, i_p, i_p, i_p, d_p, i_p,
                                      i_p]),
     "sgeo_batch_download": (C.c_int, [vp, C.c_int32, d_p, u8_p]),
+    "sgeo_batch_set_graphs": (C.c_int, [vp, C.c_int]),
     "sgeo_batch_stats": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_int64)]),
 }

@@ -55,6 +55,9 @@ class BatchEngine:
     def release(self, col):
         check(L.lib().sgeo_batch_release(self.h, col), "sgeo_batch_release")
 
+    def set_graphs(self, enable: bool):
+        check(L.lib().sgeo_batch_set_graphs(self.h, int(enable)), "sgeo_batch_set_graphs")
+
     def plastic_iter(self, is_last, cg_tol, cg_maxiter, inner_tol, nl_tol):
         n = self.ncol
         il = i32(is_last)

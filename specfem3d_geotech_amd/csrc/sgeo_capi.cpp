@@ -1018,6 +1018,11 @@ struct sgeo_batch {
   int64_t stat_inner = 0, stat_outer = 0, stat_steps = 0;
   int k_ver = -1;  // mesh stiffness version the FP32 diagonal was made from
   size_t bytes = 0;
+  // HIP graphs of 8, 16 and 32 steps (every launch argument is fixed for
+  // the engine's lifetime: the steps read their state from the device)
+  bool use_graphs = true;
+  hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
 };
 
 static int ensure_packed(sgeo_mesh* m, int kprec) {
@@ -1130,7 +1135,17 @@ extern "C" int sgeo_batch_destroy(sgeo_batch* b) {
   if (b->h_ps) hipHostFree(b->h_ps);
   if (b->h_bc) hipHostFree(b->h_bc);
   if (b->h_cst) hipHostFree(b->h_cst);
+  for (int i = 0; i < 3; i++) {
+    if (b->gexec[i]) hipGraphExecDestroy(b->gexec[i]);
+    if (b->graph[i]) hipGraphDestroy(b->graph[i]);
+  }
   delete b;
+  return 0;
+}
+
+extern "C" int sgeo_batch_set_graphs(sgeo_batch* b, int enable) {
+  if (!b) return fail(SGEO_EARG, "null batch");
+  b->use_graphs = enable != 0;
   return 0;
 }
 
@@ -1254,29 +1269,50 @@ static bool pcg_running(int st) {
 // a call or replace their residual (a no-op launch when there are none), so
 // a column's arithmetic depends only on its own iterations.  h_bc holds the
 // device scalars on return.
-static int batch_steps(sgeo_batch* b, int chunk, bool* any_running) {
+// One CG step of every column (launch sequence; no memset, no host work).
+static void batch_step_launches(sgeo_batch* b, bool prof) {
   sgeo_mesh* m = b->mesh;
   sgeo_ctx* c = m->ctx;
   hipStream_t s = c->stream;
   const int B = b->B;
   const int64_t n1 = b->n1;
-  for (int it = 0; it < chunk; it++) {
-    launch_bmv_f64(s, m->nelmt, B, c->n_cu, m->d_KP64, m->d_gdof, b->d_x, b->d_Y64,
-                   b->d_cst, BCOL_BIT(BCOL_START) | BCOL_BIT(BCOL_REPLACE));
-    launch_bresid(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y64, b->d_load, b->d_dinv32,
-                  b->d_r, b->d_p, b->d_part, b->d_cst);
-    launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_RESID);
-    launch_bpupdate(s, n1, B, b->d_r, b->d_dinv32, b->d_p, b->d_cst, b->d_bc);
-    TRY(prof_mark(c));
-    launch_bmv_mfma_f32(s, m->nelmt, B, c->n_cu, m->d_KP32, m->d_gdof, b->d_p,
-                        b->d_Y32, b->d_cst, BCOL_BIT(BCOL_INNER));
-    TRY(prof_mark(c));
-    launch_bassemble_dot(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y32, b->d_p, b->d_kp,
-                         b->d_part, b->d_cst);
-    launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_ALPHA);
-    launch_bupdate(s, n1, B, b->d_p, b->d_kp, b->d_dinv32, b->d_x, b->d_r, b->d_part,
-                   b->d_cst, b->d_bc);
-    launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_INNER);
+  launch_bmv_f64(s, m->nelmt, B, c->n_cu, m->d_KP64, m->d_gdof, b->d_x, b->d_Y64,
+                 b->d_cst, BCOL_BIT(BCOL_START) | BCOL_BIT(BCOL_REPLACE));
+  launch_bresid(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y64, b->d_load, b->d_dinv32,
+                b->d_r, b->d_p, b->d_part, b->d_cst);
+  launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_RESID);
+  launch_bpupdate(s, n1, B, b->d_r, b->d_dinv32, b->d_p, b->d_cst, b->d_bc);
+  if (prof) prof_mark(c);
+  launch_bmv_mfma_f32(s, m->nelmt, B, c->n_cu, m->d_KP32, m->d_gdof, b->d_p, b->d_Y32,
+                      b->d_cst, BCOL_BIT(BCOL_INNER));
+  if (prof) prof_mark(c);
+  launch_bassemble_dot(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y32, b->d_p, b->d_kp,
+                       b->d_part, b->d_cst);
+  launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_ALPHA);
+  launch_bupdate(s, n1, B, b->d_p, b->d_kp, b->d_dinv32, b->d_x, b->d_r, b->d_part,
+                 b->d_cst, b->d_bc);
+  launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_INNER);
+}
+
+static int batch_steps(sgeo_batch* b, int chunk, bool* any_running) {
+  sgeo_mesh* m = b->mesh;
+  sgeo_ctx* c = m->ctx;
+  hipStream_t s = c->stream;
+  const int B = b->B;
+  const int gi = chunk == 8 ? 0 : chunk == 16 ? 1 : chunk == 32 ? 2 : -1;
+  if (b->use_graphs && !c->prof && gi >= 0) {
+    if (!b->gexec[gi]) {
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      for (int it = 0; it < chunk; it++) batch_step_launches(b, false);
+      hipGraph_t g = nullptr;
+      const hipError_t ec = hipStreamEndCapture(s, &g);
+      if (ec != hipSuccess || !g) return fail(SGEO_EHIP, "graph capture failed");
+      b->graph[gi] = g;
+      HIPCHK(hipGraphInstantiate(&b->gexec[gi], g, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(b->gexec[gi], s));
+  } else {
+    for (int it = 0; it < chunk; it++) batch_step_launches(b, c->prof);
   }
   b->stat_steps += chunk;
   HIPCHK(hipGetLastError());

@@ -180,3 +180,35 @@ def test_batched_realisations_match_oracle_brackets(ctx):
         for k, r in enumerate(rows):
             if r["nl_tot"] < ctl["nl_maxiter"]:
                 assert abs(a.rows[k]["nl_tot"] - r["nl_tot"]) <= 1
+
+
+def test_graph_steps_match_eager_launches(ctx):
+    """CG steps replayed from captured HIP graphs give the same bits as the
+    same steps launched one by one (several replays per graph, with host
+    synchronisations between them)."""
+    z = golden("slope_4x2x3")
+    mesh = mesh_from_golden(z)
+    spec = hex2spec(mesh)
+    ctl = control_from_golden(z)
+    _, ym, nu, phi, coh, psi = mesh.material_arrays()
+    out = []
+    for graphs in (True, False):
+        dm, ext = device_mesh(ctx, mesh, spec)
+        try:
+            eng = BatchEngine(dm, ext, 2)
+            eng.set_graphs(graphs)
+            a = Analysis(coh=coh, phi=phi, psi=psi, srf=list(ctl["srf"]))
+            try:
+                run_analyses(eng, [a], ym=ym, nu=nu, cg_tol=ctl["cg_tol"],
+                             cg_maxiter=ctl["cg_maxiter"], nl_tol=ctl["nl_tol"],
+                             nl_maxiter=ctl["nl_maxiter"], srf_parallel=2,
+                             keep_state=True)
+                steps = eng.stats()["batched_steps"]
+            finally:
+                eng.close()
+        finally:
+            dm.close()
+        out.append((a.rows, a.bracket, a.x_last.tobytes(), steps))
+        print("graphs" if graphs else "eager", a.rows, steps)
+    assert out[0][:3] == out[1][:3]
+    assert out[0][3] > 100  # many graph replays
