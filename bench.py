@@ -72,6 +72,10 @@ def parse():
     ap.add_argument("--analyses", type=int, default=None)
     ap.add_argument("--rate", type=float, default=None,
                     help="analysis mode: Poisson arrival rate (1/s) of requests")
+    ap.add_argument("--srf-parallel", type=int, default=1,
+                    help="analysis mode, batched: SRFs of one analysis in flight")
+    ap.add_argument("--own-strengths", action="store_true",
+                    help="analysis mode: the mesh's own strengths for every analysis")
     ap.add_argument("--requests", type=int, default=32)
     ap.add_argument("--min-elements", type=int, default=500)
     ap.add_argument("--max-elements", type=int, default=5000)
@@ -298,8 +302,12 @@ def mode_analysis(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
     gam, ym, nu, phi, coh, psi = mesh.material_arrays()
     nan = a.analyses or (1 if form != "batched" else a.trials)
     real = workloads.form_a(nan, len(gam))
+    if a.own_strengths:
+        real = [(coh.copy(), phi.copy())] * nan
     lat, rows_out = [], []
-    ctx.profile(1)
+    # HIP-event timing makes the batched engine launch its steps one by one
+    # instead of replaying graphs, so it is only taken for the FP64 form
+    ctx.profile(1 if form != "batched" else 0)
     if form == "batched":
         from specfem3d_geotech_amd.batch import Analysis, BatchEngine, device_mesh, \
             run_analyses
@@ -315,12 +323,17 @@ def mode_analysis(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
         def done(x):
             log(f"  analysis {x.tag} bracket {x.bracket} latency {x.latency:.1f} s, "
                 f"{len(x.rows)} SRFs")
+
+        def prog(st):
+            log(f"    {st['batched_steps']} steps, {st['cg_iterations']} CG iterations, "
+                f"{time.perf_counter() - t0:.0f} s")
         t0 = time.perf_counter()
         try:
             stats = run_analyses(eng, an, ym=ym, nu=nu, cg_tol=ctl["cg_tol"],
                                  cg_maxiter=ctl["cg_maxiter"], nl_tol=ctl["nl_tol"],
                                  nl_maxiter=ctl["nl_maxiter"], arrivals=arrivals,
-                                 on_done=done)
+                                 on_done=done, srf_parallel=a.srf_parallel,
+                                 progress=prog, progress_steps=4000)
         finally:
             eng.close()
             dm.close()
@@ -337,7 +350,8 @@ def mode_analysis(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
         kp, kf, red, bk, S = FORMS[form]
         t0 = time.perf_counter()
         for i, (c, p) in enumerate(real):
-            m2 = mesh.with_strengths(c, p) if a.analyses else mesh
+            m2 = mesh.with_strengths(c, p) if (a.analyses and not a.own_strengths) \
+                else mesh
             t1 = time.perf_counter()
             last = [time.perf_counter()]
 
@@ -364,7 +378,9 @@ def mode_analysis(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
         dtype="f64" if form != "batched" else "f32 inner CG (MFMA) + f64", data=DATA,
         config=dict(workload=wl.description, nelmt=E, neq=N, form=form, mode="analysis",
                     analyses=nan, trials=a.trials if form == "batched" else 1,
-                    arrival_rate=a.rate),
+                    arrival_rate=a.rate, srf_parallel=a.srf_parallel,
+                    strengths="own" if (a.own_strengths or not a.analyses and
+                                        form != "batched") else "form A"),
         latency_s=dict(p50=percentile(lat, 50), p99=percentile(lat, 99), max=max(lat)),
         analyses=rows_out, kernel=dict(name=kname, launches=nl,
                                        avg_ms=kms / max(nl, 1)), **extra)

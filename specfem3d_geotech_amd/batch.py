@@ -145,7 +145,8 @@ def run_analyses(engine, analyses: List[Analysis], *, ym, nu, cg_tol: float,
                  cg_maxiter: int, nl_tol: float, nl_maxiter: int,
                  inner_tol: float = 1e-5, srf_parallel: int = 1,
                  keep_state: bool = False, arrivals: Optional[List[float]] = None,
-                 clock=time.perf_counter, on_done=None, poll_steps: int = 64):
+                 clock=time.perf_counter, on_done=None, poll_steps: int = 64,
+                 progress=None, progress_steps: int = 2000):
     """Runs every analysis to its FOS bracket, keeping the engine's columns
     busy (continuous batching).
 
@@ -158,7 +159,9 @@ def run_analyses(engine, analyses: List[Analysis], *, ym, nu, cg_tol: float,
     requests are still to come the engine returns every `poll_steps` CG
     steps so they are admitted as soon as a column is free.  Until the
     engine holds the elastic solution (the first PCG call of any trial) only
-    one trial runs.  Returns the engine's statistics."""
+    one trial runs.  `progress`, if given, is called with the engine's
+    statistics at least every `progress_steps` CG steps.  Returns the
+    engine's statistics."""
     spec = engine.dmesh.spec
     ncol = engine.ncol
     cols: List[Optional[tuple]] = [None] * ncol  # (analysis, srf index)
@@ -243,8 +246,11 @@ def run_analyses(engine, analyses: List[Analysis], *, ym, nu, cg_tol: float,
             fill()
             continue
         pending = bool(queue) and arrivals is not None
+        cap = poll_steps if pending else (progress_steps if progress else 1 << 40)
         r = engine.advance(cg_tol, cg_maxiter, inner_tol, nl_tol, nl_maxiter,
-                           max_steps=poll_steps if pending else 1 << 40)
+                           max_steps=cap)
+        if progress:
+            progress(engine.stats())
         memo_ready = r["memo_ready"]
         for j in np.nonzero(r["ended"])[0]:
             if cols[j] is None:  # abandoned above: its analysis already ended

@@ -113,14 +113,14 @@ def read_calls_var(workdir: str):
 def parse_stage_summary(path: str) -> np.ndarray:
     """Excavation rows (stage, cg_tot, nl_tot, uxmax, umax, fmax)
     (src/excavation.F90:634)."""
-    rows = []
     lines = open(path).read().splitlines()
     start = [i for i, ln in enumerate(lines) if "STEP, CGITER" in ln][0] + 1
+    toks = []  # list-directed output may wrap a row over two lines
     for ln in lines[start:]:
-        if "ELAPSED" in ln or not ln.strip() or not ln.split()[0][0].isdigit():
+        if "ELAPSED" in ln or not ln.strip():
             break
-        rows.append([float(t) for t in ln.split()])
-    return np.array(rows, dtype=np.float64)
+        toks += [float(t) for t in ln.split()]
+    return np.array(toks, dtype=np.float64).reshape(-1, 6)
 
 
 def excavation_case(name, mesh, control, excav):

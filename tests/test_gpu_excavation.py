@@ -98,3 +98,23 @@ def test_excavation_pit_tree_reductions(ctx):
         print(r, rr.tolist())
         assert r["nl_tot"] == rr[2]
         assert abs(r["uxmax"] - rr[3]) <= 1e-8 * rr[3]
+
+
+def test_reference_example_excavation_3d(ctx_seq):
+    """The reference's own example input/excavation_3d: 1,700 elements,
+    nu = 0.49, three stages removing material ids 2, 3, 4, ten load
+    increments each (yielding in stage 3).  Stage summary rows of the
+    compiled reference (reference-order reductions)."""
+    z = golden("excavation_3d")
+    mesh = mesh_from_golden(z)
+    ctl = _control(z)
+    res = excavation3d(ctx_seq, mesh, **ctl)
+    ref = z["summary"]
+    assert len(res.rows) == ref.shape[0]
+    for r, rr in zip(res.rows, ref):
+        print(r, rr.tolist())
+        assert r["stage"] == rr[0] and r["nl_tot"] == rr[2]
+        assert r["cg_tot"] == rr[1]
+        assert abs(r["uxmax"] - rr[3]) <= 1e-10 * rr[3]
+        assert abs(r["umax"] - rr[4]) <= 1e-10 * rr[4]
+    assert res.cg_log == list(z["cg_log"])
