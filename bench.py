@@ -80,6 +80,9 @@ def parse():
     ap.add_argument("--min-elements", type=int, default=500)
     ap.add_argument("--max-elements", type=int, default=5000)
     ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="batched engine: launch every step's kernels one by one "
+                         "(kernel-trace profilers record graph replays poorly)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-iters", type=int, default=30)
     return ap.parse_args()
@@ -313,6 +316,8 @@ def mode_analysis(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
             run_analyses
         dm, ext = device_mesh(ctx, mesh, spec)
         eng = BatchEngine(dm, ext, a.trials)
+        if a.no_graphs:
+            eng.set_graphs(False)
         an = [Analysis(coh=c, phi=p, psi=psi.copy(), srf=list(ctl["srf"]), tag=i)
               for i, (c, p) in enumerate(real)]
         arrivals = None

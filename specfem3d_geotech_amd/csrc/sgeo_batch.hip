@@ -178,13 +178,14 @@ void k_bmv_mfma(int32_t nelmt, int B,
 // FP64 batched element product (the true residual's K x): one element per
 // block; the packed upper triangle of K_e and the gathered X_e (81 x B) in
 // LDS; thread t owns column j = t % B and rows t / B + k * (nthreads / B),
-// each summed over c = 0..80 in order.  Columns col0 .. col0+B-1 of arrays
-// with column stride Bs; columns not in want_mask read zero, write nothing.
+// each summed over c = 0..80 in order.  blockIdx.y picks the column chunk
+// col0 = blockIdx.y * B .. col0+B-1 of arrays with column stride Bs; columns
+// not in want_mask read zero, write nothing.
 // --------------------------------------------------------------------------
 // Persistent: a few blocks per CU walk the elements, so a launch in which no
 // column is wanted (the common case: the kernel is launched every step and
 // only starting or replacing columns need it) costs a handful of blocks.
-__global__ __launch_bounds__(256) void k_bmv64(int32_t nelmt, int B, int Bs, int col0,
+__global__ __launch_bounds__(256) void k_bmv64(int32_t nelmt, int B, int Bs,
                                                const double* __restrict__ KP,
                                                const int32_t* __restrict__ gdof,
                                                const double* __restrict__ X,
@@ -195,9 +196,10 @@ __global__ __launch_bounds__(256) void k_bmv64(int32_t nelmt, int B, int Bs, int
   double* u = reinterpret_cast<double*>(smem);
   double* pe = u + SGEO_KPACK;
   const int t = threadIdx.x, nt = blockDim.x;
+  const int col0 = blockIdx.y * B;  // this block's column chunk
   bool anyw = false;
   for (int q = 0; q < B; q++) anyw = anyw || wanted(cst, col0 + q, want_mask);
-  if (!anyw) return;  // uniform per launch
+  if (!anyw) return;  // uniform per block
   const int j = t % B;
   const int nrg = nt / B, rg = t / B;
   const bool mine = wanted(cst, col0 + j, want_mask);
@@ -224,44 +226,43 @@ __global__ __launch_bounds__(256) void k_bmv64(int32_t nelmt, int B, int Bs, int
 }
 
 // --------------------------------------------------------------------------
-// batched vector kernels.  Equations are split into fixed chunks of
-// BCHUNK; thread t owns column j = t % B and chunk blockIdx*(256/B) + t/B,
-// and walks its chunk in order, so every per-(chunk, column) partial is a
-// sequential sum whose value does not depend on B.  The one-block-per-column
-// finalize then combines the chunk partials in a fixed order.
+// batched vector kernels.  Equations are split into fixed chunks of ck
+// equations (ck depends on the equation count only, see batch_chunk);
+// thread t owns column j = t % B and chunk blockIdx*(256/B) + t/B, and walks
+// its chunk in order, so every per-(chunk, column) partial is a sequential
+// sum whose value does not depend on B.  The finalize then combines the
+// chunk partials in a fixed order.
 // Partials: part[(kind*nch + chunk)*B + j].
 // --------------------------------------------------------------------------
-#define BCHUNK 64
-
 struct ChunkIdx {
   int j;
   int64_t chunk, g0, g1;
   bool valid;
 };
 
-__device__ __forceinline__ ChunkIdx chunk_of(int64_t n1, int B) {
+__device__ __forceinline__ ChunkIdx chunk_of(int64_t n1, int B, int ck) {
   ChunkIdx c;
   const int t = threadIdx.x;
   c.j = t % B;
   c.chunk = (int64_t)blockIdx.x * (blockDim.x / B) + t / B;
-  c.g0 = c.chunk * BCHUNK;
-  c.g1 = c.g0 + BCHUNK < n1 ? c.g0 + BCHUNK : n1;
+  c.g0 = c.chunk * ck;
+  c.g1 = c.g0 + ck < n1 ? c.g0 + ck : n1;
   c.valid = c.g0 < n1;
   return c;
 }
 
-__device__ __forceinline__ int64_t nchunks(int64_t n1) {
-  return (n1 + BCHUNK - 1) / BCHUNK;
+__device__ __forceinline__ int64_t nchunks(int64_t n1, int ck) {
+  return (n1 + ck - 1) / ck;
 }
 
 // p = D r + beta p for INNER columns with a pending beta
-__global__ __launch_bounds__(256) void k_bpupdate(int64_t n1, int B,
+__global__ __launch_bounds__(256) void k_bpupdate(int64_t n1, int B, int ck,
                                                   const float* __restrict__ r,
                                                   const float* __restrict__ dinv,
                                                   float* __restrict__ p,
                                                   const int32_t* __restrict__ cst,
                                                   const BCol* __restrict__ bc) {
-  const ChunkIdx c = chunk_of(n1, B);
+  const ChunkIdx c = chunk_of(n1, B, ck);
   if (!c.valid || cst[c.j] != BCOL_INNER || !bc[c.j].pending) return;
   const float beta = (float)bc[c.j].beta;
   for (int64_t g = c.g0; g < c.g1; g++) {
@@ -278,16 +279,16 @@ __global__ __launch_bounds__(256) void k_bpupdate(int64_t n1, int B,
 // loads.
 template <bool WAVE_CHUNK>
 __global__ __launch_bounds__(256) void k_bassemble_dot(
-    int64_t n1, int B, const int32_t* __restrict__ rowptr,
+    int64_t n1, int B, int ck, const int32_t* __restrict__ rowptr,
     const int32_t* __restrict__ slots, const float* __restrict__ Y,
     const float* __restrict__ p, float* __restrict__ kp,
     double* __restrict__ part, const int32_t* __restrict__ cst) {
-  ChunkIdx c = chunk_of(n1, B);
+  ChunkIdx c = chunk_of(n1, B, ck);
   if (WAVE_CHUNK) {
     const int ch = __builtin_amdgcn_readfirstlane((int)c.chunk);
     c.chunk = ch;
-    c.g0 = (int64_t)ch * BCHUNK;
-    c.g1 = c.g0 + BCHUNK < n1 ? c.g0 + BCHUNK : n1;
+    c.g0 = (int64_t)ch * ck;
+    c.g1 = c.g0 + ck < n1 ? c.g0 + ck : n1;
     c.valid = c.g0 < n1;
   }
   if (!c.valid || cst[c.j] != BCOL_INNER) return;
@@ -310,7 +311,7 @@ __global__ __launch_bounds__(256) void k_bassemble_dot(
     s = s + pg * (double)acc;
     m = fmax(m, fabs(pg));
   }
-  const int64_t nch = nchunks(n1);
+  const int64_t nch = nchunks(n1, ck);
   part[(0 * nch + c.chunk) * B + c.j] = s;
   part[(1 * nch + c.chunk) * B + c.j] = m;
 }
@@ -318,11 +319,11 @@ __global__ __launch_bounds__(256) void k_bassemble_dot(
 // x += alpha p (FP64); r -= alpha kp; partial max|x| (after the update, with
 // x(0) = 0) and r.(D r) (FP64)
 __global__ __launch_bounds__(256) void k_bupdate(
-    int64_t n1, int B, const float* __restrict__ p, const float* __restrict__ kp,
+    int64_t n1, int B, int ck, const float* __restrict__ p, const float* __restrict__ kp,
     const float* __restrict__ dinv, double* __restrict__ x, float* __restrict__ r,
     double* __restrict__ part, const int32_t* __restrict__ cst,
     const BCol* __restrict__ bc) {
-  const ChunkIdx c = chunk_of(n1, B);
+  const ChunkIdx c = chunk_of(n1, B, ck);
   if (!c.valid || cst[c.j] != BCOL_INNER) return;
   const double alpha = bc[c.j].alpha;
   const float alpha32 = (float)alpha;
@@ -337,7 +338,7 @@ __global__ __launch_bounds__(256) void k_bupdate(
     const float zg = dinv[g] * rg;
     s = s + (double)rg * (double)zg;
   }
-  const int64_t nch = nchunks(n1);
+  const int64_t nch = nchunks(n1, ck);
   part[(0 * nch + c.chunk) * B + c.j] = mx;
   part[(1 * nch + c.chunk) * B + c.j] = s;
 }
@@ -345,12 +346,12 @@ __global__ __launch_bounds__(256) void k_bupdate(
 // FP64 residual for START / REPLACE columns: r64 = load - sum Y64 (element
 // order, FP64); r = (float) r64; START: p = D r; partial r.(D r)
 __global__ __launch_bounds__(256) void k_bresid(
-    int64_t n1, int B, const int32_t* __restrict__ rowptr,
+    int64_t n1, int B, int ck, const int32_t* __restrict__ rowptr,
     const int32_t* __restrict__ slots, const double* __restrict__ Y64,
     const double* __restrict__ load, const float* __restrict__ dinv,
     float* __restrict__ r, float* __restrict__ p, double* __restrict__ part,
     const int32_t* __restrict__ cst) {
-  const ChunkIdx c = chunk_of(n1, B);
+  const ChunkIdx c = chunk_of(n1, B, ck);
   if (!c.valid) return;
   const int st = cst[c.j];
   if (st != BCOL_START && st != BCOL_REPLACE) return;
@@ -368,7 +369,7 @@ __global__ __launch_bounds__(256) void k_bresid(
     if (st == BCOL_START) p[q] = zg;
     s = s + (double)rg * (double)zg;
   }
-  const int64_t nch = nchunks(n1);
+  const int64_t nch = nchunks(n1, ck);
   part[(0 * nch + c.chunk) * B + c.j] = s;
   part[(1 * nch + c.chunk) * B + c.j] = 0.0;
 }
@@ -403,7 +404,22 @@ __global__ __launch_bounds__(256) void k_bgroup(int64_t nch, int B,
   const bool max0 = fin_max0(what), max1 = fin_max1(what);
   const int64_t b0 = g * BGROUP, b1 = b0 + BGROUP < nch ? b0 + BGROUP : nch;
   double v0 = 0.0, v1 = 0.0;
-  for (int64_t b = b0; b < b1; b++) {
+  int64_t b = b0;
+  // eight partials loaded together, then combined in chunk order
+  for (; b + 8 <= b1; b += 8) {
+    double a0[8], a1[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      a0[k] = part[(0 * nch + b + k) * B + j];
+      a1[k] = part[(1 * nch + b + k) * B + j];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      v0 = max0 ? fmax(v0, a0[k]) : v0 + a0[k];
+      v1 = max1 ? fmax(v1, a1[k]) : v1 + a1[k];
+    }
+  }
+  for (; b < b1; b++) {
     const double a0 = part[(0 * nch + b) * B + j];
     const double a1 = part[(1 * nch + b) * B + j];
     v0 = max0 ? fmax(v0, a0) : v0 + a0;
@@ -494,12 +510,12 @@ __global__ void k_bload(int64_t n1, int B, const double* __restrict__ ext,
 }
 
 // uerr per column: x(0)=0, partial max|x-oldx| and max|x|, oldx = x
-__global__ __launch_bounds__(256) void k_buerr(int64_t n1, int B,
+__global__ __launch_bounds__(256) void k_buerr(int64_t n1, int B, int ck,
                                                double* __restrict__ x,
                                                double* __restrict__ oldx,
                                                double* __restrict__ part,
                                                const int32_t* __restrict__ active) {
-  const ChunkIdx c = chunk_of(n1, B);
+  const ChunkIdx c = chunk_of(n1, B, ck);
   if (!c.valid || !active[c.j]) return;
   double md = 0.0, mx = 0.0;
   for (int64_t g = c.g0; g < c.g1; g++) {
@@ -510,7 +526,7 @@ __global__ __launch_bounds__(256) void k_buerr(int64_t n1, int B,
     mx = fmax(mx, fabs(xg));
     oldx[q] = xg;
   }
-  const int64_t nch = nchunks(n1);
+  const int64_t nch = nchunks(n1, ck);
   part[(0 * nch + c.chunk) * B + c.j] = md;
   part[(1 * nch + c.chunk) * B + c.j] = mx;
 }
@@ -590,12 +606,24 @@ static inline unsigned grid_stride(int64_t n) {
   return (unsigned)(b < 65536 ? (b > 0 ? b : 1) : 65536);
 }
 
+// Chunk length of the vector kernels: a function of the equation count
+// only (never of B), so a column's partial sums stay independent of the
+// batch.  Large systems use 64 equations per thread; small ones shorter
+// chunks, so that the launch still has enough waves to hide the latency of
+// each thread's walk.
+static int batch_chunk(int64_t n1) {
+  int ck = 64;
+  while (ck > 4 && n1 / ck < 16384) ck /= 2;
+  return ck;
+}
+
 static inline int64_t nchunks_host(int64_t n1) {
-  return (n1 + BCHUNK - 1) / BCHUNK;
+  const int ck = batch_chunk(n1);
+  return (n1 + ck - 1) / ck;
 }
 
 int64_t batch_partials(int64_t n1) {
-  const int64_t nch = (n1 + BCHUNK - 1) / BCHUNK;
+  const int64_t nch = nchunks_host(n1);
   return 2 * nch + 2 * ((nch + BGROUP - 1) / BGROUP);  // chunk + group partials
 }
 
@@ -644,34 +672,34 @@ void launch_bmv_f64(hipStream_t s, int32_t nelmt, int B, int n_cu, const double*
   int64_t g = (int64_t)n_cu * 3;
   if (g > nelmt) g = nelmt;
   if (g < 1) g = 1;
-  for (int c0 = 0; c0 < B; c0 += Bc)
-    hipLaunchKernelGGL(k_bmv64, dim3((unsigned)g), dim3(bmv_threads(Bc)), lds, s,
-                       nelmt, Bc, B, c0, KP, gdof, X, Y, cst, want_mask);
+  hipLaunchKernelGGL(k_bmv64, dim3((unsigned)g, (unsigned)(B / Bc)), dim3(bmv_threads(Bc)),
+                     lds, s, nelmt, Bc, B, KP, gdof, X, Y, cst, want_mask);
 }
 
 void launch_bassemble_dot(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
                           const int32_t* slots, const float* Y, const float* p,
                           float* kp, double* part, const int32_t* cst) {
+  const int ck = batch_chunk(n1);
   if (B == 64)
     hipLaunchKernelGGL(k_bassemble_dot<true>, dim3(chunk_blocks(n1, B)), dim3(256), 0,
-                       s, n1, B, rowptr, slots, Y, p, kp, part, cst);
+                       s, n1, B, ck, rowptr, slots, Y, p, kp, part, cst);
   else
     hipLaunchKernelGGL(k_bassemble_dot<false>, dim3(chunk_blocks(n1, B)), dim3(256), 0,
-                       s, n1, B, rowptr, slots, Y, p, kp, part, cst);
+                       s, n1, B, ck, rowptr, slots, Y, p, kp, part, cst);
 }
 
 void launch_bupdate(hipStream_t s, int64_t n1, int B, const float* p,
                     const float* kp, const float* dinv, double* x, float* r,
                     double* part, const int32_t* cst, const BCol* bc) {
   hipLaunchKernelGGL(k_bupdate, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
-                     p, kp, dinv, x, r, part, cst, bc);
+                     batch_chunk(n1), p, kp, dinv, x, r, part, cst, bc);
 }
 
 void launch_bpupdate(hipStream_t s, int64_t n1, int B, const float* r,
                      const float* dinv, float* p, const int32_t* cst,
                      const BCol* bc) {
   hipLaunchKernelGGL(k_bpupdate, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
-                     r, dinv, p, cst, bc);
+                     batch_chunk(n1), r, dinv, p, cst, bc);
 }
 
 void launch_bresid(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
@@ -679,7 +707,7 @@ void launch_bresid(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
                    const float* dinv, float* r, float* p, double* part,
                    const int32_t* cst) {
   hipLaunchKernelGGL(k_bresid, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
-                     rowptr, slots, Y64, load, dinv, r, p, part, cst);
+                     batch_chunk(n1), rowptr, slots, Y64, load, dinv, r, p, part, cst);
 }
 
 void launch_bfinalize(hipStream_t s, int64_t n1, int B, double* part,
@@ -703,7 +731,7 @@ void launch_buerr(hipStream_t s, int64_t n1, int B, double* x, double* oldx,
                   double* part, PlasticScalars* ps, const int32_t* active,
                   double nl_tol) {
   hipLaunchKernelGGL(k_buerr, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
-                     x, oldx, part, active);
+                     batch_chunk(n1), x, oldx, part, active);
   hipLaunchKernelGGL(k_buerr_finalize, dim3(B), dim3(256), 0, s, nchunks_host(n1),
                      B, part, ps, active, nl_tol);
 }
