@@ -159,6 +159,8 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
     t64, n64, ms64 = single_trial_pcg(ctx, dm, ext, ym, nu, phi, coh, psi, 200, 10)
     fp64_ms_iter = t64 / 200 * 1e3
     eng = BatchEngine(dm, ext, B)
+    if a.no_graphs:
+        eng.set_graphs(False)
     real = workloads.form_a(B, len(gam))
     srf = list(ctl["srf"])
     nxt = [0] * B  # next SRF index per column (each column runs one analysis)
@@ -199,6 +201,10 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
         while eng.stats()["batched_steps"] - s0["batched_steps"] < warm:
             handle(eng.advance(max_steps=warm, **kw))
         ctx.synchronize()
+        tune = eng.autotune(5)
+        log("autotune (MFMA product launch geometry): " + ", ".join(
+            f"{c['blocks_per_cu']}/CU {'xcd' if c['xcd_split'] else 'flat'} {c['ms']:.3f} ms"
+            for c in tune["candidates"]))
         ctx.profile(1)
         s0 = eng.stats()
         t0 = time.perf_counter()
@@ -244,6 +250,10 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
                          ebe_kernel_ms=round(ms64 / max(n64, 1), 4)),
         speedup_vs_fp64_per_trial_iteration=round(fp64_ms_iter / per_trial_ms, 2),
         trials_ended_in_run=len(done_rows),
+        autotune=dict(blocks_per_cu=tune["blocks_per_cu"], xcd_split=tune["xcd_split"],
+                      candidates_ms={f"{c['blocks_per_cu']}/CU,"
+                                     f"{'xcd' if c['xcd_split'] else 'flat'}":
+                                     round(c["ms"], 4) for c in tune["candidates"]}),
         roofline=roofline("k_bmv_mfma<4> (v_mfma_f32_16x16x4f32)", kbytes, nl, kms,
                           dict(bytes_model="E*(3328*4 + 81*4 + 2*81*4*b_eff), "
                                "b_eff = mean trials iterating per step",

@@ -255,6 +255,16 @@ int sgeo_batch_download(sgeo_batch *batch, int32_t col, double *x,
  * HIP-event timing with sgeo_profile always uses separate launches).  The
  * results are the same bits either way. */
 int sgeo_batch_set_graphs(sgeo_batch *batch, int enable);
+/* Launch-geometry autotuner of the MFMA element product (no counterpart in
+ * the reference, which has no GPU product).  Times `reps` launches of each
+ * candidate on this engine's mesh and iterating columns -- blocks per CU
+ * 3, 2, 1 with the XCD-aware element split, then 3, 2, 1 with a plain
+ * grid-stride walk (ms_table[6], in that order, may be NULL) -- and keeps the
+ * fastest for later steps.  Needs at least one column iterating
+ * (SGEO_ESTATE otherwise).  The geometry never changes a result bit, and
+ * tuning between steps does not change the trials' results. */
+int sgeo_batch_autotune(sgeo_batch *batch, int32_t reps, int32_t *blocks_per_cu,
+                        int32_t *xcd_split, double *ms_best, double *ms_table);
 /* Totals since creation, as of the last host synchronisation: CG
  * iterations of all trials (seeded elastic solutions not counted), FP64
  * residuals of all trials, and batched steps (one step = one CG iteration

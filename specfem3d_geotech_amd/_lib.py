@@ -71,6 +71,9 @@ SIGNATURES = {
                                      i_p]),
     "sgeo_batch_download": (C.c_int, [vp, C.c_int32, d_p, u8_p]),
     "sgeo_batch_set_graphs": (C.c_int, [vp, C.c_int]),
+    "sgeo_batch_autotune": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]),
     "sgeo_batch_stats": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_int64)]),
 }

@@ -58,6 +58,19 @@ class BatchEngine:
     def set_graphs(self, enable: bool):
         check(L.lib().sgeo_batch_set_graphs(self.h, int(enable)), "sgeo_batch_set_graphs")
 
+    def autotune(self, reps: int = 5) -> Dict:
+        """Launch geometry of the MFMA product, timed on this mesh with the
+        columns iterating now (at least one must be); keeps the fastest."""
+        bpc, xs = C.c_int32(), C.c_int32()
+        best = C.c_double()
+        table = np.zeros(6)
+        check(L.lib().sgeo_batch_autotune(self.h, int(reps), C.byref(bpc), C.byref(xs),
+                                          C.byref(best), ptr(table)), "sgeo_batch_autotune")
+        cands = [dict(blocks_per_cu=b, xcd_split=x, ms=float(t))
+                 for (x, b), t in zip([(1, 3), (1, 2), (1, 1), (0, 3), (0, 2), (0, 1)], table)]
+        return dict(blocks_per_cu=bpc.value, xcd_split=xs.value, ms=best.value,
+                    candidates=cands)
+
     def plastic_iter(self, is_last, cg_tol, cg_maxiter, inner_tol, nl_tol):
         n = self.ncol
         il = i32(is_last)

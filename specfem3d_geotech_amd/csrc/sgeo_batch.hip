@@ -47,18 +47,20 @@ __device__ __forceinline__ int tri_col(int q) {  // c with c(c+1)/2 <= q < (c+1)
   return c;
 }
 
-// Persistent and software-pipelined: the grid holds a few blocks per CU;
-// each XCD (blocks are dispatched to the 8 XCDs round-robin) walks its own
-// contiguous eighth of the elements, so elements in flight together share
-// nodes and their gathered P rows hit in that XCD's L2.  While element e is
-// multiplied, K and gdof of the block's next element are already in
-// registers, and the next element's P rows are gathered before its K is
-// unpacked.
+// Persistent and software-pipelined: the grid holds a few blocks per CU.
+// With nxcd = 8, each XCD (blocks are dispatched to the 8 XCDs round-robin)
+// walks its own contiguous eighth of the elements, so elements in flight
+// together share nodes and their gathered P rows hit in that XCD's L2;
+// nxcd = 1 is a plain grid-stride walk.  While element e is multiplied, K
+// and gdof of the block's next element are already in registers, and the
+// next element's P rows are gathered before its K is unpacked.  The grid
+// size and nxcd change which block computes an element, never how: the
+// launch geometry does not change any result bit.
 #define BMV_XCD 8
 
 template <int NT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-void k_bmv_mfma(int32_t nelmt, int B,
+void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
                                                   const float* __restrict__ KP,
                                                   const int32_t* __restrict__ gdof,
                                                   const float* __restrict__ P,
@@ -72,11 +74,11 @@ void k_bmv_mfma(int32_t nelmt, int B,
   __shared__ int32_t sg[2][BMV_KC];
   const int t = threadIdx.x;
   // element range of this block's XCD, and the block's stride within it
-  const int xcd = blockIdx.x % BMV_XCD;
-  const int nbx = gridDim.x / BMV_XCD;  // grid is a multiple of BMV_XCD
-  const int64_t xe0 = (int64_t)nelmt * xcd / BMV_XCD;
-  const int64_t xe1 = (int64_t)nelmt * (xcd + 1) / BMV_XCD;
-  int64_t e = xe0 + blockIdx.x / BMV_XCD;
+  const int xcd = blockIdx.x % nxcd;
+  const int nbx = gridDim.x / nxcd;  // grid is a multiple of nxcd
+  const int64_t xe0 = (int64_t)nelmt * xcd / nxcd;
+  const int64_t xe1 = (int64_t)nelmt * (xcd + 1) / nxcd;
+  int64_t e = xe0 + blockIdx.x / nxcd;
   // column tile of this wave; columns outside the mask are neither
   // gathered nor multiplied nor stored, but the wave keeps to the block's
   // barriers
@@ -639,26 +641,27 @@ static inline int bmv_threads(int B) {
   return t < 64 ? 64 : t;
 }
 
-// blocks per CU of the persistent MFMA product (3 waves per SIMD: the
-// kernel is built for at most 168 VGPRs)
-#define BMV_BLOCKS_PER_CU 3
-
-void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu,
+// Launch geometry of the persistent MFMA product: blocks per CU (at most 3:
+// the kernel is built for 3 waves per SIMD, at most 168 VGPRs) and the
+// XCD-aware element split (nxcd 8) or a plain grid-stride walk (nxcd 1).
+void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu, BmvGeom geom,
                          const float* KP, const int32_t* gdof, const float* P,
                          float* Y, const int32_t* cst, int want_mask) {
-  int64_t g = (int64_t)n_cu * BMV_BLOCKS_PER_CU;
-  const int64_t need = ((int64_t)nelmt + BMV_XCD - 1) / BMV_XCD * BMV_XCD;
+  const int nx = geom.xcd_split ? BMV_XCD : 1;
+  const int bpc = geom.blocks_per_cu < 1 ? 1 : geom.blocks_per_cu > 3 ? 3 : geom.blocks_per_cu;
+  int64_t g = (int64_t)n_cu * bpc;
+  const int64_t need = ((int64_t)nelmt + nx - 1) / nx * nx;
   if (g > need) g = need;
-  g = (g + BMV_XCD - 1) / BMV_XCD * BMV_XCD;  // a multiple of the XCD count
+  g = (g + nx - 1) / nx * nx;  // a multiple of the XCD count
   const dim3 grid((unsigned)g);
   if (B >= 64)
-    hipLaunchKernelGGL(k_bmv_mfma<4>, grid, dim3(256), 0, s, nelmt, B, KP, gdof, P,
+    hipLaunchKernelGGL(k_bmv_mfma<4>, grid, dim3(256), 0, s, nelmt, B, nx, KP, gdof, P,
                        Y, cst, want_mask);
   else if (B == 32)
-    hipLaunchKernelGGL(k_bmv_mfma<2>, grid, dim3(256), 0, s, nelmt, B, KP, gdof, P,
+    hipLaunchKernelGGL(k_bmv_mfma<2>, grid, dim3(256), 0, s, nelmt, B, nx, KP, gdof, P,
                        Y, cst, want_mask);
   else
-    hipLaunchKernelGGL(k_bmv_mfma<1>, grid, dim3(256), 0, s, nelmt, B, KP, gdof, P,
+    hipLaunchKernelGGL(k_bmv_mfma<1>, grid, dim3(256), 0, s, nelmt, B, nx, KP, gdof, P,
                        Y, cst, want_mask);
 }
 

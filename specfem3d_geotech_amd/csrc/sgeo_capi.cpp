@@ -1023,6 +1023,7 @@ struct sgeo_batch {
   bool use_graphs = true;
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+  BmvGeom geom;  // launch geometry of the MFMA product
 };
 
 static int ensure_packed(sgeo_mesh* m, int kprec) {
@@ -1121,6 +1122,15 @@ extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extlo
   return 0;
 }
 
+static void batch_drop_graphs(sgeo_batch* b) {
+  for (int i = 0; i < 3; i++) {
+    if (b->gexec[i]) hipGraphExecDestroy(b->gexec[i]);
+    if (b->graph[i]) hipGraphDestroy(b->graph[i]);
+    b->gexec[i] = nullptr;
+    b->graph[i] = nullptr;
+  }
+}
+
 extern "C" int sgeo_batch_destroy(sgeo_batch* b) {
   if (!b) return 0;
   hipSetDevice(b->mesh->ctx->device);
@@ -1135,10 +1145,7 @@ extern "C" int sgeo_batch_destroy(sgeo_batch* b) {
   if (b->h_ps) hipHostFree(b->h_ps);
   if (b->h_bc) hipHostFree(b->h_bc);
   if (b->h_cst) hipHostFree(b->h_cst);
-  for (int i = 0; i < 3; i++) {
-    if (b->gexec[i]) hipGraphExecDestroy(b->gexec[i]);
-    if (b->graph[i]) hipGraphDestroy(b->graph[i]);
-  }
+  batch_drop_graphs(b);
   delete b;
   return 0;
 }
@@ -1146,6 +1153,65 @@ extern "C" int sgeo_batch_destroy(sgeo_batch* b) {
 extern "C" int sgeo_batch_set_graphs(sgeo_batch* b, int enable) {
   if (!b) return fail(SGEO_EARG, "null batch");
   b->use_graphs = enable != 0;
+  return 0;
+}
+
+// Launch-geometry autotuner of the MFMA product: times `reps` launches of
+// every candidate (blocks per CU 1..3, XCD-aware split on/off) on the
+// engine's own mesh and the columns iterating now, and keeps the fastest.
+// The product only rewrites the element products of iterating columns from
+// their current search directions, which the next step recomputes before
+// any use, so tuning between steps changes no result.
+extern "C" int sgeo_batch_autotune(sgeo_batch* b, int32_t reps, int32_t* blocks_per_cu,
+                                   int32_t* xcd_split, double* ms_best,
+                                   double* ms_table) {
+  if (!b || reps < 1) return fail(SGEO_EARG, "sgeo_batch_autotune: bad arguments");
+  sgeo_mesh* m = b->mesh;
+  sgeo_ctx* c = m->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  int inner = 0;
+  for (int j = 0; j < b->B; j++) inner += b->phase[j] == BPH_RUN && b->cst[j] == BCOL_INNER;
+  if (!inner) return fail(SGEO_ESTATE, "sgeo_batch_autotune: no column is iterating");
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  double best = 1e30;
+  BmvGeom pick = b->geom;
+  int k = 0;
+  for (int xs = 1; xs >= 0; xs--) {
+    for (int bpc = 3; bpc >= 1; bpc--, k++) {
+      BmvGeom g;
+      g.blocks_per_cu = bpc;
+      g.xcd_split = xs;
+      auto launch = [&]() {
+        launch_bmv_mfma_f32(s, m->nelmt, b->B, c->n_cu, g, m->d_KP32, m->d_gdof, b->d_p,
+                            b->d_Y32, b->d_cst, BCOL_BIT(BCOL_INNER));
+      };
+      launch();  // warm-up
+      HIPCHK(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; r++) launch();
+      HIPCHK(hipEventRecord(e1, s));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      const double avg = (double)ms / reps;
+      if (ms_table) ms_table[k] = avg;
+      if (avg < best) {
+        best = avg;
+        pick = g;
+      }
+    }
+  }
+  HIPCHK(hipGetLastError());
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (pick.blocks_per_cu != b->geom.blocks_per_cu || pick.xcd_split != b->geom.xcd_split)
+    batch_drop_graphs(b);  // captured launches carry the old geometry
+  b->geom = pick;
+  if (blocks_per_cu) *blocks_per_cu = pick.blocks_per_cu;
+  if (xcd_split) *xcd_split = pick.xcd_split;
+  if (ms_best) *ms_best = best;
   return 0;
 }
 
@@ -1283,8 +1349,8 @@ static void batch_step_launches(sgeo_batch* b, bool prof) {
   launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_RESID);
   launch_bpupdate(s, n1, B, b->d_r, b->d_dinv32, b->d_p, b->d_cst, b->d_bc);
   if (prof) prof_mark(c);
-  launch_bmv_mfma_f32(s, m->nelmt, B, c->n_cu, m->d_KP32, m->d_gdof, b->d_p, b->d_Y32,
-                      b->d_cst, BCOL_BIT(BCOL_INNER));
+  launch_bmv_mfma_f32(s, m->nelmt, B, c->n_cu, b->geom, m->d_KP32, m->d_gdof, b->d_p,
+                      b->d_Y32, b->d_cst, BCOL_BIT(BCOL_INNER));
   if (prof) prof_mark(c);
   launch_bassemble_dot(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y32, b->d_p, b->d_kp,
                        b->d_part, b->d_cst);

@@ -131,8 +131,15 @@ struct BCol {  // per-column scalars, device resident
   int32_t nrep;         // FP64 residuals of this solve
 };
 
+// launch geometry of the persistent MFMA product (result bits do not
+// depend on it; chosen by sgeo_batch_autotune)
+struct BmvGeom {
+  int blocks_per_cu = 3;
+  int xcd_split = 1;
+};
+
 int64_t batch_partials(int64_t n1);
-void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu,
+void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu, BmvGeom geom,
                          const float* KP, const int32_t* gdof, const float* P,
                          float* Y, const int32_t* cst, int want_mask);
 void launch_bmv_f64(hipStream_t s, int32_t nelmt, int B, int n_cu, const double* KP,
