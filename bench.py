@@ -2,12 +2,15 @@
 """Benchmark of the EBE-PCG + viscoplastic hot path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--mode sweep|pcg|analysis] [--form ...] [--workload c1|c0]
+                    [--mode sweep|pcg|analysis|serve|mixed] [--form ...]
+                    [--workload c1|c0|c2]
 
 Workloads (synthetic, seeded; specfem3d_geotech_amd.workloads):
   c1  144,000-element two-layer 2H:1V slope with +-2% interior jitter
       (3,489,480 equations).
   c0  1,152-element homogeneous slope (the reference's CPU-runnable case).
+  c2  400,000-element two-layer slope, face angle 31.4 deg (9,687,800
+      equations), the mixed-precision configuration.
 
 Modes:
   sweep (default)  the batched mixed-precision form on the c1 mesh: 64
@@ -27,6 +30,10 @@ Modes:
             8 SRFs in flight), 30% staged box-pit excavations (3-6 stages);
             admission by HBM footprint, concurrent requests on their own
             streams.  value = analyses/hour, with p50/p99 latency.
+  mixed     the mixed-precision form (FP32 stiffness and inner CG, FP64
+            refinement) against the FP64 drop-in on the same mesh (default
+            --workload c2 settings: the 400,000-element slope), SRF by SRF
+            (--srf list): displacement error and plastic-flag differences.
   analysis  complete FOS analyses: --form fp64 runs the reference's sweep
             with the FP64 drop-in, one analysis after another; --form batched
             runs --analyses N form-A analyses through the continuous-batching
@@ -64,10 +71,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--mode", default="sweep",
-                    choices=["sweep", "pcg", "analysis", "serve"])
+                    choices=["sweep", "pcg", "analysis", "serve", "mixed"])
     ap.add_argument("--form", default=None,
                     choices=list(FORMS) + ["batched"])
-    ap.add_argument("--workload", default="c1", choices=["c1", "c0"])
+    ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2"])
+    ap.add_argument("--srf", default=None,
+                    help="mixed mode: comma-separated SRF list")
     ap.add_argument("--trials", type=int, default=64)
     ap.add_argument("--analyses", type=int, default=None)
     ap.add_argument("--rate", type=float, default=None,
@@ -402,6 +411,71 @@ def mode_analysis(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
     return out
 
 
+def mode_mixed(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
+    """The mixed-precision form against the plain FP64 drop-in, SRF by SRF on
+    one mesh: each form runs the same SRF list (FP64: packed FP64 operator,
+    one SRF after another; mixed: FP32 stiffness and inner CG with FP64
+    refinement, up to 8 SRFs at once).  Per SRF: both summary rows, the
+    relative L2 difference of the displacement, and the number of GLL
+    points whose plastic flag differs."""
+    import numpy as np
+    from specfem3d_geotech_amd.batch import slope3d_mixed
+    from specfem3d_geotech_amd.slope import slope3d
+    from specfem3d_geotech_amd.solver import K_F64, KFMT_PACKED
+    srf = [float(v) for v in a.srf.split(",")] if a.srf else [1.0, 1.05, 1.1, 1.15]
+    ctl = dict(wl.control, srf=srf)
+    ncol = 1
+    while ncol < min(len(srf), 8):
+        ncol *= 2
+    keep64, keepm = {}, {}
+
+    def keep(store, tag):
+        def f(row, x, flags):
+            store[row["srf"]] = (row, x, flags)
+            log(f"  {tag} row", json.dumps(row))
+        return f
+    t0 = time.perf_counter()
+    r64 = slope3d(ctx, mesh, spec=spec, kprec=K_F64, kfmt=KFMT_PACKED,
+                  on_srf=keep(keep64, "fp64"),
+                  iter_progress=lambda nl, cg: nl % 10 == 0 and log(
+                      f"    fp64 plastic iteration {nl}, cg {cg}"), **ctl)
+    t64 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rm = slope3d_mixed(ctx, mesh, spec=spec, ncol=ncol, on_srf=keep(keepm, "mixed"),
+                       progress=lambda st: log(f"    mixed {st['batched_steps']} steps"),
+                       **ctl)
+    tm = time.perf_counter() - t0
+    per = []
+    for s in srf:
+        if s not in keep64 or s not in keepm:
+            continue
+        (ra, xa, fa), (rb, xb, fb) = keep64[s], keepm[s]
+        per.append(dict(
+            srf=s, fp64=ra, mixed=rb,
+            x_rel_l2=float(np.linalg.norm(xb - xa) / max(np.linalg.norm(xa), 1e-300)),
+            uxmax_rel=abs(rb["uxmax"] - ra["uxmax"]) / ra["uxmax"],
+            nl_tot_equal=ra["nl_tot"] == rb["nl_tot"],
+            plastic_flags_differing=int(np.count_nonzero(fa != fb))))
+    cg64 = sum(r["cg_tot"] for r in r64.rows)
+    cgm = sum(r["cg_tot"] for r in rm.rows)
+    E, N = spec.nelmt, spec.neq
+    return dict(
+        metric=METRIC, value=N * cgm / tm, unit="DOF-iterations/s", n_gpus=1,
+        steps=cgm, warmup=0, ms_per_step=tm / max(cgm, 1) * 1e3, higher_is_better=True,
+        scaling="weak", vs_baseline=None,
+        dtype="f32 stiffness + inner CG, f64 refinement/dots/x/yield", data=DATA,
+        config=dict(workload=wl.description, nelmt=E, neq=N, mode="mixed", srf=srf,
+                    srf_in_flight=ncol, cg_tol=ctl["cg_tol"], inner_tol=1e-5,
+                    nl_tol=ctl["nl_tol"], nl_maxiter=ctl["nl_maxiter"],
+                    host_prep_s=round(prep_s, 2)),
+        fp64=dict(seconds=round(t64, 2), cg_iterations=cg64,
+                  dof_iterations_per_s=N * cg64 / t64, bracket=r64.fos_bracket),
+        mixed=dict(seconds=round(tm, 2), cg_iterations=cgm, bracket=rm.fos_bracket,
+                   engine=rm.extra.get("stats")),
+        per_srf=per,
+        max_x_rel_l2=max((p["x_rel_l2"] for p in per), default=None))
+
+
 def mode_serve(a):
     from specfem3d_geotech_amd.serving import Server, make_requests
     reqs = make_requests(a.requests, rate=a.rate or 4.0,
@@ -449,11 +523,13 @@ def main():
             res = mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs)
         elif a.mode == "pcg":
             res = mode_pcg(a, ctx, wl, mesh, spec, prep_s, copy_gbs)
+        elif a.mode == "mixed":
+            res = mode_mixed(a, ctx, wl, mesh, spec, prep_s, copy_gbs)
         else:
             res = mode_analysis(a, ctx, wl, mesh, spec, prep_s, copy_gbs)
     finally:
         ctx.close()
-    if not a.no_cpu_baseline and a.mode != "analysis":
+    if not a.no_cpu_baseline and a.mode not in ("analysis", "mixed"):
         res["cpu_baseline"] = cpu_baseline(spec, mesh, a.cpu_sample_iters)
     print(json.dumps(res), flush=True)
 

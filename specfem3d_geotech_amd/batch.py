@@ -157,7 +157,8 @@ class PCGNotConverged(RuntimeError):
 def run_analyses(engine, analyses: List[Analysis], *, ym, nu, cg_tol: float,
                  cg_maxiter: int, nl_tol: float, nl_maxiter: int,
                  inner_tol: float = 1e-5, srf_parallel: int = 1,
-                 keep_state: bool = False, arrivals: Optional[List[float]] = None,
+                 keep_state: bool = False, on_trial=None,
+                 arrivals: Optional[List[float]] = None,
                  clock=time.perf_counter, on_done=None, poll_steps: int = 64,
                  progress=None, progress_steps: int = 2000):
     """Runs every analysis to its FOS bracket, keeping the engine's columns
@@ -278,6 +279,8 @@ def run_analyses(engine, analyses: List[Analysis], *, ym, nu, cg_tol: float,
                              fmax=float(r["fmax"][j]))
             if keep_state:
                 a.flags_last, a.x_last = flags, x
+            if on_trial:
+                on_trial(a, k, x, flags)
             if r["nl_tot"][j] >= nl_maxiter and (a.fail_idx is None or k < a.fail_idx):
                 a.fail_idx = k
             engine.release(j)
@@ -298,11 +301,14 @@ def device_mesh(ctx: Context, mesh: HexMesh, spec: Optional[SpecMesh] = None):
 
 def slope3d_mixed(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
                   nl_tol: float, nl_maxiter: int, srf: Sequence[float],
-                  inner_tol: float = 1e-5, ncol: int = 1, spec=None) -> SlopeResult:
+                  inner_tol: float = 1e-5, ncol: int = 1, spec=None, on_srf=None,
+                  progress=None) -> SlopeResult:
     """The reference's SRF sweep (src/slope.F90:244-443) in the mixed-precision
     form: FP32 stiffness and inner CG, FP64 refinement to cg_tol.  With
     ncol > 1 up to ncol SRFs of the sweep run at once; the rows and the
-    bracket are those of the sequential sweep."""
+    bracket are those of the sequential sweep.  `on_srf` is called with
+    (row, x, plastic flags) as each SRF trial ends (in completion order),
+    `progress` with the engine statistics now and then."""
     spec = spec or hex2spec(mesh)
     _, ym, nu, phi, coh, psi = mesh.material_arrays()
     dm, ext = device_mesh(ctx, mesh, spec)
@@ -311,7 +317,9 @@ def slope3d_mixed(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int
     try:
         run_analyses(eng, [a], ym=ym, nu=nu, cg_tol=cg_tol, cg_maxiter=cg_maxiter,
                      nl_tol=nl_tol, nl_maxiter=nl_maxiter, inner_tol=inner_tol,
-                     srf_parallel=ncol, keep_state=True)
+                     srf_parallel=ncol, keep_state=True, progress=progress,
+                     on_trial=(lambda an, k, x, f: on_srf(an.rows[k], x, f))
+                     if on_srf else None)
         stats = eng.stats()
     finally:
         eng.close()

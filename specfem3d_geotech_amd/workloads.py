@@ -5,7 +5,10 @@ c0  1,152 elements (24x4x12), homogeneous 2H:1V slope, c=10 kPa, phi=20 deg,
 c1  144,000 elements (120x20x60), same geometry, two layers split at
     z=10 m (upper c=8 kPa, phi=25 deg; lower c=20 kPa, phi=30 deg), interior
     nodes jittered by U[-0.02,0.02] x local spacing (seed 1).
-Control of both: SRF 1.00:0.05:2.00, cg_tol 1e-8, cg_maxiter 5000,
+c2  400,000 elements (200x20x100), the layers and jitter of c1, slope
+    angle drawn from U[26.6, 45] deg (numpy default_rng(2): 31.41 deg), the
+    mixed-precision configuration.
+Control of all: SRF 1.00:0.05:2.00, cg_tol 1e-8, cg_maxiter 5000,
 nl_tol 5e-4, nl_maxiter 300.
 """
 from __future__ import annotations
@@ -44,10 +47,24 @@ def _c1():
         layer_z=[10.0], jitter=0.02, seed=1)
 
 
+def c2_angle() -> float:
+    import numpy as np
+    return float(np.random.default_rng(2).uniform(26.6, 45.0))
+
+
+def _c2():
+    return slope_mesh(200, 20, 100, angle_deg=c2_angle(), materials=[
+        Material(gam=20.0, ym=1e5, nu=0.3, phi=25.0, coh=8.0, psi=0.0),
+        Material(gam=20.0, ym=1e5, nu=0.3, phi=30.0, coh=20.0, psi=0.0)],
+        layer_z=[10.0], jitter=0.02, seed=1)
+
+
 WORKLOADS = {
     "c0": Workload("c0", "slope 24x4x12 (1152 el), homogeneous, FP64", _c0),
     "c1": Workload("c1", "slope 120x20x60 (144000 el, 3489480 eq), two layers, "
                          "2% jitter seed 1", _c1),
+    "c2": Workload("c2", "slope 200x20x100 (400000 el), face angle 31.41 deg "
+                         "(U[26.6,45] seed 2), two layers, 2% jitter seed 1", _c2),
 }
 
 
