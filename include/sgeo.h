@@ -44,6 +44,7 @@ typedef struct sgeo_ctx sgeo_ctx;
 typedef struct sgeo_mesh sgeo_mesh;
 typedef struct sgeo_slope sgeo_slope;
 typedef struct sgeo_batch sgeo_batch;
+typedef struct sgeo_spec sgeo_spec;
 
 /* ---- context ---------------------------------------------------------- */
 int sgeo_init(int device, sgeo_ctx **ctx);
@@ -92,6 +93,26 @@ int sgeo_pcg_solver(sgeo_ctx *ctx, int32_t neq, int32_t nelmt,
                     const double *dprecon, const int32_t *gdof_elmt,
                     double cg_tol, int32_t cg_maxiter, int32_t *cg_iter,
                     int32_t *errcode, char *errtag);
+
+/* ---- ingest: GLL numbering on the device ---------------------------------
+ * Replaces hex2spec + get_global + get_global_indirect_addressing
+ * (src/mesh_spec.f90:17-386) and the equation numbering of apply_bc
+ * (src/apply_bc.f90:36-137) for a hex8 mesh.
+ *   coord8(3, nnode8), conn(8, nelmt) 1-based EXODUS corner order (the
+ *   reference's g_coord / g_num before hex2spec); bc(3, nbc) rows (element
+ *   1-based, face 1..6, component 0..2 = ux, uy, uz) of the face conditions.
+ * Points are merged with the reference's tolerance 1e-10*|xmax - xmin|
+ * (x over the hex8 nodes); nodes are numbered by first appearance, free
+ * components node-major (component fastest), 1-based, 0 = fixed.
+ * sgeo_spec_fetch copies any of (NULL to skip): g_num(27, nelmt) 0-based
+ * node ids, gdof(3, nnode), gdof_elmt(81, nelmt), g_coord(3, nnode),
+ * corners(3, 8, nelmt). */
+int sgeo_hex2spec(sgeo_ctx *ctx, int32_t nelmt, int32_t nnode8, const double *coord8,
+                  const int32_t *conn, int32_t nbc, const int32_t *bc, sgeo_spec **spec,
+                  int32_t *nnode, int32_t *neq);
+int sgeo_spec_fetch(sgeo_spec *spec, int32_t *g_num, int32_t *gdof, int32_t *gdof_elmt,
+                    double *g_coord, double *corners);
+int sgeo_spec_destroy(sgeo_spec *spec);
 
 /* ---- resident mesh ------------------------------------------------------
  * A mesh keeps gdof_elmt, corner coordinates, material ids and the element

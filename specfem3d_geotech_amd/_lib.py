@@ -70,6 +70,10 @@ SIGNATURES = {
                                      C.c_int32, C.c_int64, i_p, i_p, i_p, d_p, i_p,
                                      i_p]),
     "sgeo_batch_download": (C.c_int, [vp, C.c_int32, d_p, u8_p]),
+    "sgeo_hex2spec": (C.c_int, [vp, C.c_int32, C.c_int32, vp, vp, C.c_int32, vp,
+                                C.POINTER(vp), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "sgeo_spec_fetch": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+    "sgeo_spec_destroy": (C.c_int, [vp]),
     "sgeo_batch_set_graphs": (C.c_int, [vp, C.c_int]),
     "sgeo_batch_autotune": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_int32),
                                       C.POINTER(C.c_int32), C.POINTER(C.c_double),

@@ -30,7 +30,7 @@ from .mesh import HexMesh
 from .plastic import dt_viscoplas, strength_reduction
 from .slope import SlopeResult, uxmax_umax
 from .solver import K_F64, Context, DeviceMesh
-from .spec import SpecMesh, hex2spec
+from .spec import SpecMesh, hex2spec_device
 
 
 class BatchEngine:
@@ -292,7 +292,7 @@ def run_analyses(engine, analyses: List[Analysis], *, ym, nu, cg_tol: float,
 
 def device_mesh(ctx: Context, mesh: HexMesh, spec: Optional[SpecMesh] = None):
     """Resident mesh with its elastic stiffness; returns (DeviceMesh, extload)."""
-    spec = spec or hex2spec(mesh)
+    spec = spec or hex2spec_device(ctx, mesh)
     gam, ym, nu, *_ = mesh.material_arrays()
     dm = DeviceMesh(ctx, spec, len(gam))
     ext, _ = dm.build_stiffness(gam, ym, nu, kprec=K_F64)
@@ -309,7 +309,7 @@ def slope3d_mixed(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int
     bracket are those of the sequential sweep.  `on_srf` is called with
     (row, x, plastic flags) as each SRF trial ends (in completion order),
     `progress` with the engine statistics now and then."""
-    spec = spec or hex2spec(mesh)
+    spec = spec or hex2spec_device(ctx, mesh)
     _, ym, nu, phi, coh, psi = mesh.material_arrays()
     dm, ext = device_mesh(ctx, mesh, spec)
     eng = BatchEngine(dm, ext, ncol)

@@ -17,7 +17,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsgeo.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["sgeo_kernels.hip", "sgeo_pcg.hip", "sgeo_capi.cpp", "sgeo_batch.hip"]
+SOURCES = ["sgeo_kernels.hip", "sgeo_pcg.hip", "sgeo_capi.cpp", "sgeo_batch.hip",
+           "sgeo_ingest.hip"]
 
 # -packed-fp32-ops: no packed FP32 VALU instructions in any kernel.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -468,6 +468,96 @@ static void ebe_elem(sgeo_mesh* m, const double* p, double* Y, int skip_mode) {
   }
 }
 
+// --------------------------------------------------------------------------
+// ingest: GLL numbering on the device (src/mesh_spec.f90:17-386,
+// src/apply_bc.f90:36-137)
+// --------------------------------------------------------------------------
+struct sgeo_spec {
+  sgeo_ctx* ctx = nullptr;
+  SpecDev d;
+};
+
+extern "C" int sgeo_hex2spec(sgeo_ctx* c, int32_t nelmt, int32_t nnode8,
+                             const double* coord8, const int32_t* conn, int32_t nbc,
+                             const int32_t* bc, sgeo_spec** out, int32_t* nnode,
+                             int32_t* neq) {
+  if (!c || !coord8 || !conn || !out || nelmt <= 0 || nnode8 <= 0 || nbc < 0 ||
+      (nbc > 0 && !bc))
+    return fail(SGEO_EARG, "sgeo_hex2spec: bad arguments");
+  if ((int64_t)nelmt * 81 > INT32_MAX)
+    return fail(SGEO_EARG, "sgeo_hex2spec: nelmt*81 exceeds int32");
+  for (int64_t q = 0; q < (int64_t)nelmt * 8; q++)
+    if (conn[q] < 1 || conn[q] > nnode8)
+      return fail(SGEO_EARG, "sgeo_hex2spec: conn[%lld]=%d outside 1..%d", (long long)q,
+                  conn[q], nnode8);
+  for (int64_t r = 0; r < nbc; r++)
+    if (bc[r * 3] < 1 || bc[r * 3] > nelmt || bc[r * 3 + 1] < 1 || bc[r * 3 + 1] > 6 ||
+        bc[r * 3 + 2] < 0 || bc[r * 3 + 2] > 2)
+      return fail(SGEO_EARG, "sgeo_hex2spec: boundary row %lld (%d, %d, %d) invalid",
+                  (long long)r, bc[r * 3], bc[r * 3 + 1], bc[r * 3 + 2]);
+  // tolerance of get_global: 1e-10 * |xmax - xmin| over the hex8 nodes
+  // (src/mesh_spec.f90:178)
+  double xmin = coord8[0], xmax = coord8[0];
+  for (int64_t g = 1; g < nnode8; g++) {
+    xmin = std::min(xmin, coord8[g * 3]);
+    xmax = std::max(xmax, coord8[g * 3]);
+  }
+  const double tol = 1.e-10 * std::fabs(xmax - xmin);
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  double* d_coord = nullptr;
+  int32_t *d_conn = nullptr, *d_bc = nullptr;
+  TRY(dalloc(&d_coord, (size_t)nnode8 * 3));
+  TRY(dalloc(&d_conn, (size_t)nelmt * 8));
+  TRY(dalloc(&d_bc, (size_t)std::max(nbc, 1) * 3));
+  HIPCHK(hipMemcpyAsync(d_coord, coord8, (size_t)nnode8 * 24, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d_conn, conn, (size_t)nelmt * 32, hipMemcpyHostToDevice, s));
+  if (nbc > 0)
+    HIPCHK(hipMemcpyAsync(d_bc, bc, (size_t)nbc * 12, hipMemcpyHostToDevice, s));
+  SpecDev d;
+  const hipError_t e = hex2spec_device(s, nelmt, d_coord, d_conn, nbc, d_bc, tol, &d);
+  hipFree(d_coord);
+  hipFree(d_conn);
+  hipFree(d_bc);
+  if (e != hipSuccess)
+    return fail(SGEO_EHIP, "sgeo_hex2spec: %s", hipGetErrorString(e));
+  sgeo_spec* sp = new sgeo_spec();
+  sp->ctx = c;
+  sp->d = d;
+  *out = sp;
+  if (nnode) *nnode = d.nnode;
+  if (neq) *neq = d.neq;
+  return 0;
+}
+
+extern "C" int sgeo_spec_fetch(sgeo_spec* sp, int32_t* g_num, int32_t* gdof,
+                               int32_t* gdof_elmt, double* g_coord, double* corners) {
+  if (!sp) return fail(SGEO_EARG, "null spec");
+  HIPCHK(hipSetDevice(sp->ctx->device));
+  hipStream_t s = sp->ctx->stream;
+  const SpecDev& d = sp->d;
+  const size_t ne = (size_t)d.nelmt, nn = (size_t)d.nnode;
+  if (g_num) HIPCHK(hipMemcpyAsync(g_num, d.g_num, ne * 27 * 4, hipMemcpyDeviceToHost, s));
+  if (gdof) HIPCHK(hipMemcpyAsync(gdof, d.gdof, nn * 3 * 4, hipMemcpyDeviceToHost, s));
+  if (gdof_elmt)
+    HIPCHK(hipMemcpyAsync(gdof_elmt, d.gdof_elmt, ne * 81 * 4, hipMemcpyDeviceToHost, s));
+  if (g_coord)
+    HIPCHK(hipMemcpyAsync(g_coord, d.g_coord, nn * 3 * 8, hipMemcpyDeviceToHost, s));
+  if (corners)
+    HIPCHK(hipMemcpyAsync(corners, d.corners, ne * 24 * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int sgeo_spec_destroy(sgeo_spec* sp) {
+  if (!sp) return 0;
+  hipSetDevice(sp->ctx->device);
+  hipStreamSynchronize(sp->ctx->stream);
+  spec_device_free(&sp->d);
+  delete sp;
+  return 0;
+}
+
 extern "C" int sgeo_calibrate_copy(sgeo_ctx* c, size_t bytes, int reps,
                                    double* gbs) {
   if (!c || !gbs || bytes < 1024 || reps < 1)

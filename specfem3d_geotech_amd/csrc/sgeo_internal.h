@@ -131,6 +131,20 @@ struct BCol {  // per-column scalars, device resident
   int32_t nrep;         // FP64 residuals of this solve
 };
 
+// GLL numbering of a hex8 mesh, built on the device (sgeo_ingest.hip)
+struct SpecDev {
+  int32_t nelmt = 0, nnode = 0, neq = 0;
+  int32_t* g_num = nullptr;      // [e][27] node ids, 0-based
+  int32_t* gdof = nullptr;       // [node][3] equations, 1-based, 0 = fixed
+  int32_t* gdof_elmt = nullptr;  // [e][81]
+  double* g_coord = nullptr;     // [node][3]
+  double* corners = nullptr;     // [e][8][3], EXODUS corner order
+};
+hipError_t hex2spec_device(hipStream_t s, int32_t nelmt, const double* d_coord8,
+                           const int32_t* d_conn, int32_t nbc, const int32_t* d_bc,
+                           double tol, SpecDev* out);
+void spec_device_free(SpecDev* d);
+
 // launch geometry of the persistent MFMA product (result bits do not
 // depend on it; chosen by sgeo_batch_autotune)
 struct BmvGeom {

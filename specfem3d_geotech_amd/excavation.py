@@ -22,7 +22,7 @@ from .mesh import HexMesh
 from .plastic import dt_viscoplas, strength_reduction
 from .slope import DeviceSlope, PCGNotConverged
 from .solver import K_F64, KFMT_FULL, Context, DeviceMesh
-from .spec import SpecMesh, hex2spec
+from .spec import SpecMesh, hex2spec_device
 
 
 def intact_void(mat_id: np.ndarray, ismat: np.ndarray, removed: Sequence[int]):
@@ -76,7 +76,7 @@ def excavation3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
     inside its SRF loop); the first SRF is used."""
     if s0_type != 0:
         raise NotImplementedError("initial stress type 1 (overburden) is not built")
-    spec = spec or hex2spec(mesh)
+    spec = spec or hex2spec_device(ctx, mesh)
     gam, ym, nu, phi, coh, psi = mesh.material_arrays()
     nmat, E = len(gam), spec.nelmt
     g_num = np.asarray(spec.g_num)

@@ -18,7 +18,7 @@ from ._lib import check, f64, i32, ptr
 from .mesh import HexMesh
 from .plastic import dt_viscoplas, strength_reduction
 from .solver import K_F64, Context, DeviceMesh
-from .spec import SpecMesh, hex2spec
+from .spec import SpecMesh, hex2spec_device
 
 
 class DeviceSlope:
@@ -120,7 +120,7 @@ def slope3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
     given, is called with each summary row, `iter_progress` with (nl_iter,
     cg_iter) after each plastic iteration, `on_srf` with (row, x, plastic
     flags) at the end of each SRF."""
-    spec = spec or hex2spec(mesh)
+    spec = spec or hex2spec_device(ctx, mesh)
     gam, ym, nu, phi, coh, psi = mesh.material_arrays()
     dm = DeviceMesh(ctx, spec, len(gam))
     ext, _ = dm.build_stiffness(gam, ym, nu, kprec=K_F64)

@@ -170,3 +170,40 @@ def structured_colours(mesh: HexMesh) -> np.ndarray:
     e = np.arange(nx * ny * nz)
     i, j, k = e % nx, (e // nx) % ny, e // (nx * ny)
     return ((i % 2) + 2 * (j % 2) + 4 * (k % 2)).astype(np.int32)
+
+
+def hex2spec_device(ctx, mesh: HexMesh) -> SpecMesh:
+    """`hex2spec` on the GPU (sgeo_hex2spec): the same numbering, built by
+    radix sorts and scans on the device instead of host sorts."""
+    import ctypes as C
+
+    from . import _lib as L
+    coord = np.ascontiguousarray(mesh.coord, dtype=np.float64)
+    conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+    nel = conn.shape[0]
+    rows = []
+    for comp, name in enumerate(("ux", "uy", "uz")):
+        pairs = np.asarray(mesh.bc[name], dtype=np.int32).reshape(-1, 2)
+        rows.append(np.column_stack([pairs, np.full(len(pairs), comp, np.int32)]))
+    bc = np.ascontiguousarray(np.concatenate(rows), dtype=np.int32)
+    h = C.c_void_p()
+    nnode, neq = C.c_int32(), C.c_int32()
+    L.check(L.lib().sgeo_hex2spec(ctx.h, nel, coord.shape[0], L.ptr(coord),
+                                  L.ptr(conn, C.c_int32), len(bc), L.ptr(bc, C.c_int32),
+                                  C.byref(h), C.byref(nnode), C.byref(neq)),
+            "sgeo_hex2spec")
+    try:
+        nn = nnode.value
+        g_num = np.empty((nel, NGLL), np.int32)
+        gdof = np.empty((nn, 3), np.int32)
+        gdof_elmt = np.empty((nel, NEDOF), np.int32)
+        g_coord = np.empty((nn, 3))
+        corners = np.empty((nel, 8, 3))
+        L.check(L.lib().sgeo_spec_fetch(h, L.ptr(g_num, C.c_int32), L.ptr(gdof, C.c_int32),
+                                        L.ptr(gdof_elmt, C.c_int32), L.ptr(g_coord),
+                                        L.ptr(corners)), "sgeo_spec_fetch")
+    finally:
+        L.lib().sgeo_spec_destroy(h)
+    return SpecMesh(nelmt=nel, nnode=nn, neq=neq.value, g_num=g_num.astype(np.int64),
+                    g_coord=g_coord, gdof=gdof.astype(np.int64), gdof_elmt=gdof_elmt,
+                    corners=corners, mat_id=np.asarray(mesh.mat_id, dtype=np.int32))
