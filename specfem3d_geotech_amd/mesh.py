@@ -220,9 +220,10 @@ def write_reference_input(mesh: HexMesh, out_dir: str, head: str, *,
                           nl_maxiter: int, srf: Sequence[float],
                           out_path: Optional[str] = None,
                           extra_control: str = "",
-                          extra_lines: str = "") -> str:
+                          extra_lines: str = "", save_disp: bool = False) -> str:
     """Write `mesh` plus a `.sem` control file readable by the reference's
-    `semgeotech` (serial).  Returns the `.sem` path."""
+    `semgeotech` (serial); `save_disp` asks it for its EnSight displacement
+    output.  Returns the `.sem` path."""
     os.makedirs(out_dir, exist_ok=True)
     out_path = out_path or os.path.join(out_dir, "output")
     os.makedirs(out_path, exist_ok=True)
@@ -268,7 +269,7 @@ def write_reference_input(mesh: HexMesh, out_dir: str, head: str, *,
                  f"srf={srf_s}{extra_control}\n")
         if extra_lines:
             fh.write(extra_lines)
-        fh.write("save: disp=0\n")
+        fh.write(f"save: disp={1 if save_disp else 0}\n")
     return sem
 
 

@@ -33,12 +33,12 @@ FN_PROBE = os.path.join(ROOT, "oracle", "_ref", "fn_probe")
 REF_INPUT = "/root/reference/input"
 
 
-def run_tap(sem: str, workdir: str, mode: str = "full") -> None:
+def run_tap(sem: str, workdir: str, mode: str = "full", extra_env=None) -> None:
     for f in ("pcg_static.bin", "pcg_calls.bin", "pcg_times.txt", "mohcouf.bin"):
         p = os.path.join(workdir, f)
         if os.path.exists(p):
             os.remove(p)
-    env = dict(os.environ, SGEO_TAP_DIR=workdir, SGEO_TAP_MODE=mode)
+    env = dict(os.environ, SGEO_TAP_DIR=workdir, SGEO_TAP_MODE=mode, **(extra_env or {}))
     # flang places the reference's automatic arrays on the stack
     r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {REF_TAP} {sem}"],
                        cwd=workdir, env=env, capture_output=True, text=True)
@@ -184,6 +184,47 @@ def excavation_example_case(workdir=None):
     print("excavation_3d", summ.tolist())
 
 
+def excavation_example_spread():
+    """The same example run twice more with libm results moved by one ulp on
+    about 1/64 of the arguments (tests/golden/ulp_shim.c preloaded): asin
+    only, then asin, sin, cos, tan and sincos.  The stage rows of these runs
+    measure how far the reference's own results move under last-bit changes
+    of its transcendental functions; they are stored beside the unperturbed
+    summary as spread_rows[variant, stage, (stage, cg_tot, nl_tot, uxmax,
+    umax, fmax)]."""
+    work = tempfile.mkdtemp(prefix="golden_ex3d_spread_")
+    shim = os.path.join(work, "ulp_shim.so")
+    subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-o", shim,
+                    os.path.join(HERE, "ulp_shim.c"), "-ldl", "-lm"], check=True)
+    funcs = ["asin", "all"]
+    jobs = []
+    for v in funcs:  # both variants at once, each in its own directory
+        wd = os.path.join(work, v)
+        os.makedirs(os.path.join(wd, "input"))
+        os.makedirs(os.path.join(wd, "output"))
+        for f in os.listdir(REF_INPUT):
+            if f.startswith("excavation_3d") and "single" not in f and \
+                    "elastic" not in f and not f.endswith("psem"):
+                shutil.copy(os.path.join(REF_INPUT, f), os.path.join(wd, "input"))
+        env = dict(os.environ, SGEO_TAP_DIR=wd, SGEO_TAP_MODE="time",
+                   LD_PRELOAD=shim, SGEO_ULP_FUNCS=v)
+        sem = os.path.join(wd, "input", "excavation_3d.sem")
+        jobs.append(subprocess.Popen(
+            ["bash", "-c", f"ulimit -s unlimited; exec {REF_TAP} {sem}"], cwd=wd,
+            env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL))
+    for j in jobs:
+        j.wait()
+    rows = np.stack([parse_stage_summary(find_summary(os.path.join(work, v)))
+                     for v in funcs])
+    path = os.path.join(HERE, "excavation_3d.npz")
+    with np.load(path) as z:
+        out = {k: z[k] for k in z.files}
+    out.update(spread_rows=rows, spread_funcs=np.array(funcs))
+    np.savez_compressed(path, **out)
+    shutil.rmtree(work)
+    print("excavation_3d spread", rows.tolist())
+
+
 def mesh_arrays(m):
     gam, ym, nu, phi, coh, psi = m.material_arrays()
     return dict(coord=m.coord, conn=m.conn, mat_id=m.mat_id,
@@ -310,12 +351,14 @@ def main():
     fn_case()
     excavation_cases()
     excavation_example_case()
+    excavation_example_spread()
 
 
 if __name__ == "__main__":
     if len(sys.argv) > 1:  # single cases: excavation | excavation_example [workdir]
-        {"excavation": excavation_cases,
+        {"excavation": excavation_cases,       # | excavation_spread
          "excavation_example": lambda: excavation_example_case(
-             sys.argv[2] if len(sys.argv) > 2 else None)}[sys.argv[1]]()
+             sys.argv[2] if len(sys.argv) > 2 else None),
+         "excavation_spread": excavation_example_spread}[sys.argv[1]]()
     else:
         main()

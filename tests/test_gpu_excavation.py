@@ -8,6 +8,9 @@ the reference's own outputs and the oracle.
   yielding in stages 2-3) with reference-order reductions: the stage
   summary rows of the reference -- cg_tot and nl_tot exactly, displacements
   within 1e-10.
+* The reference's own example input/excavation_3d: the elastic stage bit for
+  bit, the yielding stages within twice the reference's own spread under
+  one-ulp changes of its libm results.
 """
 import numpy as np
 import pytest
@@ -103,18 +106,33 @@ def test_excavation_pit_tree_reductions(ctx):
 def test_reference_example_excavation_3d(ctx_seq):
     """The reference's own example input/excavation_3d: 1,700 elements,
     nu = 0.49, three stages removing material ids 2, 3, 4, ten load
-    increments each (yielding in stage 3).  Stage summary rows of the
-    compiled reference (reference-order reductions)."""
+    increments each; yielding starts in stage 2.
+
+    Stage 1 stays elastic, so it passes through no transcendental function:
+    every PCG call and the stage row match the compiled reference bit for
+    bit.  Yield checks evaluate asin, sin, cos and tan, and the device math
+    library differs from the host libm in the last bit on some arguments.
+    This example sits on the yield surface (nu = 0.49, in-situ stress), and
+    the reference moves by itself when its own libm results move by one
+    ulp: spread_rows holds its stage rows with asin, or asin/sin/cos/tan,
+    moved by one ulp on 1/64 of the arguments.  A value that does not move
+    there (stage 1, nl_tot) must match exactly; the others must lie within
+    twice the reference's own spread."""
     z = golden("excavation_3d")
     mesh = mesh_from_golden(z)
     ctl = _control(z)
     res = excavation3d(ctx_seq, mesh, **ctl)
     ref = z["summary"]
-    assert len(res.rows) == ref.shape[0]
-    for r, rr in zip(res.rows, ref):
-        print(r, rr.tolist())
-        assert r["stage"] == rr[0] and r["nl_tot"] == rr[2]
-        assert r["cg_tot"] == rr[1]
-        assert abs(r["uxmax"] - rr[3]) <= 1e-10 * rr[3]
-        assert abs(r["umax"] - rr[4]) <= 1e-10 * rr[4]
-    assert res.cg_log == list(z["cg_log"])
+    spread = z["spread_rows"]
+    assert len(res.rows) == ref.shape[0] == spread.shape[1]
+    keys = ["stage", "cg_tot", "nl_tot", "uxmax", "umax", "fmax"]
+    for s, (r, rr) in enumerate(zip(res.rows, ref)):
+        print(r, rr.tolist(), spread[:, s].tolist())
+        for k in range(6):
+            sp = float(np.abs(spread[:, s, k] - rr[k]).max())
+            if sp == 0.0:
+                assert r[keys[k]] == rr[k], (s, keys[k])
+            else:
+                assert abs(r[keys[k]] - rr[k]) <= 2.0 * sp, (s, keys[k], sp)
+    n_elastic = 1 + int(ref[0, 2])  # the gravity solve and stage 1's calls
+    assert res.cg_log[:n_elastic] == [int(v) for v in z["cg_log"][:n_elastic]]
