@@ -299,16 +299,30 @@ def device_mesh(ctx: Context, mesh: HexMesh, spec: Optional[SpecMesh] = None):
     return dm, ext
 
 
+def _trial_sink(on_srf, output):
+    if on_srf is None and output is None:
+        return None
+
+    def sink(an, k, x, flags):
+        if on_srf:
+            on_srf(an.rows[k], x, flags)
+        if output is not None:
+            output.srf_done(k + 1, an.rows[k], x)
+    return sink
+
+
 def slope3d_mixed(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
                   nl_tol: float, nl_maxiter: int, srf: Sequence[float],
                   inner_tol: float = 1e-5, ncol: int = 1, spec=None, on_srf=None,
-                  progress=None) -> SlopeResult:
+                  progress=None, output=None) -> SlopeResult:
     """The reference's SRF sweep (src/slope.F90:244-443) in the mixed-precision
     form: FP32 stiffness and inner CG, FP64 refinement to cg_tol.  With
     ncol > 1 up to ncol SRFs of the sweep run at once; the rows and the
     bracket are those of the sequential sweep.  `on_srf` is called with
     (row, x, plastic flags) as each SRF trial ends (in completion order),
-    `progress` with the engine statistics now and then."""
+    `progress` with the engine statistics now and then; `output`
+    (output.SlopeOutput) receives the rows and displacements for the
+    reference's result files."""
     spec = spec or hex2spec_device(ctx, mesh)
     _, ym, nu, phi, coh, psi = mesh.material_arrays()
     dm, ext = device_mesh(ctx, mesh, spec)
@@ -318,8 +332,7 @@ def slope3d_mixed(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int
         run_analyses(eng, [a], ym=ym, nu=nu, cg_tol=cg_tol, cg_maxiter=cg_maxiter,
                      nl_tol=nl_tol, nl_maxiter=nl_maxiter, inner_tol=inner_tol,
                      srf_parallel=ncol, keep_state=True, progress=progress,
-                     on_trial=(lambda an, k, x, f: on_srf(an.rows[k], x, f))
-                     if on_srf else None)
+                     on_trial=_trial_sink(on_srf, output))
         stats = eng.stats()
     finally:
         eng.close()

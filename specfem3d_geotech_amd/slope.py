@@ -112,14 +112,15 @@ def slope3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
             nl_tol: float, nl_maxiter: int, srf: Sequence[float],
             phinu: bool = False, kprec: int = K_F64, kfmt: int = 0,
             spec: Optional[SpecMesh] = None, progress=None,
-            iter_progress=None, on_srf=None) -> SlopeResult:
+            iter_progress=None, on_srf=None, output=None) -> SlopeResult:
     """Strength-reduction sweep of src/slope.F90:244-443 on the GPU.
 
     kprec/kfmt select the operator the PCG streams (FP64 full reproduces
     the reference's arithmetic; packed halves the bytes).  `progress`, if
     given, is called with each summary row, `iter_progress` with (nl_iter,
     cg_iter) after each plastic iteration, `on_srf` with (row, x, plastic
-    flags) at the end of each SRF."""
+    flags) at the end of each SRF.  `output` (output.SlopeOutput) receives
+    each SRF's row and displacement for the reference's result files."""
     spec = spec or hex2spec_device(ctx, mesh)
     gam, ym, nu, phi, coh, psi = mesh.material_arrays()
     dm = DeviceMesh(ctx, spec, len(gam))
@@ -165,6 +166,8 @@ def slope3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
                 progress(rows[-1])
             if on_srf:
                 on_srf(rows[-1], state["x"], state["flags"])
+            if output is not None:
+                output.srf_done(len(rows), rows[-1], state["x"])
             if nl_iter == nl_maxiter:  # src/slope.F90:441
                 bracket = (prev, float(s))
                 break

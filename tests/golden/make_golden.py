@@ -31,6 +31,8 @@ REF_TAP = os.path.join(ROOT, "oracle", "_ref", "semgeotech_tap")
 GLL_PROBE = os.path.join(ROOT, "oracle", "_ref", "gll_probe")
 FN_PROBE = os.path.join(ROOT, "oracle", "_ref", "fn_probe")
 REF_INPUT = "/root/reference/input"
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "semgeotech")
+LISTFMT_PROBE = os.path.join(ROOT, "oracle", "_ref", "listfmt_probe")
 
 
 def run_tap(sem: str, workdir: str, mode: str = "full", extra_env=None) -> None:
@@ -225,6 +227,80 @@ def excavation_example_spread():
     print("excavation_3d spread", rows.tolist())
 
 
+def list_format_case():
+    """The reference compiler's list-directed output (the summary file's
+    number format, src/slope.F90:217-219,415): doubles of every magnitude
+    and sign, values near the F/E switch, SRF-like and O(1) values, and
+    summary rows (srf, cg_tot, nl_tot, uxmax, umax, fmax) whose length makes
+    the runtime wrap its record."""
+    rng = np.random.default_rng(11)
+    special = [0.0, 1.0, -1.0, 0.5, 0.1, 0.099999, 0.0951, 0.0949, 1e-8, 5e-4, 1.2, 1.05,
+               2.0, 1e14, 9.99e14, 1e15, 1e16, 123456789012345678.0, 0.125, 659.0,
+               100000.0, 1e-300, 1.7976931348623157e308, 5e-324, 3.5001576573408467]
+    vals = np.concatenate([
+        np.array(special),
+        10.0 ** rng.uniform(-20, 20, 1500) * rng.choice([-1.0, 1.0], 1500),
+        np.round(rng.uniform(0.0, 3.0, 200), 2),
+        rng.uniform(0.0, 1.0, 300),
+        rng.uniform(0.09, 0.11, 200),
+        10.0 ** rng.uniform(13.5, 16.5, 100)])
+    m = 300
+    srf = np.array([round(v, int(d)) for v, d in zip(rng.uniform(0.5, 3.0, m),
+                                                      rng.integers(0, 3, m))])
+    cg = rng.integers(1, 400000, m).astype(np.int32)
+    nl = rng.integers(1, 5001, m).astype(np.int32)
+    ux = 10.0 ** rng.uniform(-6, 1, m)
+    um = ux * rng.uniform(1.0, 1.5, m)
+    fm = np.where(rng.uniform(size=m) < 0.2, 0.0, 10.0 ** rng.uniform(-4, 2, m))
+    work = tempfile.mkdtemp(prefix="golden_listfmt_")
+    path = os.path.join(work, "in.bin")
+    rec = np.zeros(m, dtype=[("srf", "<f8"), ("cg", "<i4"), ("nl", "<i4"), ("ux", "<f8"),
+                             ("um", "<f8"), ("fm", "<f8")])
+    rec["srf"], rec["cg"], rec["nl"], rec["ux"], rec["um"], rec["fm"] = srf, cg, nl, ux, um, fm
+    with open(path, "wb") as fh:
+        fh.write(np.int32(vals.size).tobytes())
+        fh.write(vals.astype("<f8").tobytes())
+        fh.write(np.int32(m).tobytes())
+        fh.write(rec.tobytes())
+    out = subprocess.run([LISTFMT_PROBE, path], capture_output=True, text=True,
+                         check=True).stdout
+    lines = out.split("\n")
+    strings = np.array(lines[:vals.size])
+    rows_text = "\n".join(lines[vals.size:])
+    np.savez_compressed(os.path.join(HERE, "list_format.npz"), values=vals, strings=strings,
+                        row_srf=srf, row_cg=cg, row_nl=nl, row_ux=ux, row_um=um, row_fm=fm,
+                        rows_text=np.array(rows_text))
+    shutil.rmtree(work)
+    print("list_format", vals.size, "values,", m, "rows")
+
+
+def output_case():
+    """The reference's own output files for a two-SRF sweep of the 4x2x3
+    slope with displacement output (save: disp=1): the EnSight Gold case and
+    geometry files of the hex8 mesh and of the GLL mesh, one displacement
+    file per SRF, and the summary file (src/semgeotech.F90:138-322,
+    src/postprocess.f90:128-192, src/visual.f90:17-231, src/slope.F90:214-220,
+    414-416)."""
+    mesh = slope_mesh(4, 2, 3)
+    work = tempfile.mkdtemp(prefix="golden_out_")
+    sem = write_reference_input(mesh, work, "slope_out", cg_tol=1e-8, cg_maxiter=5000,
+                                nl_tol=5e-4, nl_maxiter=300, srf=[1.0, 1.2], save_disp=True)
+    r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {REF_BIN} {sem}"],
+                       cwd=work, capture_output=True, text=True)
+    outdir = os.path.join(work, "output")
+    names = sorted(os.listdir(outdir))
+    if "slope_out_summary" not in names:
+        raise RuntimeError(f"reference run failed: {r.stdout[-2000:]} {r.stderr}")
+    out = dict(mesh_arrays(mesh))
+    out["file_names"] = np.array(names)
+    for f in names:
+        out["file:" + f] = np.frombuffer(open(os.path.join(outdir, f), "rb").read(),
+                                         dtype=np.uint8)
+    np.savez_compressed(os.path.join(HERE, "output_slope_4x2x3.npz"), **out)
+    shutil.rmtree(work)
+    print("output", names)
+
+
 def mesh_arrays(m):
     gam, ym, nu, phi, coh, psi = m.material_arrays()
     return dict(coord=m.coord, conn=m.conn, mat_id=m.mat_id,
@@ -352,6 +428,8 @@ def main():
     excavation_cases()
     excavation_example_case()
     excavation_example_spread()
+    list_format_case()
+    output_case()
 
 
 if __name__ == "__main__":
@@ -359,6 +437,7 @@ if __name__ == "__main__":
         {"excavation": excavation_cases,       # | excavation_spread
          "excavation_example": lambda: excavation_example_case(
              sys.argv[2] if len(sys.argv) > 2 else None),
-         "excavation_spread": excavation_example_spread}[sys.argv[1]]()
+         "excavation_spread": excavation_example_spread,
+         "list_format": list_format_case, "output": output_case}[sys.argv[1]]()
     else:
         main()

@@ -1,0 +1,100 @@
+"""The reference's result files written from the GPU sweeps.
+
+The FP64 sweep with reference-order reductions produces the reference's
+displacements bit for bit on converged SRFs, so every EnSight file of the
+two-SRF sweep of the 4x2x3 slope equals the compiled reference's byte for
+byte and the summary file equals it value for value.  The mixed-precision
+sweep (both SRFs at once) writes the same files in SRF order; its
+displacements agree with the reference's within the solver tolerance.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import golden
+from specfem3d_geotech_amd.batch import slope3d_mixed
+from specfem3d_geotech_amd.mesh import HexMesh, Material
+from specfem3d_geotech_amd.output import SlopeOutput
+from specfem3d_geotech_amd.slope import slope3d
+from specfem3d_geotech_amd.solver import REDUCE_SEQUENTIAL, Context
+from specfem3d_geotech_amd.spec import hex2spec_device
+
+pytestmark = pytest.mark.gpu
+
+CTL = dict(cg_tol=1e-8, cg_maxiter=5000, nl_tol=5e-4, nl_maxiter=300)
+SRF = [1.0, 1.2]
+
+
+def _mesh(z):
+    mats = [Material(gam=r[0], ym=r[1], nu=r[2], phi=r[3], coh=r[4], psi=r[5])
+            for r in z["materials"]]
+    return HexMesh(coord=z["coord"], conn=z["conn"], mat_id=z["mat_id"], materials=mats,
+                   bc={"ux": z["bc_ux"], "uy": z["bc_uy"], "uz": z["bc_uz"]})
+
+
+def _summary_values(text):
+    lines = text.split("\n")[:-2]  # the last line holds elapsed times
+    out = []
+    for ln in lines:
+        for t in ln.split():
+            try:
+                out.append(float(t))
+            except ValueError:
+                out.append(t)
+    return out
+
+
+def test_fp64_sweep_files_match_reference(tmp_path):
+    z = golden("output_slope_4x2x3")
+    mesh = _mesh(z)
+    ctx = Context(0)
+    try:
+        ctx.set_reduction(REDUCE_SEQUENTIAL)
+        spec = hex2spec_device(ctx, mesh)
+        out = SlopeOutput(str(tmp_path), "slope_out", mesh, spec, CTL, SRF)
+        slope3d(ctx, mesh, spec=spec, srf=SRF, output=out, **CTL)
+        out.close()
+    finally:
+        ctx.close()
+    names = [str(n) for n in z["file_names"]]
+    assert sorted(os.listdir(tmp_path)) == names
+    for n in names:
+        ref, mine = z["file:" + n].tobytes(), (tmp_path / n).read_bytes()
+        if n.endswith("_summary"):
+            assert _summary_values(mine.decode()) == _summary_values(ref.decode())
+        else:
+            assert mine == ref, n
+
+
+def test_mixed_sweep_files(tmp_path):
+    z = golden("output_slope_4x2x3")
+    mesh = _mesh(z)
+    ctx = Context(0)
+    try:
+        spec = hex2spec_device(ctx, mesh)
+        out = SlopeOutput(str(tmp_path), "slope_out", mesh, spec, CTL, SRF)
+        slope3d_mixed(ctx, mesh, spec=spec, srf=SRF, ncol=2, output=out, **CTL)
+        out.close()
+    finally:
+        ctx.close()
+    names = [str(n) for n in z["file_names"]]
+    assert sorted(os.listdir(tmp_path)) == names
+    for n in names:
+        ref, mine = z["file:" + n].tobytes(), (tmp_path / n).read_bytes()
+        if n.endswith(".dis"):
+            a = np.frombuffer(mine[244:], np.float32)
+            b = np.frombuffer(ref[244:], np.float32)
+            assert mine[:244] == ref[:244]
+            assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(b)
+        elif n.endswith("_summary"):
+            va, vb = _summary_values(mine.decode()), _summary_values(ref.decode())
+            assert len(va) == len(vb)
+            # rows: srf, cg_tot (mixed counts may differ slightly), nl_tot, uxmax, umax
+            rows_a, rows_b = va[-12:], vb[-12:]
+            for r in range(2):
+                a, b = rows_a[6 * r:6 * r + 6], rows_b[6 * r:6 * r + 6]
+                assert a[0] == b[0] and a[2] == b[2]
+                assert abs(a[3] - b[3]) <= 1e-7 * b[3] and abs(a[4] - b[4]) <= 1e-7 * b[4]
+        else:
+            assert mine == ref, n

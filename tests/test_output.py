@@ -1,0 +1,96 @@
+"""Result files (specfem3d_geotech_amd.output) against the reference's own.
+
+* Number format of the summary file: 2,325 doubles of every magnitude and
+  300 summary rows as the reference's compiler writes them list-directed
+  (tests/golden/list_format.npz, from oracle/listfmt_probe.f90).  Every item
+  converts to the same double, in the same F or E form, and the rows wrap at
+  the same records (at most 79 columns).  About 93% of the strings are identical
+  character for character; in the rest the compiler picks another last digit
+  among the shortest decimal strings that convert back to the same double,
+  a choice its runtime does not document.
+* The files of a two-SRF sweep of the 4x2x3 slope with displacement output
+  (tests/golden/output_slope_4x2x3.npz, written by the compiled reference):
+  the EnSight case and geometry files and the FP32 displacement files byte
+  for byte, the summary file value for value, from the oracle's
+  displacements (which equal the reference's bit for bit).
+"""
+import os
+
+import numpy as np
+
+import oracle as O
+from conftest import golden
+from specfem3d_geotech_amd.mesh import HexMesh, Material
+from specfem3d_geotech_amd.output import (SlopeOutput, case_slope, fortran_list,
+                                          fortran_real)
+from specfem3d_geotech_amd.spec import hex2spec
+
+
+def _val(tok: str) -> float:
+    return float(tok)
+
+
+def test_list_directed_numbers():
+    z = golden("list_format")
+    same = 0
+    for v, s in zip(z["values"], z["strings"]):
+        v = float(v)
+        mine, ref = fortran_real(v), str(s).strip()
+        assert _val(mine) == _val(ref) == v, (v, mine, ref)
+        assert ("E" in mine) == ("E" in ref), (v, mine, ref)
+        same += mine == ref
+    assert same / len(z["values"]) > 0.9
+
+
+def test_list_directed_rows_wrap_like_the_reference():
+    z = golden("list_format")
+    mine = "".join(fortran_list(float(a), int(b), int(c), float(d), float(e), float(f))
+                   for a, b, c, d, e, f in zip(z["row_srf"], z["row_cg"], z["row_nl"],
+                                               z["row_ux"], z["row_um"], z["row_fm"]))
+    ref = str(z["rows_text"])
+    ml, rl = mine.rstrip("\n").split("\n"), ref.rstrip("\n").split("\n")
+    assert len(ml) == len(rl)
+    for a, b in zip(ml, rl):
+        ta, tb = a.split(), b.split()
+        assert len(ta) == len(tb) and a.startswith(" ")
+        assert [_val(t) for t in ta] == [_val(t) for t in tb]
+
+
+def test_case_file_time_values():
+    assert case_slope("h", [1.0, 1.2]).endswith(
+        "time values:  0.10000E+01\n 0.12000E+01\n")
+
+
+def test_sweep_files_match_reference(tmp_path):
+    z = golden("output_slope_4x2x3")
+    mats = [Material(gam=r[0], ym=r[1], nu=r[2], phi=r[3], coh=r[4], psi=r[5])
+            for r in z["materials"]]
+    mesh = HexMesh(coord=z["coord"], conn=z["conn"], mat_id=z["mat_id"], materials=mats,
+                   bc={"ux": z["bc_ux"], "uy": z["bc_uy"], "uz": z["bc_uz"]})
+    spec = hex2spec(mesh)
+    ctl = dict(cg_tol=1e-8, cg_maxiter=5000, nl_tol=5e-4, nl_maxiter=300)
+    srf = [1.0, 1.2]
+    out = SlopeOutput(str(tmp_path), "slope_out", mesh, spec, ctl, srf)
+    for i, s in enumerate(srf):  # SRF trials are independent (src/slope.F90:248-282)
+        r = O.slope3d(spec, mesh, srf=[s], **ctl)
+        out.srf_done(i + 1, r["rows"][0], r["x"])
+    out.close()
+    names = [str(n) for n in z["file_names"]]
+    assert sorted(os.listdir(tmp_path)) == names
+    for n in names:
+        ref = z["file:" + n].tobytes()
+        mine = (tmp_path / n).read_bytes()
+        if n.endswith("_summary"):
+            # the last line holds elapsed times
+            ra, ma = ref.decode().split("\n")[:-2], mine.decode().split("\n")[:-2]
+            assert len(ra) == len(ma)
+            for a, b in zip(ma, ra):
+                ta, tb = a.split(), b.split()
+                assert len(ta) == len(tb), (a, b)
+                for x, y in zip(ta, tb):
+                    try:
+                        assert _val(x) == _val(y), (a, b)
+                    except ValueError:
+                        assert x == y, (a, b)
+        else:
+            assert mine == ref, n
