@@ -27,13 +27,18 @@ __device__ __forceinline__ bool wanted(const int32_t* cst, int j, int mask) {
 // --------------------------------------------------------------------------
 // FP32 batched element product on MFMA:
 //   Y[e][r][j] = sum_c K_e(r,c) P[g(e,c)][j]
-// with v_mfma_f32_16x16x4f32 (A 16x4, B 4x16, D 16x16): rows in 6 tiles of
-// 16 (81 -> 96), columns in tiles of 16 (B < 16: one partial tile), k in 21
-// steps of 4 (81 -> 84).  One element per block of 4 waves; wave w takes
-// column tile n = w % NT and row tiles m = w / NT (+ 4 / NT ...).
+// with v_mfma_f32_16x16x4f32 (A 16x4, B 4x16, D 16x16): rows 0..79 in 5
+// tiles of 16, columns in tiles of 16 (B < 16: one partial tile), k in 21
+// steps of 4 (81 -> 84).  Row 80 is a dot product on the vector ALUs: lane
+// (j, kq) sums K(80, 4s+kq) P(4s+kq, j) over the k-steps s -- the B operand
+// it already holds -- and the four kq partials are added across lanes in a
+// fixed order.  A sixth MFMA row tile would spend 15 of its 16 rows on
+// zero padding.  One element per block of 4 waves; wave w takes column
+// tile n = w % NT and row tiles m = w / NT (+ 4 / NT ...); the wave of each
+// column tile with the fewest row tiles also takes row 80.
 // The packed upper triangle of K_e is unpacked (with its mirror) into LDS
-// as Kf[c * BMV_LDK + r], zero padded; BMV_LDK = 112 puts the four 16-lane
-// groups of an A-operand read (c, c+1, c+2, c+3) on disjoint banks.
+// as Kf[c * BMV_LDK + r], zero padded in k; BMV_LDK = 112 puts the four
+// 16-lane groups of an A-operand read (c, c+1, c+2, c+3) on disjoint banks.
 // A column's result depends only on its own P column, so trials are
 // independent of the batch; columns not in `want_mask` are not written.
 // --------------------------------------------------------------------------
@@ -68,7 +73,8 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
                                                   const int32_t* __restrict__ cst,
                                                   int want_mask) {
   constexpr int NS = 4 / NT;             // waves per column tile
-  constexpr int NM = (6 + NS - 1) / NS;  // row tiles per wave
+  constexpr int NMT = 5;                 // MFMA row tiles: rows 0..79
+  constexpr int NM = (NMT + NS - 1) / NS;  // row tiles per wave
   constexpr int KPT = (3321 + 255) / 256;  // packed entries per thread
   __shared__ float Kf[BMV_KC * BMV_LDK];
   __shared__ int32_t sg[2][BMV_KC];
@@ -90,9 +96,8 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
   const bool wave_on = __any(jv);
   const int jl = j < B ? j : B - 1;  // lanes past B read column B-1, store nothing
   const int kq = l >> 4;
-  // zero padding, once: rows 81..95 of every k column, and k columns 81..83
-  for (int q = t; q < BMV_KC * 15; q += 256)
-    Kf[(q / 15) * BMV_LDK + 81 + q % 15] = 0.0f;
+  const bool row80 = mpart == NS - 1;  // wave-uniform
+  // zero padding, once: k columns 81..83 (rows 0..80)
   for (int q = t; q < 3 * 81; q += 256) Kf[(81 + q / 81) * BMV_LDK + q % 81] = 0.0f;
   if (t >= 81 && t < BMV_KC) { sg[0][t] = 0; sg[1][t] = 0; }
   // prologue: K and gdof of the first element
@@ -142,28 +147,33 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
       f32x4 acc[NM];
 #pragma unroll
       for (int i = 0; i < NM; i++) acc[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      float r80 = 0.0f;
 #pragma unroll
       for (int s = 0; s < 21; s++) {
         const float* kc = Kf + (4 * s + kq) * BMV_LDK + (l & 15);
 #pragma unroll
         for (int i = 0; i < NM; i++) {
           const int mt = mpart + i * NS;  // wave-uniform
-          if (NM * NS > 6 && mt >= 6) break;
+          if (NM * NS > NMT && mt >= NMT) break;
           acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(kc[mt * 16], bv[s], acc[i], 0, 0,
                                                         0);
         }
+        if (row80) r80 = __builtin_fmaf(Kf[(4 * s + kq) * BMV_LDK + 80], bv[s], r80);
+      }
+      if (row80) {  // the four k-groups of column j: lanes j, j+16, j+32, j+48
+        r80 = r80 + __shfl_xor(r80, 16);
+        r80 = r80 + __shfl_xor(r80, 32);
       }
       if (jv) {
 #pragma unroll
         for (int i = 0; i < NM; i++) {
           const int mt = mpart + i * NS;
-          if (NM * NS > 6 && mt >= 6) break;
+          if (NM * NS > NMT && mt >= NMT) break;
 #pragma unroll
-          for (int v = 0; v < 4; v++) {
-            const int row = mt * 16 + 4 * kq + v;
-            if (row < 81) Y[(e * 81 + row) * B + j] = acc[i][v];
-          }
+          for (int v = 0; v < 4; v++)
+            Y[(e * 81 + mt * 16 + 4 * kq + v) * B + j] = acc[i][v];
         }
+        if (row80 && kq == 0) Y[(e * 81 + 80) * B + j] = r80;
       }
     }
     if (t < 81) sg[cur ^ 1][t] = greg;

@@ -1079,6 +1079,8 @@ extern "C" int sgeo_slope_download(sgeo_slope* t, double* x, double* bodyload,
 // iterating.
 enum BatchPhase { BPH_IDLE = 0, BPH_RUN = 1, BPH_ENDED = 2 };
 
+#define STAGE_SLOTS_N 5  // pinned staging slots of a batch (see stage_upload)
+
 struct sgeo_batch {
   sgeo_mesh* mesh = nullptr;
   int B = 0;
@@ -1114,6 +1116,8 @@ struct sgeo_batch {
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
   BmvGeom geom;  // launch geometry of the MFMA product
+  int32_t* h_stage = nullptr;  // pinned staging of the small uploads
+  unsigned stage_busy = 0;
 };
 
 static int ensure_packed(sgeo_mesh* m, int kprec) {
@@ -1185,6 +1189,7 @@ extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extlo
   HIPCHK(hipHostMalloc((void**)&b->h_ps, sizeof(PlasticScalars) * ncol));
   HIPCHK(hipHostMalloc((void**)&b->h_bc, sizeof(BCol) * ncol));
   HIPCHK(hipHostMalloc((void**)&b->h_cst, sizeof(int32_t) * ncol));
+  HIPCHK(hipHostMalloc((void**)&b->h_stage, sizeof(int32_t) * ncol * STAGE_SLOTS_N));
   b->phase.assign(ncol, BPH_IDLE);
   b->nl_iter.assign(ncol, 0);
   b->cg_tot.assign(ncol, 0);
@@ -1235,6 +1240,7 @@ extern "C" int sgeo_batch_destroy(sgeo_batch* b) {
   if (b->h_ps) hipHostFree(b->h_ps);
   if (b->h_bc) hipHostFree(b->h_bc);
   if (b->h_cst) hipHostFree(b->h_cst);
+  if (b->h_stage) hipHostFree(b->h_stage);
   batch_drop_graphs(b);
   delete b;
   return 0;
@@ -1376,19 +1382,32 @@ extern "C" int sgeo_batch_release(sgeo_batch* b, int32_t col) {
   return 0;
 }
 
-// Small per-call uploads (column masks, lists, flags) come from host vectors
-// that die with the call: each is followed by a stream synchronisation.
-static int upload_sync(sgeo_batch* b, void* dst, const void* src, size_t bytes) {
+// Small per-call uploads (column masks, lists, states) go through pinned
+// staging slots and are queued on the stream without a synchronisation; a
+// slot is rewritten only after a synchronisation has retired its previous
+// copy (stage_busy marks slots whose copy may still be queued).
+enum StageSlot { STAGE_START_MASK = 0, STAGE_CST, STAGE_BODY_MASK, STAGE_ISLAST,
+                 STAGE_COLS, STAGE_SLOTS };
+
+static_assert(STAGE_SLOTS == STAGE_SLOTS_N, "staging slots");
+
+static int stage_upload(sgeo_batch* b, int slot, void* dst, const int32_t* src, int n) {
   hipStream_t s = b->mesh->ctx->stream;
-  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
-  HIPCHK(hipStreamSynchronize(s));
+  if (b->stage_busy & (1u << slot)) {
+    HIPCHK(hipStreamSynchronize(s));
+    b->stage_busy = 0;
+  }
+  int32_t* h = b->h_stage + (size_t)slot * b->B;
+  memcpy(h, src, (size_t)n * 4);
+  HIPCHK(hipMemcpyAsync(dst, h, (size_t)n * 4, hipMemcpyHostToDevice, s));
+  b->stage_busy |= 1u << slot;
   return 0;
 }
 
-static int upload_mask(sgeo_batch* b, const std::vector<int32_t>& cols) {
+static int upload_mask(sgeo_batch* b, int slot, const std::vector<int32_t>& cols) {
   std::vector<int32_t> mask(b->B, 0);
   for (int j : cols) mask[j] = 1;
-  return upload_sync(b, b->d_mask, mask.data(), b->B * 4);
+  return stage_upload(b, slot, b->d_mask, mask.data(), b->B);
 }
 
 // Starts a PCG call for `cols`: load = extload + bodyload, fresh scalars;
@@ -1396,7 +1415,7 @@ static int upload_mask(sgeo_batch* b, const std::vector<int32_t>& cols) {
 static int batch_start(sgeo_batch* b, const std::vector<int32_t>& cols, double cg_tol,
                        int32_t cg_maxiter, double inner_tol) {
   hipStream_t s = b->mesh->ctx->stream;
-  TRY(upload_mask(b, cols));
+  TRY(upload_mask(b, STAGE_START_MASK, cols));
   launch_bload(s, b->n1, b->B, b->d_ext, b->d_body, b->d_load, b->d_mask);
   for (int j : cols) {
     const bool skip = b->seeded[j] && b->nl_iter[j] == 0;
@@ -1411,7 +1430,7 @@ static int batch_start(sgeo_batch* b, const std::vector<int32_t>& cols, double c
     b->last_it[j] = bc.it;
     b->last_rep[j] = 0;
   }
-  TRY(upload_sync(b, b->d_cst, b->cst.data(), b->B * 4));
+  TRY(stage_upload(b, STAGE_CST, b->d_cst, b->cst.data(), b->B));
   HIPCHK(hipMemcpyAsync(b->d_bc, b->h_bc, b->B * sizeof(BCol), hipMemcpyHostToDevice, s));
   return 0;
 }
@@ -1475,6 +1494,7 @@ static int batch_steps(sgeo_batch* b, int chunk, bool* any_running) {
   HIPCHK(hipMemcpyAsync(b->h_cst, b->d_cst, B * 4, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(b->h_bc, b->d_bc, B * sizeof(BCol), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  b->stage_busy = 0;
   if (c->prof) TRY(prof_flush(c));
   bool running = false;
   for (int j = 0; j < B; j++) {
@@ -1511,9 +1531,9 @@ static int batch_body(sgeo_batch* b, const std::vector<int32_t>& cols, double nl
   for (int j : cols)
     islast[j] = is_last_override ? (is_last_override[j] ? 1 : 0)
                                  : (b->nl_iter[j] + 1 == nl_maxiter ? 1 : 0);
-  TRY(upload_mask(b, cols));
-  TRY(upload_sync(b, b->d_islast, islast.data(), B * 4));
-  TRY(upload_sync(b, b->d_cols, cols.data(), cols.size() * 4));
+  TRY(upload_mask(b, STAGE_BODY_MASK, cols));
+  TRY(stage_upload(b, STAGE_ISLAST, b->d_islast, islast.data(), B));
+  TRY(stage_upload(b, STAGE_COLS, b->d_cols, cols.data(), (int)cols.size()));
   launch_buerr(s, b->n1, B, b->d_x, b->d_oldx, b->d_part, b->d_ps, b->d_mask, nl_tol);
   BodyArgs ba;
   ba.nel = m->nelmt;
@@ -1539,6 +1559,7 @@ static int batch_body(sgeo_batch* b, const std::vector<int32_t>& cols, double nl
   HIPCHK(hipMemcpyAsync(b->h_ps, b->d_ps, B * sizeof(PlasticScalars),
                         hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  b->stage_busy = 0;
   for (int j : cols) {
     b->nl_iter[j] += 1;
     b->cg_tot[j] += b->h_bc[j].it;
