@@ -27,7 +27,8 @@ Modes:
   serve     a Poisson stream (--rate, default 4/s) of --requests analyses on
             seeded meshes of --min/--max-elements (log-uniform): 70% slope FOS
             sweeps (angle 20-45 deg, 1-3 layers; mixed multi-trial engine,
-            8 SRFs in flight), 30% staged box-pit excavations (3-6 stages);
+            --columns SRFs in flight, default 8), 30% staged box-pit
+            excavations (3-6 stages);
             admission by HBM footprint, concurrent requests on their own
             streams.  value = analyses/hour, with p50/p99 latency.
   mixed     the mixed-precision form (FP32 stiffness and inner CG, FP64
@@ -89,6 +90,8 @@ def parse():
     ap.add_argument("--min-elements", type=int, default=500)
     ap.add_argument("--max-elements", type=int, default=5000)
     ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--columns", type=int, default=8,
+                    help="serve mode: trial columns per slope request (SRFs in flight)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="batched engine: launch every step's kernels one by one "
                          "(kernel-trace profilers record graph replays poorly)")
@@ -480,7 +483,7 @@ def mode_serve(a):
     from specfem3d_geotech_amd.serving import Server, make_requests
     reqs = make_requests(a.requests, rate=a.rate or 4.0,
                          nelmt_range=(a.min_elements, a.max_elements), seed=4)
-    srv = Server(max_workers=a.workers, slope_columns=8)
+    srv = Server(max_workers=a.workers, slope_columns=a.columns)
 
     def done(r):
         log(f"  request {r.tag} {r.kind} {r.nelmt} el: latency {r.latency:.2f} s")
@@ -494,7 +497,8 @@ def mode_serve(a):
         config=dict(workload=f"serving: Poisson {a.rate or 4.0}/s, {len(reqs)} requests, "
                     f"meshes {a.min_elements}-{a.max_elements} elements log-uniform, "
                     f"{kinds.count('slope')} slope sweeps + {kinds.count('excavation')} "
-                    "staged excavations", workers=a.workers, mode="serve"),
+                    "staged excavations", workers=a.workers, columns=a.columns,
+                    mode="serve"),
         latency_s=dict(p50=st["p50_s"], p99=st["p99_s"], max=st["max_s"]),
         serving=st)
 

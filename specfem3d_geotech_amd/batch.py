@@ -314,7 +314,7 @@ def _trial_sink(on_srf, output):
 def slope3d_mixed(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
                   nl_tol: float, nl_maxiter: int, srf: Sequence[float],
                   inner_tol: float = 1e-5, ncol: int = 1, spec=None, on_srf=None,
-                  progress=None, output=None) -> SlopeResult:
+                  progress=None, output=None, phinu: bool = False) -> SlopeResult:
     """The reference's SRF sweep (src/slope.F90:244-443) in the mixed-precision
     form: FP32 stiffness and inner CG, FP64 refinement to cg_tol.  With
     ncol > 1 up to ncol SRFs of the sweep run at once; the rows and the
@@ -323,6 +323,11 @@ def slope3d_mixed(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int
     `progress` with the engine statistics now and then; `output`
     (output.SlopeOutput) receives the rows and displacements for the
     reference's result files."""
+    if phinu:
+        # src/plastic_library.f90:72-86 changes nu with the SRF, and with it
+        # the stiffness (src/slope.F90:259-265): trials no longer share one
+        # operator, which the multi-trial engine relies on
+        raise NotImplementedError("phinu sweeps change K per SRF: use slope.slope3d")
     spec = spec or hex2spec_device(ctx, mesh)
     _, ym, nu, phi, coh, psi = mesh.material_arrays()
     dm, ext = device_mesh(ctx, mesh, spec)
