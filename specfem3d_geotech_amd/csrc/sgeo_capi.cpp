@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1118,6 +1119,7 @@ struct sgeo_batch {
   BmvGeom geom;  // launch geometry of the MFMA product
   int32_t* h_stage = nullptr;  // pinned staging of the small uploads
   unsigned stage_busy = 0;
+  double step_us = 0.0;  // running mean of the wall time of one step
 };
 
 static int ensure_packed(sgeo_mesh* m, int kprec) {
@@ -1474,6 +1476,7 @@ static int batch_steps(sgeo_batch* b, int chunk, bool* any_running) {
   sgeo_ctx* c = m->ctx;
   hipStream_t s = c->stream;
   const int B = b->B;
+  const auto t0 = std::chrono::steady_clock::now();
   const int gi = chunk == 8 ? 0 : chunk == 16 ? 1 : chunk == 32 ? 2 : -1;
   if (b->use_graphs && !c->prof && gi >= 0) {
     if (!b->gexec[gi]) {
@@ -1495,6 +1498,11 @@ static int batch_steps(sgeo_batch* b, int chunk, bool* any_running) {
   HIPCHK(hipMemcpyAsync(b->h_bc, b->d_bc, B * sizeof(BCol), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   b->stage_busy = 0;
+  {
+    const double us = std::chrono::duration<double, std::micro>(
+                          std::chrono::steady_clock::now() - t0).count() / chunk;
+    b->step_us = b->step_us > 0.0 ? 0.8 * b->step_us + 0.2 * us : us;
+  }
   if (c->prof) TRY(prof_flush(c));
   bool running = false;
   for (int j = 0; j < B; j++) {
@@ -1633,7 +1641,7 @@ extern "C" int sgeo_batch_advance(sgeo_batch* b, double cg_tol, int32_t cg_maxit
   for (int j = 0; j < B; j++) ended[j] = 0;
   const bool memo0 = b->memo_valid;
   int64_t steps = 0;
-  int chunk = 8;
+  int chunk = (b->step_us > 0.0 && b->step_us < 1000.0) ? 32 : 8;
   for (;;) {
     // columns that need a new PCG call
     std::vector<int32_t> starts;
@@ -1641,7 +1649,11 @@ extern "C" int sgeo_batch_advance(sgeo_batch* b, double cg_tol, int32_t cg_maxit
       if (b->phase[j] == BPH_RUN && b->need_start[j]) starts.push_back(j);
     if (!starts.empty()) {
       TRY(batch_start(b, starts, cg_tol, cg_maxiter, inner_tol));
-      chunk = 8;
+      // Long steps: short first chunks, so a column that ends its call
+      // soon after a start waits little.  Short steps (small meshes, where
+      // each synchronisation and body pass costs more than a step): whole
+      // 32-step chunks, so ends are handled together.
+      chunk = (b->step_us > 0.0 && b->step_us < 1000.0) ? 32 : 8;
     }
     bool running = false;
     for (int j = 0; j < B; j++)
