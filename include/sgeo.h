@@ -271,6 +271,11 @@ int sgeo_batch_advance(sgeo_batch *batch, double cg_tol, int32_t cg_maxiter,
                        int32_t *memo_ready);
 int sgeo_batch_download(sgeo_batch *batch, int32_t col, double *x,
                         uint8_t *flags);
+/* Plastic iterations done so far by each column's current trial (ncol
+ * values; 0 where no trial runs), as of the last body pass.  Lets a
+ * scheduler see a trial heading for nl_maxiter (a failing SRF) before it
+ * ends.  No device access. */
+int sgeo_batch_nl_iters(sgeo_batch *batch, int32_t *nl_iter);
 /* CG steps run as captured HIP graphs of 8, 16 or 32 steps (default, fewer
  * launch submissions for small meshes) or as separate launches (enable 0;
  * HIP-event timing with sgeo_profile always uses separate launches).  The

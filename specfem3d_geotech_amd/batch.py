@@ -95,6 +95,12 @@ class BatchEngine:
         return dict(ended=ended, nl_tot=nl, cg_tot=cg, fmax=fm, errcode=ec,
                     memo_ready=bool(memo.value))
 
+    def nl_iters(self) -> np.ndarray:
+        out = np.zeros(self.ncol, np.int32)
+        check(L.lib().sgeo_batch_nl_iters(self.h, ptr(out, C.c_int32)),
+              "sgeo_batch_nl_iters")
+        return out
+
     def download(self, col):
         sp = self.dmesh.spec
         x = np.zeros(sp.neq + 1)
@@ -144,6 +150,7 @@ class Analysis:
     t_end: Optional[float] = None
     flags_last: Optional[np.ndarray] = None
     x_last: Optional[np.ndarray] = None
+    hold: Optional[int] = None  # SRF index of a trial heading for failure
 
     @property
     def latency(self):
@@ -160,7 +167,8 @@ def run_analyses(engine, analyses: List[Analysis], *, ym, nu, cg_tol: float,
                  keep_state: bool = False, on_trial=None,
                  arrivals: Optional[List[float]] = None,
                  clock=time.perf_counter, on_done=None, poll_steps: int = 64,
-                 progress=None, progress_steps: int = 2000):
+                 progress=None, progress_steps: int = 2000,
+                 defer_nl: Optional[int] = None):
     """Runs every analysis to its FOS bracket, keeping the engine's columns
     busy (continuous batching).
 
@@ -174,8 +182,14 @@ def run_analyses(engine, analyses: List[Analysis], *, ym, nu, cg_tol: float,
     steps so they are admitted as soon as a column is free.  Until the
     engine holds the elastic solution (the first PCG call of any trial) only
     one trial runs.  `progress`, if given, is called with the engine's
-    statistics at least every `progress_steps` CG steps.  Returns the
-    engine's statistics."""
+    statistics at least every `progress_steps` CG steps.
+
+    `defer_nl` limits speculation: once a trial of an analysis has run
+    defer_nl plastic iterations without converging (a failing SRF runs to
+    nl_maxiter), the analysis's trials at higher SRFs are released and none
+    is started above it until it ends; if it converges after all, they run
+    again from the start.  A trial's results do not depend on when it runs,
+    so rows and brackets are unchanged.  Returns the engine's statistics."""
     spec = engine.dmesh.spec
     ncol = engine.ncol
     cols: List[Optional[tuple]] = [None] * ncol  # (analysis, srf index)
@@ -188,6 +202,35 @@ def run_analyses(engine, analyses: List[Analysis], *, ym, nu, cg_tol: float,
 
     def running_of(a):
         return sum(1 for c in cols if c is not None and c[0] is a)
+
+    def next_k(a):
+        """Lowest SRF index of `a` neither done nor running, below a known
+        failure and not above a held trial."""
+        busy = {c[1] for c in cols if c is not None and c[0] is a}
+        for k in range(len(a.srf)):
+            if k in a.rows or k in busy:
+                continue
+            if a.fail_idx is not None and k >= a.fail_idx:
+                return None
+            if a.hold is not None and k > a.hold:
+                return None
+            return k
+        return None
+
+    def apply_holds():
+        nl = engine.nl_iters()
+        for a in admitted:
+            if a.done:
+                continue
+            mine = [(j, c[1]) for j, c in enumerate(cols) if c is not None and c[0] is a]
+            slow = [k for j, k in mine if nl[j] >= defer_nl]
+            a.hold = min(slow) if slow else None
+            if a.hold is None:
+                continue
+            for j, k in mine:
+                if k > a.hold:
+                    engine.release(j)
+                    cols[j] = None
 
     def start_trial(j, a, k):
         s = a.srf[k]
@@ -207,13 +250,12 @@ def run_analyses(engine, analyses: List[Analysis], *, ym, nu, cg_tol: float,
                 return  # one trial computes the elastic solution first
             placed = False
             for a in admitted:  # next SRF of an admitted analysis
-                if (not a.done and running_of(a) < srf_parallel
-                        and a.next_srf < len(a.srf)
-                        and (a.fail_idx is None or a.next_srf < a.fail_idx)):
-                    start_trial(j, a, a.next_srf)
-                    a.next_srf += 1
-                    placed = True
-                    break
+                if not a.done and running_of(a) < srf_parallel:
+                    k = next_k(a)
+                    if k is not None:
+                        start_trial(j, a, k)
+                        placed = True
+                        break
             if placed:
                 continue
             ready = [i for i in queue if arrivals is None or arrivals[i] <= now - t0]
@@ -222,8 +264,7 @@ def run_analyses(engine, analyses: List[Analysis], *, ym, nu, cg_tol: float,
             queue.remove(ready[0])
             a = analyses[ready[0]]
             admitted.append(a)
-            start_trial(j, a, a.next_srf)
-            a.next_srf += 1
+            start_trial(j, a, next_k(a))
 
     def finish_if_possible(a):
         if a.done:
@@ -261,6 +302,8 @@ def run_analyses(engine, analyses: List[Analysis], *, ym, nu, cg_tol: float,
             continue
         pending = bool(queue) and arrivals is not None
         cap = poll_steps if pending else (progress_steps if progress else 1 << 40)
+        if defer_nl is not None and srf_parallel > 1:
+            cap = min(cap, poll_steps)  # look at the trials' progress now and then
         r = engine.advance(cg_tol, cg_maxiter, inner_tol, nl_tol, nl_maxiter,
                            max_steps=cap)
         if progress:
@@ -286,6 +329,8 @@ def run_analyses(engine, analyses: List[Analysis], *, ym, nu, cg_tol: float,
             engine.release(j)
             cols[j] = None
             finish_if_possible(a)
+        if defer_nl is not None and srf_parallel > 1:
+            apply_holds()
         fill()
     return engine.stats()
 
@@ -314,7 +359,8 @@ def _trial_sink(on_srf, output):
 def slope3d_mixed(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
                   nl_tol: float, nl_maxiter: int, srf: Sequence[float],
                   inner_tol: float = 1e-5, ncol: int = 1, spec=None, on_srf=None,
-                  progress=None, output=None, phinu: bool = False) -> SlopeResult:
+                  progress=None, output=None, phinu: bool = False,
+                  defer_nl: Optional[int] = None) -> SlopeResult:
     """The reference's SRF sweep (src/slope.F90:244-443) in the mixed-precision
     form: FP32 stiffness and inner CG, FP64 refinement to cg_tol.  With
     ncol > 1 up to ncol SRFs of the sweep run at once; the rows and the
@@ -337,6 +383,7 @@ def slope3d_mixed(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int
         run_analyses(eng, [a], ym=ym, nu=nu, cg_tol=cg_tol, cg_maxiter=cg_maxiter,
                      nl_tol=nl_tol, nl_maxiter=nl_maxiter, inner_tol=inner_tol,
                      srf_parallel=ncol, keep_state=True, progress=progress,
+                     defer_nl=defer_nl,
                      on_trial=_trial_sink(on_srf, output))
         stats = eng.stats()
     finally:

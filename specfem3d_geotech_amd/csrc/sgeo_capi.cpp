@@ -1695,6 +1695,14 @@ extern "C" int sgeo_batch_advance(sgeo_batch* b, double cg_tol, int32_t cg_maxit
   return 0;
 }
 
+// Plastic iterations each column's current trial has done (host copy,
+// as of the last body pass; 0 for columns without a running trial).
+extern "C" int sgeo_batch_nl_iters(sgeo_batch* b, int32_t* nl_iter) {
+  if (!b || !nl_iter) return fail(SGEO_EARG, "sgeo_batch_nl_iters: null argument");
+  for (int j = 0; j < b->B; j++) nl_iter[j] = b->phase[j] == BPH_RUN ? b->nl_iter[j] : 0;
+  return 0;
+}
+
 extern "C" int sgeo_batch_download(sgeo_batch* b, int32_t col, double* x,
                                    uint8_t* flags) {
   if (!b || col < 0 || col >= b->B) return fail(SGEO_EARG, "bad column");

@@ -104,7 +104,11 @@ class Server:
         mesh = req.make_mesh()
         if req.kind == "slope":
             from .batch import slope3d_mixed
-            return slope3d_mixed(ctx, mesh, ncol=self.slope_columns, **req.control)
+            # SRFs run slope_columns at a time; trials above one that has run
+            # a fifth of nl_maxiter plastic iterations wait for it
+            defer = max(10, int(req.control["nl_maxiter"]) // 5)
+            return slope3d_mixed(ctx, mesh, ncol=self.slope_columns, defer_nl=defer,
+                                 **req.control)
         from .excavation import excavation3d
         return excavation3d(ctx, mesh, **req.control)
 

@@ -40,6 +40,10 @@ class StandIn:
     def release(self, col):
         self.col[col] = None
 
+    def nl_iters(self):
+        return np.array([0 if s is None or s["ended"] else s["it"] for s in self.col],
+                        np.int32)
+
     def advance(self, cg_tol, cg_maxiter, inner_tol, nl_tol, nl_maxiter,
                 max_steps=1 << 40):
         n = self.ncol
@@ -141,3 +145,17 @@ def test_arrivals_delay_admission():
     eng, an = _run(2, COHS[:3], SRF, arrivals=[0.0, 0.0, 0.5], clock=clock)
     assert an[2].t_start >= an[2].t_arrival
     assert all(a.done for a in an)
+
+
+@pytest.mark.parametrize("ncol,srf_parallel,defer", [(4, 3, 2), (8, 6, 3), (8, 6, 1)])
+def test_failing_trial_holds_speculation(ncol, srf_parallel, defer):
+    """Trials above a trial heading for nl_maxiter are released and rerun
+    only if it converges: rows and brackets equal the sequential sweep's,
+    with fewer trials started above the first failure."""
+    eng, an = _run(ncol, COHS, SRF, srf_parallel=srf_parallel, defer_nl=defer,
+                   poll_steps=1)
+    eng0, ref = _run(ncol, COHS, SRF, srf_parallel=srf_parallel)
+    for a, r in zip(an, ref):
+        assert a.done and a.bracket == r.bracket == expected_bracket(float(a.coh[0]), SRF)
+        assert a.rows == r.rows
+    assert all(s is None for s in eng.col)
