@@ -82,10 +82,11 @@ class Server:
 
     def __init__(self, device: int = 0, hbm_budget: Optional[int] = None,
                  max_workers: int = 8, slope_columns: int = 8,
-                 context_factory=None):
+                 context_factory=None, defer_frac: Optional[float] = None):
         self.device = device
         self.max_workers = max_workers
         self.slope_columns = slope_columns
+        self.defer_frac = defer_frac
         self.context_factory = context_factory or (lambda: Context(device))
         if hbm_budget is None:
             c = self.context_factory()
@@ -104,9 +105,10 @@ class Server:
         mesh = req.make_mesh()
         if req.kind == "slope":
             from .batch import slope3d_mixed
-            # SRFs run slope_columns at a time; trials above one that has run
-            # a fifth of nl_maxiter plastic iterations wait for it
-            defer = max(10, int(req.control["nl_maxiter"]) // 5)
+            # SRFs run slope_columns at a time; with defer_frac, trials above
+            # one that has run that fraction of nl_maxiter wait for it
+            defer = (max(10, int(self.defer_frac * int(req.control["nl_maxiter"])))
+                     if self.defer_frac else None)
             return slope3d_mixed(ctx, mesh, ncol=self.slope_columns, defer_nl=defer,
                                  **req.control)
         from .excavation import excavation3d
