@@ -95,6 +95,9 @@ def parse():
     ap.add_argument("--no-graphs", action="store_true",
                     help="batched engine: launch every step's kernels one by one "
                          "(kernel-trace profilers record graph replays poorly)")
+    ap.add_argument("--geometry", default=None,
+                    help="sweep mode: MFMA product launch geometry 'blocks_per_cu,xcd_split' "
+                         "instead of the autotuner's pick (e.g. 3,1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-iters", type=int, default=30)
     return ap.parse_args()
@@ -214,6 +217,10 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
             handle(eng.advance(max_steps=warm, **kw))
         ctx.synchronize()
         tune = eng.autotune(5)
+        if a.geometry:  # a fixed launch geometry instead of the tuner's pick
+            bpc, xs = (int(v) for v in a.geometry.split(","))
+            eng.set_geometry(bpc, bool(xs))
+            tune.update(blocks_per_cu=bpc, xcd_split=xs, fixed=True)
         log("autotune (MFMA product launch geometry): " + ", ".join(
             f"{c['blocks_per_cu']}/CU {'xcd' if c['xcd_split'] else 'flat'} {c['ms']:.3f} ms"
             for c in tune["candidates"]))

@@ -291,6 +291,9 @@ int sgeo_batch_set_graphs(sgeo_batch *batch, int enable);
  * tuning between steps does not change the trials' results. */
 int sgeo_batch_autotune(sgeo_batch *batch, int32_t reps, int32_t *blocks_per_cu,
                         int32_t *xcd_split, double *ms_best, double *ms_table);
+/* Sets that geometry directly (blocks per CU 1..3, XCD split 0/1); the
+ * default is 3 blocks per CU with the XCD split. */
+int sgeo_batch_set_geometry(sgeo_batch *batch, int32_t blocks_per_cu, int32_t xcd_split);
 /* Totals since creation, as of the last host synchronisation: CG
  * iterations of all trials (seeded elastic solutions not counted), FP64
  * residuals of all trials, and batched steps (one step = one CG iteration

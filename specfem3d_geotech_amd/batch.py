@@ -95,6 +95,10 @@ class BatchEngine:
         return dict(ended=ended, nl_tot=nl, cg_tot=cg, fmax=fm, errcode=ec,
                     memo_ready=bool(memo.value))
 
+    def set_geometry(self, blocks_per_cu: int, xcd_split: bool):
+        check(L.lib().sgeo_batch_set_geometry(self.h, int(blocks_per_cu), int(xcd_split)),
+              "sgeo_batch_set_geometry")
+
     def nl_iters(self) -> np.ndarray:
         out = np.zeros(self.ncol, np.int32)
         check(L.lib().sgeo_batch_nl_iters(self.h, ptr(out, C.c_int32)),

@@ -1254,6 +1254,17 @@ extern "C" int sgeo_batch_set_graphs(sgeo_batch* b, int enable) {
   return 0;
 }
 
+extern "C" int sgeo_batch_set_geometry(sgeo_batch* b, int32_t blocks_per_cu,
+                                       int32_t xcd_split) {
+  if (!b || blocks_per_cu < 1 || blocks_per_cu > 3 || xcd_split < 0 || xcd_split > 1)
+    return fail(SGEO_EARG, "sgeo_batch_set_geometry: blocks_per_cu 1..3, xcd_split 0/1");
+  if (blocks_per_cu != b->geom.blocks_per_cu || xcd_split != b->geom.xcd_split)
+    batch_drop_graphs(b);
+  b->geom.blocks_per_cu = blocks_per_cu;
+  b->geom.xcd_split = xcd_split;
+  return 0;
+}
+
 // Launch-geometry autotuner of the MFMA product: times `reps` launches of
 // every candidate (blocks per CU 1..3, XCD-aware split on/off) on the
 // engine's own mesh and the columns iterating now, and keeps the fastest.
