@@ -152,6 +152,12 @@ int sgeo_mesh_download_stiffness(sgeo_mesh *mesh, double *storekm);
 int sgeo_mesh_matvec(sgeo_mesh *mesh, const double *p, double *kp);
 /* HBM bytes held by the mesh. */
 int sgeo_mesh_bytes(sgeo_mesh *mesh, size_t *bytes);
+/* Node map for the nodal result fields (sgeo_slope_nodal_fields,
+ * sgeo_batch_nodal_fields): g_num(27, nelmt) 0-based node ids, as
+ * sgeo_spec_fetch returns them; node_valency is the number of (element,
+ * point) pairs at a node (src/slope.F90:203-210).  The reference computes
+ * the same in its drivers; it has no separate interface. */
+int sgeo_mesh_set_nodes(sgeo_mesh *mesh, int32_t nnode, const int32_t *g_num);
 /* pcg_solver (src/solver.f90:14-79) on a resident mesh: its selected
  * operator and inverted diagonal; u (host, neq+1) warm start, overwritten;
  * f (host, neq+1).  cg_iter and errcode as sgeo_pcg_solver. */
@@ -216,6 +222,17 @@ int sgeo_slope_run_srf(sgeo_slope *slope, const int32_t *gdof_node,
  * (27*nelmt bytes: f>=0 at the last body pass).  Any pointer may be NULL. */
 int sgeo_slope_download(sgeo_slope *slope, double *x, double *bodyload,
                         double *evpt, double *stress_elmt, uint8_t *flags);
+/* The nodal fields save_data writes for the trial's final stress state
+ * (src/slope.F90:418-439, src/postprocess.f90:128-248), computed on the
+ * device and rounded to FP32 as the EnSight files hold them, in component
+ * blocks of nnode values: stress (6 blocks) the mean of stress_elmt over a
+ * node's elements summed in element order, psigma (3 blocks) the principal
+ * stresses ascending, nsigma = (psigma1 + psigma3) / 2, vmeps the nodal
+ * mean of the excavation's per-point plastic strain (zero for a slope
+ * trial, which does not accumulate it).  Any pointer may be NULL; needs
+ * sgeo_mesh_set_nodes. */
+int sgeo_slope_nodal_fields(sgeo_slope *slope, float *stress, float *psigma,
+                            float *nsigma, float *vmeps);
 
 /* ---- multi-trial mixed-precision engine ---------------------------------
  * Up to 64 (a power of two) strength-reduction trials on one resident mesh,
@@ -271,6 +288,9 @@ int sgeo_batch_advance(sgeo_batch *batch, double cg_tol, int32_t cg_maxiter,
                        int32_t *memo_ready);
 int sgeo_batch_download(sgeo_batch *batch, int32_t col, double *x,
                         uint8_t *flags);
+/* sgeo_slope_nodal_fields for the trial held by column col. */
+int sgeo_batch_nodal_fields(sgeo_batch *batch, int32_t col, float *stress, float *psigma,
+                            float *nsigma, float *vmeps);
 /* Plastic iterations done so far by each column's current trial (ncol
  * values; 0 where no trial runs), as of the last body pass.  Lets a
  * scheduler see a trial heading for nl_maxiter (a failing SRF) before it

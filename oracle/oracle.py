@@ -53,8 +53,9 @@ def lib():
                                      + [d] * 6 + [C.c_int, C.c_double, C.c_int32,
                                                   C.c_double, C.c_int32, C.c_int, d,
                                                   i32, i32, d, d, d, d, i32,
-                                                  C.c_int32, i32])
+                                                  C.c_int32, i32, d])
         _LIB.orc_slope3d.restype = C.c_int
+        _LIB.orc_nodal_fields.argtypes = [C.c_int32, C.c_int32, i32, d, d, d, d]
         _LIB.orc_elastic_stress.argtypes = [C.c_int32, i32, d, i32, i32, d, d, d, d]
         _LIB.orc_excavation_load_nodal.argtypes = [C.c_int32, i32, d, i32, i32, d, d,
                                                    C.c_int32, d]
@@ -179,7 +180,7 @@ def plastic_body(spec, ym, nuf, phif, cohf, psif, dt, x, evpt, stress,
 def slope3d(spec, mesh, *, cg_tol, cg_maxiter, nl_tol, nl_maxiter, srf,
             phinu=False, cg_log_cap=100000):
     """Full SRF sweep; returns dict with per-SRF summary rows, the final x
-    and the cg_iter of every PCG call."""
+    and stress state (stress_elmt) and the cg_iter of every PCG call."""
     gam, ym, nu, phi, coh, psi = mesh.material_arrays()
     srf = _f64(srf)
     n = len(srf)
@@ -187,6 +188,7 @@ def slope3d(spec, mesh, *, cg_tol, cg_maxiter, nl_tol, nl_maxiter, srf,
     ux = np.zeros(n); um = np.zeros(n); fm = np.zeros(n)
     x = np.zeros(spec.neq + 1)
     log = np.zeros(cg_log_cap, np.int32); nlog = C.c_int32()
+    stress = np.zeros((spec.nelmt, 27, 6))
     gnode = _i32(spec.gdof)
     rc = lib().orc_slope3d(spec.nelmt, spec.nnode, spec.neq,
                            _p(_f64(spec.corners)),
@@ -196,13 +198,26 @@ def slope3d(spec, mesh, *, cg_tol, cg_maxiter, nl_tol, nl_maxiter, srf,
                            _p(psi), int(phinu), cg_tol, cg_maxiter, nl_tol,
                            nl_maxiter, n, _p(srf), _p(scg, C.c_int32),
                            _p(snl, C.c_int32), _p(ux), _p(um), _p(fm), _p(x),
-                           _p(log, C.c_int32), cg_log_cap, C.byref(nlog))
+                           _p(log, C.c_int32), cg_log_cap, C.byref(nlog), _p(stress))
     if rc < 0:
         raise RuntimeError("ERROR: PCG solver doesn't converge!")
     rows = [dict(srf=float(srf[s]), cg_tot=int(scg[s]), nl_tot=int(snl[s]),
                  uxmax=float(ux[s]), umax=float(um[s]), fmax=float(fm[s]))
             for s in range(rc)]
-    return dict(rows=rows, x=x, cg_log=log[:min(nlog.value, cg_log_cap)].copy())
+    return dict(rows=rows, x=x, stress=stress,
+                cg_log=log[:min(nlog.value, cg_log_cap)].copy())
+
+
+def nodal_fields(spec, stress):
+    """The nodal fields the reference writes for a trial's final stress
+    state (src/slope.F90:418-437, src/postprocess.f90:146-183): stress
+    (nnode, 6) the mean over the elements at a node, psigma (nnode, 3) the
+    principal stresses ascending, nsigma (nnode,)."""
+    sig = np.zeros((spec.nnode, 6)); psig = np.zeros((spec.nnode, 3))
+    nsig = np.zeros(spec.nnode)
+    lib().orc_nodal_fields(spec.nelmt, spec.nnode, _p(_i32(spec.g_num), C.c_int32),
+                           _p(_f64(stress)), _p(sig), _p(psig), _p(nsig))
+    return dict(stress=sig, psigma=psig, nsigma=nsig)
 
 
 # ----------------------------------------------------------------------------

@@ -740,7 +740,7 @@ int orc_slope3d(int32_t nelmt, int32_t nnode, int32_t neq, const double *xc,
                 int32_t nl_maxiter, int nsrf, const double *srf,
                 int32_t *sum_cg, int32_t *sum_nl, double *uxmax, double *umax,
                 double *fmax_out, double *x_out, int32_t *cg_log,
-                int32_t cg_log_cap, int32_t *n_cg_log) {
+                int32_t cg_log_cap, int32_t *n_cg_log, double *stress_out) {
   const int64_t n1 = (int64_t)neq + 1;
   const size_t nk = (size_t)NED * NED * nelmt;
   double *storekm = (double *)malloc(sizeof(double) * nk);
@@ -827,6 +827,7 @@ int orc_slope3d(int32_t nelmt, int32_t nnode, int32_t neq, const double *xc,
     fmax_out[s] = fmax;
     nrun = s + 1;
     if (x_out) memcpy(x_out, x, sizeof(double) * n1);
+    if (stress_out) memcpy(stress_out, stress, sizeof(double) * 6 * NG * (size_t)nelmt);
     if (nl_iter == nl_maxiter) break;
   }
 done:
@@ -835,6 +836,53 @@ done:
   free(x); free(oldx); free(evpt); free(stress); free(cohf); free(nuf);
   free(phif); free(psif);
   return rc < 0 ? rc : nrun;
+}
+
+/* ------------------------------------------------------------------------ */
+/* nodal result fields (src/slope.F90:203-210, 418-437;                     */
+/*                      src/postprocess.f90:146-183)                        */
+/* ------------------------------------------------------------------------ */
+
+void orc_nodal_fields(int32_t nelmt, int32_t nnode, const int32_t *g_num,
+                      const double *stress, double *sig, double *psig,
+                      double *nsig) {
+  int32_t *val = (int32_t *)calloc((size_t)nnode, sizeof(int32_t));
+  /* node_valency (src/slope.F90:205-210) */
+  for (int64_t q = 0; q < (int64_t)nelmt * NG; q++) val[g_num[q]]++;
+  /* stress_global: element order, then the mean (src/slope.F90:424-437) */
+  memset(sig, 0, sizeof(double) * NST * (size_t)nnode);
+  for (int64_t e = 0; e < nelmt; e++)
+    for (int i = 0; i < NG; i++) {
+      const int64_t n = g_num[e * NG + i];
+      for (int c = 0; c < NST; c++)
+        sig[n * NST + c] = sig[n * NST + c] + stress[(e * NG + i) * NST + c];
+    }
+  for (int64_t n = 0; n < nnode; n++)
+    for (int c = 0; c < NST; c++) sig[n * NST + c] = sig[n * NST + c] / (double)val[n];
+  /* principal stresses, ascending, and the normal stress on the plane of
+   * maximum shear (src/postprocess.f90:146-183, rquick_sort
+   * src/math_library.f90:211-228) */
+  const double two_third = 2.0 / 3.0;
+  for (int64_t n = 0; n < nnode; n++) {
+    double sigm, dsbar, theta;
+    orc_stress_invariant(sig + n * NST, &sigm, &dsbar, &theta);
+    const double a = two_third * dsbar;
+    double p[3] = {sigm + a * sin(theta - two_third * PI_REF), sigm + a * sin(theta),
+                   sigm + a * sin(theta + two_third * PI_REF)};
+    for (int i = 1; i < 3; i++) {
+      const double t = p[i];
+      int j = i - 1;
+      while (j >= 0 && p[j] > t) {
+        p[j + 1] = p[j];
+        j--;
+      }
+      p[j + 1] = t;
+    }
+    if (psig)
+      for (int k = 0; k < 3; k++) psig[n * 3 + k] = p[k];
+    if (nsig) nsig[n] = 0.5 * (p[0] + p[2]);
+  }
+  free(val);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -132,7 +132,8 @@ double orc_dt_viscoplas_mask(int nmat, const double *nuf, const double *phif,
  * prepared mesh.  Per SRF s writes sum_cg[s], sum_nl[s], uxmax[s], umax[s],
  * fmax[s]; returns the number of SRFs run (the loop stops after the first
  * SRF with nl_iter == nl_maxiter).  x_out (may be NULL) receives the final
- * displacement of the last SRF run; cg_log (may be NULL, capacity
+ * displacement of the last SRF run, stress_out (may be NULL, (E,27,6))
+ * its stress_elmt; cg_log (may be NULL, capacity
  * cg_log_cap) receives cg_iter of every PCG call, *n_cg_log the count.
  * gdof_node[c + 3*n] is gdof(c+1, n+1) (0 = constrained).
  * Returns a negative value if a PCG call fails (errcode -1). */
@@ -145,7 +146,16 @@ int orc_slope3d(int32_t nelmt, int32_t nnode, int32_t neq, const double *xc,
                 int32_t nl_maxiter, int nsrf, const double *srf,
                 int32_t *sum_cg, int32_t *sum_nl, double *uxmax, double *umax,
                 double *fmax, double *x_out, int32_t *cg_log,
-                int32_t cg_log_cap, int32_t *n_cg_log);
+                int32_t cg_log_cap, int32_t *n_cg_log, double *stress_out);
+
+/* Nodal result fields of a trial's final stress state (src/slope.F90:
+ * 203-210, 418-437; src/postprocess.f90:146-183): stress(E,27,6) is
+ * stress_elmt; sig[n*6+c] the nodal mean in element order, psig[n*3+k]
+ * the principal stresses ascending, nsig[n] = (psig1 + psig3) / 2 (psig,
+ * nsig may be NULL).  g_num[e*27+i] are 0-based node ids. */
+void orc_nodal_fields(int32_t nelmt, int32_t nnode, const int32_t *g_num,
+                      const double *stress, double *sig, double *psig,
+                      double *nsig);
 
 #ifdef __cplusplus
 }

@@ -29,7 +29,7 @@ from ._lib import check, f64, i32, ptr
 from .mesh import HexMesh
 from .plastic import dt_viscoplas, strength_reduction
 from .slope import SlopeResult, uxmax_umax
-from .solver import K_F64, Context, DeviceMesh
+from .solver import K_F64, Context, DeviceMesh, nodal_fields_call
 from .spec import SpecMesh, hex2spec_device
 
 
@@ -94,6 +94,14 @@ class BatchEngine:
             ptr(ec, C.c_int32), C.byref(memo)), "sgeo_batch_advance")
         return dict(ended=ended, nl_tot=nl, cg_tot=cg, fmax=fm, errcode=ec,
                     memo_ready=bool(memo.value))
+
+    def nodal_fields(self, col: int) -> Dict[str, np.ndarray]:
+        """The nodal result fields of the trial held by column `col`,
+        computed on the device (sgeo_batch_nodal_fields)."""
+        self.dmesh.ensure_nodes()
+        return nodal_fields_call(L.lib().sgeo_batch_nodal_fields,
+                                 "sgeo_batch_nodal_fields", self.dmesh.spec.nnode,
+                                 self.h, int(col))
 
     def set_geometry(self, blocks_per_cu: int, xcd_split: bool):
         check(L.lib().sgeo_batch_set_geometry(self.h, int(blocks_per_cu), int(xcd_split)),
@@ -193,7 +201,12 @@ def run_analyses(engine, analyses: List[Analysis], *, ym, nu, cg_tol: float,
     nl_maxiter), the analysis's trials at higher SRFs are released and none
     is started above it until it ends; if it converges after all, they run
     again from the start.  A trial's results do not depend on when it runs,
-    so rows and brackets are unchanged.  Returns the engine's statistics."""
+    so rows and brackets are unchanged.
+
+    `on_trial(analysis, srf_index, x, flags, fields)` is called as each
+    trial ends, while its column still holds the trial's state; `fields()`
+    computes that state's nodal result fields.  Returns the engine's
+    statistics."""
     spec = engine.dmesh.spec
     ncol = engine.ncol
     cols: List[Optional[tuple]] = [None] * ncol  # (analysis, srf index)
@@ -326,8 +339,8 @@ def run_analyses(engine, analyses: List[Analysis], *, ym, nu, cg_tol: float,
                              fmax=float(r["fmax"][j]))
             if keep_state:
                 a.flags_last, a.x_last = flags, x
-            if on_trial:
-                on_trial(a, k, x, flags)
+            if on_trial:  # the column still holds the trial's state
+                on_trial(a, k, x, flags, lambda j=j: engine.nodal_fields(j))
             if r["nl_tot"][j] >= nl_maxiter and (a.fail_idx is None or k < a.fail_idx):
                 a.fail_idx = k
             engine.release(j)
@@ -352,11 +365,11 @@ def _trial_sink(on_srf, output):
     if on_srf is None and output is None:
         return None
 
-    def sink(an, k, x, flags):
+    def sink(an, k, x, flags, fields):
         if on_srf:
             on_srf(an.rows[k], x, flags)
         if output is not None:
-            output.srf_done(k + 1, an.rows[k], x)
+            output.srf_done(k + 1, an.rows[k], x, fields=fields)
     return sink
 
 
@@ -371,8 +384,8 @@ def slope3d_mixed(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int
     bracket are those of the sequential sweep.  `on_srf` is called with
     (row, x, plastic flags) as each SRF trial ends (in completion order),
     `progress` with the engine statistics now and then; `output`
-    (output.SlopeOutput) receives the rows and displacements for the
-    reference's result files."""
+    (output.SlopeOutput) receives the rows, displacements and nodal fields
+    for the reference's result files."""
     if phinu:
         # src/plastic_library.f90:72-86 changes nu with the SRF, and with it
         # the stiffness (src/slope.F90:259-265): trials no longer share one

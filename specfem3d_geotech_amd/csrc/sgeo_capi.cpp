@@ -113,6 +113,10 @@ struct sgeo_mesh {
   float* d_KP32 = nullptr;
   int k_ver = 0, k32_ver = -1, kp64_ver = -1, kp32_ver = -1;
   double* d_dinv = nullptr;
+  // node map of the nodal result fields (sgeo_mesh_set_nodes)
+  int32_t nnode = 0;
+  int32_t* d_nrowptr = nullptr;
+  int32_t* d_nslots = nullptr;
   // work vectors
   double* d_Y = nullptr;
   double *d_r = nullptr, *d_z = nullptr, *d_p = nullptr, *d_kp = nullptr;
@@ -318,10 +322,78 @@ extern "C" int sgeo_mesh_destroy(sgeo_mesh* m) {
   void* ptrs[] = {m->d_gdof, m->d_rowptr, m->d_slots, m->d_corners, m->d_mat,
                   m->d_K64,  m->d_K32,    m->d_KP64,  m->d_KP32,
                   m->d_dinv,  m->d_Y,       m->d_r,
-                  m->d_z,    m->d_p,      m->d_kp};
+                  m->d_z,    m->d_p,      m->d_kp,    m->d_nrowptr, m->d_nslots};
   for (void* p : ptrs)
     if (p) hipFree(p);
   delete m;
+  return 0;
+}
+
+// Node map of the nodal result fields: for every node, its (element, GLL
+// point) slots e*27+i in ascending element order (counting sort, stable),
+// so that device sums over a node's elements run in the reference's order
+// (src/slope.F90:424-430).
+extern "C" int sgeo_mesh_set_nodes(sgeo_mesh* m, int32_t nnode, const int32_t* g_num) {
+  if (!m || !g_num || nnode <= 0)
+    return fail(SGEO_EARG, "sgeo_mesh_set_nodes: bad arguments");
+  const int64_t ns = (int64_t)m->nelmt * 27;
+  std::vector<int32_t> rowptr((size_t)nnode + 1, 0);
+  for (int64_t q = 0; q < ns; q++) {
+    const int32_t n = g_num[q];
+    if (n < 0 || n >= nnode)
+      return fail(SGEO_EARG, "g_num[%lld]=%d outside 0..%d", (long long)q, n, nnode - 1);
+    rowptr[(size_t)n + 1]++;
+  }
+  for (size_t n = 1; n < rowptr.size(); n++) rowptr[n] += rowptr[n - 1];
+  std::vector<int32_t> slots((size_t)ns), next(rowptr.begin(), rowptr.end() - 1);
+  for (int64_t q = 0; q < ns; q++) slots[(size_t)next[(size_t)g_num[q]]++] = (int32_t)q;
+  HIPCHK(hipSetDevice(m->ctx->device));
+  hipStream_t s = m->ctx->stream;
+  HIPCHK(hipStreamSynchronize(s));
+  if (m->d_nrowptr) hipFree(m->d_nrowptr);
+  if (m->d_nslots) hipFree(m->d_nslots);
+  m->d_nrowptr = m->d_nslots = nullptr;
+  m->nnode = 0;
+  TRY(dalloc(&m->d_nrowptr, rowptr.size(), &m->bytes));
+  TRY(dalloc(&m->d_nslots, (size_t)ns, &m->bytes));
+  HIPCHK(hipMemcpyAsync(m->d_nrowptr, rowptr.data(), rowptr.size() * 4,
+                        hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(m->d_nslots, slots.data(), (size_t)ns * 4, hipMemcpyHostToDevice,
+                        s));
+  HIPCHK(hipStreamSynchronize(s));  // host vectors die at return
+  m->nnode = nnode;
+  return 0;
+}
+
+// Nodal result fields of one trial's stress state (device, FP64 [e][27][6])
+// and, when given, its per-point plastic strain; FP32 outputs in component
+// blocks, any of them may be NULL.  vmeps without a per-point source is
+// zero: the slope driver never accumulates it (src/slope.F90:366-368).
+static int nodal_fields(sgeo_mesh* m, const double* d_stress, const double* d_vm,
+                        float* sig, float* psig, float* nsig, float* vmeps) {
+  if (!m->d_nrowptr)
+    return fail(SGEO_ESTATE, "no node map: call sgeo_mesh_set_nodes first");
+  const size_t nn = (size_t)m->nnode;
+  HIPCHK(hipSetDevice(m->ctx->device));
+  hipStream_t s = m->ctx->stream;
+  float* d_out = nullptr;
+  TRY(dalloc(&d_out, nn * 11));
+  float *ds = d_out, *dp = d_out + 6 * nn, *dn = d_out + 9 * nn, *dv = d_out + 10 * nn;
+  launch_nodal_fields(s, m->nnode, m->d_nrowptr, m->d_nslots, d_stress, d_vm,
+                      sig ? ds : nullptr, psig ? dp : nullptr, nsig ? dn : nullptr,
+                      vmeps && d_vm ? dv : nullptr);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && sig)
+    e = hipMemcpyAsync(sig, ds, nn * 6 * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && psig)
+    e = hipMemcpyAsync(psig, dp, nn * 3 * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && nsig) e = hipMemcpyAsync(nsig, dn, nn * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && vmeps && d_vm)
+    e = hipMemcpyAsync(vmeps, dv, nn * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  hipFree(d_out);
+  if (e != hipSuccess) return fail(SGEO_EHIP, "nodal fields: %s", hipGetErrorString(e));
+  if (vmeps && !d_vm) std::memset(vmeps, 0, nn * 4);
   return 0;
 }
 
@@ -1069,6 +1141,13 @@ extern "C" int sgeo_slope_download(sgeo_slope* t, double* x, double* bodyload,
   return 0;
 }
 
+extern "C" int sgeo_slope_nodal_fields(sgeo_slope* t, float* stress, float* psigma,
+                                       float* nsigma, float* vmeps) {
+  if (!t) return fail(SGEO_EARG, "null slope");
+  return nodal_fields(t->mesh, t->d_stress, t->excav ? t->d_vm : nullptr, stress, psigma,
+                      nsigma, vmeps);
+}
+
 // --------------------------------------------------------------------------
 // multi-trial mixed-precision engine
 // --------------------------------------------------------------------------
@@ -1733,6 +1812,14 @@ extern "C" int sgeo_batch_download(sgeo_batch* b, int32_t col, double* x,
                           (size_t)m->nelmt * 27, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   return 0;
+}
+
+extern "C" int sgeo_batch_nodal_fields(sgeo_batch* b, int32_t col, float* stress,
+                                       float* psigma, float* nsigma, float* vmeps) {
+  if (!b || col < 0 || col >= b->B) return fail(SGEO_EARG, "bad column");
+  const size_t ns = (size_t)b->mesh->nelmt * 27 * 6;
+  return nodal_fields(b->mesh, b->d_stress + (size_t)col * ns, nullptr, stress, psigma,
+                      nsigma, vmeps);
 }
 
 extern "C" int sgeo_batch_stats(sgeo_batch* b, int64_t* inner_iters,

@@ -100,6 +100,9 @@ void launch_elastic_stress(hipStream_t s, int32_t nelmt, const double* corners,
 void launch_excav_load(hipStream_t s, int32_t nelmt, const double* corners,
                        const int32_t* mat_id, const double* gam, const double* stress,
                        double* Y);
+void launch_nodal_fields(hipStream_t s, int32_t nnode, const int32_t* rowptr,
+                         const int32_t* slots, const double* stress, const double* vm,
+                         float* sig, float* psig, float* nsig, float* vmo);
 
 // ---- multi-trial mixed-precision engine (sgeo_batch.hip) -----------------
 // Per-column states of the mixed-precision PCG.  The inner iteration runs in

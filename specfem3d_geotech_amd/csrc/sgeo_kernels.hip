@@ -395,6 +395,71 @@ void launch_elastic_stress(hipStream_t s, int32_t nelmt, const double* corners,
                      0, s, nelmt, corners, gdof, mat_id, ym, nu, x, stress);
 }
 
+// --------------------------------------------------------------------------
+// nodal result fields (src/slope.F90:418-437, src/postprocess.f90:146-183):
+// thread per node.  The point values of the elements at the node (`slots`
+// = e*27+i, ascending in e) are summed from zero in that order and divided
+// by the node's valency in FP64; the principal stresses follow from the
+// invariants, ascending, and the normal stress on the plane of maximum
+// shear is their mean.  Outputs are rounded to FP32, component blocks of
+// nnode values, as the EnSight writers store them.  Any output may be null.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_nodal_fields(
+    int32_t nnode, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ slots,
+    const double* __restrict__ stress, const double* __restrict__ vm,
+    float* __restrict__ sig, float* __restrict__ psig, float* __restrict__ nsig,
+    float* __restrict__ vmo) {
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (n >= nnode) return;
+  const int32_t q0 = rowptr[n], q1 = rowptr[n + 1];
+  const double val = (double)(q1 - q0);  // node_valency
+  if (vm && vmo) {
+    double v = 0.0;
+    for (int32_t q = q0; q < q1; q++) v = v + vm[slots[q]];
+    vmo[n] = (float)(v / val);
+  }
+  if (!stress || !(sig || psig || nsig)) return;
+  double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int32_t q = q0; q < q1; q++) {
+    const double* p = stress + (int64_t)slots[q] * 6;
+#pragma unroll
+    for (int c = 0; c < 6; c++) s[c] = s[c] + p[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 6; c++) s[c] = s[c] / val;
+  if (sig)
+#pragma unroll
+    for (int c = 0; c < 6; c++) sig[(int64_t)c * nnode + n] = (float)s[c];
+  if (!(psig || nsig)) return;
+  double sigm, dsbar, theta;
+  stress_invariant(s, sigm, dsbar, theta);
+  const double two_third = 2.0 / 3.0;
+  const double pi = 3.141592653589793;  // src/global.f90:19
+  const double a = two_third * dsbar;
+  double ps[3] = {sigm + a * sin(theta - two_third * pi), sigm + a * sin(theta),
+                  sigm + a * sin(theta + two_third * pi)};
+  for (int i = 1; i < 3; i++) {  // insertion sort, ascending (src/math_library.f90:211-228)
+    const double t = ps[i];
+    int j = i - 1;
+    while (j >= 0 && ps[j] > t) {
+      ps[j + 1] = ps[j];
+      j--;
+    }
+    ps[j + 1] = t;
+  }
+  if (psig)
+    for (int k = 0; k < 3; k++) psig[(int64_t)k * nnode + n] = (float)ps[k];
+  if (nsig) nsig[n] = (float)(0.5 * (ps[0] + ps[2]));
+}
+
+void launch_nodal_fields(hipStream_t s, int32_t nnode, const int32_t* rowptr,
+                         const int32_t* slots, const double* stress, const double* vm,
+                         float* sig, float* psig, float* nsig, float* vmo) {
+  if (nnode <= 0) return;
+  hipLaunchKernelGGL(k_nodal_fields, dim3(nblk(nnode, 256)), dim3(256), 0, s, nnode,
+                     rowptr, slots, stress, vm, sig, psig, nsig, vmo);
+}
+
 void launch_excav_load(hipStream_t s, int32_t nelmt, const double* corners,
                        const int32_t* mat_id, const double* gam, const double* stress,
                        double* Y) {

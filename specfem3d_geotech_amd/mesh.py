@@ -220,10 +220,13 @@ def write_reference_input(mesh: HexMesh, out_dir: str, head: str, *,
                           nl_maxiter: int, srf: Sequence[float],
                           out_path: Optional[str] = None,
                           extra_control: str = "",
-                          extra_lines: str = "", save_disp: bool = False) -> str:
+                          extra_lines: str = "", save_disp: bool = False,
+                          save: Sequence[str] = ()) -> str:
     """Write `mesh` plus a `.sem` control file readable by the reference's
     `semgeotech` (serial); `save_disp` asks it for its EnSight displacement
-    output.  Returns the `.sem` path."""
+    output, `save` for any of its result fields by their `save:` keys
+    (disp, stress, psigma, nsigma, scf, vmeps; src/read_input.f90:380-403).
+    Returns the `.sem` path."""
     os.makedirs(out_dir, exist_ok=True)
     out_path = out_path or os.path.join(out_dir, "output")
     os.makedirs(out_path, exist_ok=True)
@@ -269,7 +272,10 @@ def write_reference_input(mesh: HexMesh, out_dir: str, head: str, *,
                  f"srf={srf_s}{extra_control}\n")
         if extra_lines:
             fh.write(extra_lines)
-        fh.write(f"save: disp={1 if save_disp else 0}\n")
+        keys = list(save) or ["disp"]
+        flags = {k: 1 for k in save}
+        flags.setdefault("disp", 1 if save_disp else 0)
+        fh.write("save: " + ", ".join(f"{k}={flags[k]}" for k in keys) + "\n")
     return sem
 
 

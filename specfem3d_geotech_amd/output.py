@@ -4,10 +4,13 @@
   414-416, src/semgeotech.F90:341-347): list-directed lines as the
   reference's compiler writes them -- `fortran_list` reproduces its
   records (at most 79 columns) and its number forms.
-* EnSight Gold, C Binary (src/semgeotech.F90:138-313, src/visual.f90:17-231,
-  src/postprocess.f90:185-192): the `_original` case and geometry of the hex8
-  mesh, the case file of the sweep, the GLL geometry (every 27-node element
-  as 8 hexa8 cells) and one FP32 displacement file per SRF.
+* EnSight Gold, C Binary (src/semgeotech.F90:138-313, src/visual.f90:17-271,
+  src/postprocess.f90:128-248): the `_original` case and geometry of the
+  hex8 mesh, the case file of the sweep, the GLL geometry (every 27-node
+  element as 8 hexa8 cells) and per SRF one FP32 file per saved field:
+  displacement, nodal stress, principal stresses, normal stress, stress
+  concentration factor and plastic strain.  The stress-derived fields are
+  computed on the device (sgeo_*_nodal_fields); this module only writes.
 
 `SlopeOutput` writes these beside a sweep; file writes run on a background
 thread, so they overlap the next SRF's GPU work.
@@ -166,10 +169,41 @@ def geo_gll(spec: SpecMesh) -> bytes:
 
 def pernode_vector(destag: str, var: np.ndarray) -> bytes:
     """write_ensight_pernodeVECAS (src/visual.f90:191-231): FP32 components
-    as blocks."""
-    v = np.asarray(var, dtype=np.float64).astype(np.float32)   # real(nodalu)
+    as blocks; var is (nnode, ncomp), FP64 (rounded here, as real() does)
+    or already FP32."""
+    v = np.asarray(var)
+    v = v.astype(np.float32) if v.dtype != np.float32 else v
     return b"".join([_s80(destag), _s80("part"), _i32(1), _s80("coordinates"),
                      np.ascontiguousarray(v.T).tobytes()])
+
+
+def pernode_scalar(destag: str, var: np.ndarray) -> bytes:
+    """write_ensight_pernodeSCALAS (src/visual.f90:236-271): one FP32
+    value per node."""
+    v = np.asarray(var)
+    v = v.astype(np.float32) if v.dtype != np.float32 else v
+    return b"".join([_s80(destag), _s80("part"), _i32(1), _s80("coordinates"),
+                     np.ascontiguousarray(v).tobytes()])
+
+
+# Result fields of a sweep in the reference's case-file order
+# (src/semgeotech.F90:216-249): save key, case line type, variable name,
+# file suffix, description written into the file (src/postprocess.f90:185-247).
+# porep (no pore pressure here) and maxtau (not settable from the input
+# file, src/read_input.f90:380-403) are not written by the reference's
+# slope runs either.
+FIELDS = [
+    ("disp", "vector per node", "displacement", "dis", "Displacement field"),
+    ("stress", "tensor symm per node", "stress", "sig", "Effective stress tensor"),
+    ("psigma", "vector per node", "principal_stress", "psig", "principal stresses"),
+    ("vmeps", "scalar per node", "plastic_strain", "eps",
+     "von Mises effective plastic strain"),
+    ("scf", "scalar per node", "stress_concentration_factor", "scf",
+     "principal stress concentration factor"),
+    ("nsigma", "scalar per node", "normal_stress", "nsig", "normal stress"),
+]
+SAVE_KEYS = tuple(f[0] for f in FIELDS)
+INFTOL = 1.0e32  # src/global.f90:22: scf of a slope trial (src/slope.F90:249)
 
 
 def step_width(nt: int) -> int:
@@ -198,17 +232,20 @@ def _e12_5(v: float) -> str:
     return txt.rjust(12)
 
 
-def case_slope(head: str, srf: Sequence[float], disp: bool = True, ptail: str = "") -> str:
-    """The sweep's case file (src/semgeotech.F90:196-266): time values are
-    the SRFs."""
+def case_slope(head: str, srf: Sequence[float], disp: bool = True, ptail: str = "",
+               save: Optional[Sequence[str]] = None) -> str:
+    """The sweep's case file (src/semgeotech.F90:196-266): one variable line
+    per saved field, time values are the SRFs.  `save` lists the saved
+    fields by key (default: the displacement when `disp`)."""
+    save = _save_set(save, disp)
     nt = len(srf)
     tw = step_width(nt)
     s = "FORMAT\ntype:  ensight gold\n\nGEOMETRY\n"
     s += f"model:    {head}{ptail}.geo\n\n"
     s += "VARIABLE\n"
-    if disp:
-        s += (f"vector per node: {1:10d} displacement "
-              f"{head}_step{'*' * tw}{ptail}.dis\n\n")
+    for key, kind, name, suffix, _ in FIELDS:
+        if key in save:
+            s += f"{kind}: {1:10d} {name} {head}_step{'*' * tw}{ptail}.{suffix}\n\n"
     s += "TIME\n"
     s += f"time set:{1:10d}\n"
     s += f"number of steps:{nt:10d}\n"
@@ -217,6 +254,14 @@ def case_slope(head: str, srf: Sequence[float], disp: bool = True, ptail: str = 
     s += "time values: "
     s += "".join(_e12_5(float(v)) + "\n" for v in srf)
     return s
+
+
+def _save_set(save: Optional[Sequence[str]], disp: bool) -> set:
+    out = set(save) if save is not None else ({"disp"} if disp else set())
+    bad = out - set(SAVE_KEYS)
+    if bad:
+        raise ValueError(f"unknown save fields {sorted(bad)}; known: {SAVE_KEYS}")
+    return out
 
 
 def nodal_displacement(spec: SpecMesh, x: np.ndarray) -> np.ndarray:
@@ -231,15 +276,21 @@ def nodal_displacement(spec: SpecMesh, x: np.ndarray) -> np.ndarray:
 # ----------------------------------------------------------------------------
 
 class SlopeOutput:
-    """The files the reference writes for an SRF sweep (save: disp=1), in
-    `out_path` with file head `head`.  Writes run on a background thread;
+    """The files the reference writes for an SRF sweep, in `out_path` with
+    file head `head`.  `save` lists the result fields as the input file's
+    `save:` keys (disp, stress, psigma, nsigma, scf, vmeps); the default is
+    the displacement alone (`disp`).  Writes run on a background thread;
     `close` waits for them and appends the elapsed-time lines."""
 
     def __init__(self, out_path: str, head: str, mesh: HexMesh, spec: SpecMesh,
-                 control: Dict, srf: Sequence[float], disp: bool = True):
+                 control: Dict, srf: Sequence[float], disp: bool = True,
+                 save: Optional[Sequence[str]] = None):
         self.out_path, self.head, self.spec = out_path, head, spec
         self.srf = list(srf)
-        self.disp = disp
+        self.save = _save_set(save, disp)
+        self.disp = "disp" in self.save
+        # fields that need the trial's stress state
+        self.wants_fields = bool(self.save & {"stress", "psigma", "nsigma"})
         self.nl_maxiter = int(control["nl_maxiter"])
         self._pending: Dict[int, tuple] = {}
         self._next = 1
@@ -253,7 +304,7 @@ class SlopeOutput:
         p = self.path
         self._put(p(f"{head}_original.case"), case_original(head).encode())
         self._put(p(f"{head}_original.geo"), geo_original(mesh))
-        self._put(p(f"{head}.case"), case_slope(head, self.srf, disp).encode())
+        self._put(p(f"{head}.case"), case_slope(head, self.srf, save=self.save).encode())
         self._put(p(f"{head}.geo"), geo_gll(spec))
         summ = (fortran_list("--------------------------------------------")
                 + fortran_list("Result summary produced by SPECFEM3D_GEOTECH")
@@ -285,27 +336,50 @@ class SlopeOutput:
             except BaseException as e:  # reported at close
                 self._err = e
 
-    def srf_done(self, i_srf: int, row: Dict, x: Optional[np.ndarray]):
-        """Summary row of SRF number i_srf (1-based) and its displacement
-        file (src/slope.F90:414-416, 439).  Trials may end in any order (SRFs
-        run in parallel); rows are written in SRF order, and nothing after
-        the first SRF whose plastic loop reached nl_maxiter, where the
-        reference's sweep stops (src/slope.F90:441)."""
-        self._pending[i_srf] = (row, x if self.disp else None)
+    def srf_done(self, i_srf: int, row: Dict, x: Optional[np.ndarray], fields=None):
+        """Summary row of SRF number i_srf (1-based) and its field files
+        (src/slope.F90:414-439, src/postprocess.f90:128-248).  `fields` is
+        the trial's nodal result fields (stress, psigma, nsigma: a dict, or
+        a callable returning one, called here only if a saved field needs
+        it).  Trials may end in any order (SRFs run in parallel); rows are
+        written in SRF order, and nothing after the first SRF whose plastic
+        loop reached nl_maxiter, where the reference's sweep stops
+        (src/slope.F90:441)."""
+        f = None
+        if self.wants_fields:
+            if fields is None:
+                raise ValueError("the saved fields need the trial's nodal fields")
+            f = fields() if callable(fields) else fields
+        self._pending[i_srf] = (row, x if self.disp else None, f)
         while self._next in self._pending and not self._stopped:
-            row, x = self._pending.pop(self._next)
+            row, x, f = self._pending.pop(self._next)
             self._put(self._summary, fortran_list(
                 float(row["srf"]), int(row["cg_tot"]), int(row["nl_tot"]),
                 float(row["uxmax"]), float(row["umax"]), float(row["fmax"])).encode(),
                 append=True)
-            if x is not None:
-                tw = step_width(len(self.srf))
-                name = f"{self.head}_step{self._next:0{tw}d}.dis"
-                u = nodal_displacement(self.spec, x)
-                self._put(self.path(name), pernode_vector("Displacement field", u))
+            self._write_fields(self._next, x, f)
             if int(row["nl_tot"]) >= self.nl_maxiter:
                 self._stopped = True
             self._next += 1
+
+    def _write_fields(self, step: int, x, f):
+        tw = step_width(len(self.srf))
+        name = lambda suffix: self.path(f"{self.head}_step{step:0{tw}d}.{suffix}")
+        nn = self.spec.nnode
+        for key, _, _, suffix, destag in FIELDS:
+            if key not in self.save:
+                continue
+            if key == "disp":
+                data = pernode_vector(destag, nodal_displacement(self.spec, x))
+            elif key in ("stress", "psigma"):
+                data = pernode_vector(destag, f[key])
+            elif key == "nsigma":
+                data = pernode_scalar(destag, f["nsigma"])
+            elif key == "scf":  # never updated by the slope driver (src/slope.F90:249, 376)
+                data = pernode_scalar(destag, np.full(nn, INFTOL))
+            else:  # vmeps: not accumulated by the slope driver (src/slope.F90:366-368)
+                data = pernode_scalar(destag, np.zeros(nn))
+            self._put(name(suffix), data)
 
     def close(self):
         """Elapsed-time lines (src/semgeotech.F90:341-347), then waits for

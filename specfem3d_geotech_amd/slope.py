@@ -17,7 +17,7 @@ from . import _lib as L
 from ._lib import check, f64, i32, ptr
 from .mesh import HexMesh
 from .plastic import dt_viscoplas, strength_reduction
-from .solver import K_F64, Context, DeviceMesh
+from .solver import K_F64, Context, DeviceMesh, nodal_fields_call
 from .spec import SpecMesh, hex2spec_device
 
 
@@ -74,6 +74,13 @@ class DeviceSlope:
               "sgeo_slope_download")
         return dict(x=x, bodyload=body, evpt=evpt, stress=stress, flags=flags)
 
+    def nodal_fields(self) -> Dict[str, np.ndarray]:
+        """The nodal result fields of the trial's stress state, computed on
+        the device (sgeo_slope_nodal_fields)."""
+        self.dmesh.ensure_nodes()
+        return nodal_fields_call(L.lib().sgeo_slope_nodal_fields,
+                                 "sgeo_slope_nodal_fields", self.dmesh.spec.nnode, self.h)
+
     def close(self):
         if self.h:
             L.lib().sgeo_slope_destroy(self.h)
@@ -120,7 +127,8 @@ def slope3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
     given, is called with each summary row, `iter_progress` with (nl_iter,
     cg_iter) after each plastic iteration, `on_srf` with (row, x, plastic
     flags) at the end of each SRF.  `output` (output.SlopeOutput) receives
-    each SRF's row and displacement for the reference's result files."""
+    each SRF's row, displacement and nodal fields for the reference's
+    result files."""
     spec = spec or hex2spec_device(ctx, mesh)
     gam, ym, nu, phi, coh, psi = mesh.material_arrays()
     dm = DeviceMesh(ctx, spec, len(gam))
@@ -167,7 +175,7 @@ def slope3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
             if on_srf:
                 on_srf(rows[-1], state["x"], state["flags"])
             if output is not None:
-                output.srf_done(len(rows), rows[-1], state["x"])
+                output.srf_done(len(rows), rows[-1], state["x"], fields=st.nodal_fields)
             if nl_iter == nl_maxiter:  # src/slope.F90:441
                 bracket = (prev, float(s))
                 break

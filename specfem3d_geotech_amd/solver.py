@@ -8,7 +8,7 @@ libsgeo.so on the GPU.
 from __future__ import annotations
 
 import ctypes as C
-from typing import Optional, Tuple
+from typing import Dict, Optional, Tuple
 
 import numpy as np
 
@@ -105,6 +105,19 @@ def pcg_solver(ctx: Context, neq: int, nelmt: int, k: np.ndarray, u: np.ndarray,
     return u, it.value, ec.value, tag.raw[:250].decode().rstrip()
 
 
+def nodal_fields_call(fn, what: str, nnode: int, *args) -> Dict[str, np.ndarray]:
+    """Runs one of the sgeo_*_nodal_fields entry points: FP32 stress
+    (nnode, 6), psigma (nnode, 3), nsigma and vmeps (nnode,), as the
+    reference's EnSight files hold them."""
+    sig = np.zeros((6, nnode), np.float32)
+    ps = np.zeros((3, nnode), np.float32)
+    ns = np.zeros(nnode, np.float32)
+    vm = np.zeros(nnode, np.float32)
+    fp = lambda a: ptr(a, C.c_float)
+    check(fn(*args, fp(sig), fp(ps), fp(ns), fp(vm)), what)
+    return dict(stress=sig.T, psigma=ps.T, nsigma=ns, vmeps=vm)
+
+
 class DeviceMesh:
     """A mesh resident in HBM (sgeo_mesh): gdof_elmt, corners, materials and
     the element stiffness."""
@@ -132,6 +145,14 @@ class DeviceMesh:
         check(L.lib().sgeo_mesh_solve(self.h, ptr(u), ptr(f64(f)), cg_tol, cg_maxiter,
                                       C.byref(it), C.byref(ec)), "sgeo_mesh_solve")
         return u, it.value, ec.value
+
+    def ensure_nodes(self):
+        """The node map of the nodal result fields (built once)."""
+        if not getattr(self, "_nodes_set", False):
+            g = i32(self.spec.g_num)
+            check(L.lib().sgeo_mesh_set_nodes(self.h, self.spec.nnode, ptr(g, C.c_int32)),
+                  "sgeo_mesh_set_nodes")
+            self._nodes_set = True
 
     def elastic_stress(self, ym, nu, x):
         """(nelmt, 27, 6) elastic stress at the GLL points from x."""

@@ -274,17 +274,21 @@ def list_format_case():
     print("list_format", vals.size, "values,", m, "rows")
 
 
+SAVE_ALL = ("disp", "stress", "psigma", "nsigma", "scf", "vmeps")
+
+
 def output_case():
     """The reference's own output files for a two-SRF sweep of the 4x2x3
-    slope with displacement output (save: disp=1): the EnSight Gold case and
-    geometry files of the hex8 mesh and of the GLL mesh, one displacement
-    file per SRF, and the summary file (src/semgeotech.F90:138-322,
-    src/postprocess.f90:128-192, src/visual.f90:17-231, src/slope.F90:214-220,
-    414-416)."""
+    slope with every result field it can be asked for (save: disp, stress,
+    psigma, nsigma, scf, vmeps): the EnSight Gold case and geometry files
+    of the hex8 mesh and of the GLL mesh, the field files per SRF, and the
+    summary file (src/semgeotech.F90:138-322, src/postprocess.f90:128-248,
+    src/visual.f90:17-271, src/slope.F90:214-220, 414-439)."""
     mesh = slope_mesh(4, 2, 3)
     work = tempfile.mkdtemp(prefix="golden_out_")
     sem = write_reference_input(mesh, work, "slope_out", cg_tol=1e-8, cg_maxiter=5000,
-                                nl_tol=5e-4, nl_maxiter=300, srf=[1.0, 1.2], save_disp=True)
+                                nl_tol=5e-4, nl_maxiter=300, srf=[1.0, 1.2],
+                                save=SAVE_ALL)
     r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {REF_BIN} {sem}"],
                        cwd=work, capture_output=True, text=True)
     outdir = os.path.join(work, "output")

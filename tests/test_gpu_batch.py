@@ -212,3 +212,33 @@ def test_graph_steps_match_eager_launches(ctx):
         print("graphs" if graphs else "eager", a.rows, steps)
     assert out[0][:3] == out[1][:3]
     assert out[0][3] > 100  # many graph replays
+
+
+def test_launch_geometry_does_not_change_bits(ctx):
+    """The MFMA product's launch geometry (blocks per CU, the XCD-aware
+    element split) changes which block computes an element, never how: a
+    64-column engine gives the same rows and displacement bits under each."""
+    z = golden("slope_8x4x6_layered")
+    mesh = mesh_from_golden(z)
+    spec = hex2spec(mesh)
+    ctl = control_from_golden(z)
+    _, ym, nu, *_ = mesh.material_arrays()
+    kw = dict(ym=ym, nu=nu, cg_tol=ctl["cg_tol"], cg_maxiter=ctl["cg_maxiter"],
+              nl_tol=ctl["nl_tol"], nl_maxiter=40, keep_state=True, srf_parallel=2)
+    out = []
+    dm, ext = device_mesh(ctx, mesh, spec)
+    try:
+        for geom in [(3, True), (1, False), (2, True)]:
+            an = _analyses(mesh, 20, [1.0, 1.5], seed=11)
+            eng = BatchEngine(dm, ext, 64)
+            try:
+                eng.set_geometry(*geom)
+                run_analyses(eng, an, **kw)
+            finally:
+                eng.close()
+            print(geom, [a.bracket for a in an[:4]], flush=True)
+            out.append([(a.rows, a.bracket, a.x_last.tobytes()) for a in an])
+    finally:
+        dm.close()
+    for o in out[1:]:
+        assert o == out[0]

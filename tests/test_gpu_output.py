@@ -1,15 +1,18 @@
 """The reference's result files written from the GPU sweeps.
 
-Two-SRF sweep of the 4x2x3 slope with displacement output, against the
-compiled reference's files:
-* the EnSight case and geometry files byte for byte;
+Two-SRF sweep of the 4x2x3 slope with every result field the input file
+can ask for (displacement, nodal stress, principal stresses, normal
+stress, stress concentration factor, plastic strain), against the compiled
+reference's files:
+* the EnSight case and geometry files, and the constant scf and vmeps
+  files, byte for byte;
 * FP64 sweep with reference-order reductions: CG and plastic iteration
   counts exactly, displacements within 1e-10 and fmax within 1e-8 (the
   yield checks' device transcendentals differ from the host libm in the
-  last bit), so each FP32 displacement value equals the reference's or lies
-  one FP32 spacing away;
+  last bit), so each FP32 displacement and stress-field value equals the
+  reference's or lies one FP32 spacing away;
 * mixed-precision sweep (both SRFs at once): the same files in SRF order,
-  displacements within the solver tolerance.
+  displacements and stress fields within the solver tolerance.
 """
 import os
 
@@ -28,6 +31,8 @@ pytestmark = pytest.mark.gpu
 
 CTL = dict(cg_tol=1e-8, cg_maxiter=5000, nl_tol=5e-4, nl_maxiter=300)
 SRF = [1.0, 1.2]
+SAVE_ALL = ("disp", "stress", "psigma", "nsigma", "scf", "vmeps")
+FLOAT_FIELDS = (".dis", ".sig", ".psig", ".nsig")  # computed from the solution
 
 
 def _mesh(z):
@@ -67,7 +72,7 @@ def test_fp64_sweep_files_match_reference(tmp_path):
     try:
         ctx.set_reduction(REDUCE_SEQUENTIAL)
         spec = hex2spec_device(ctx, mesh)
-        out = SlopeOutput(str(tmp_path), "slope_out", mesh, spec, CTL, SRF)
+        out = SlopeOutput(str(tmp_path), "slope_out", mesh, spec, CTL, SRF, save=SAVE_ALL)
         slope3d(ctx, mesh, spec=spec, srf=SRF, output=out, **CTL)
         out.close()
     finally:
@@ -89,7 +94,7 @@ def test_fp64_sweep_files_match_reference(tmp_path):
                     assert abs(a - b) <= 1e-10 * abs(b), (a, b)
                 else:
                     assert a == b
-        elif n.endswith(".dis"):
+        elif n.endswith(FLOAT_FIELDS):
             assert mine[:244] == ref[:244]
             assert _fp32_close(mine[244:], ref[244:]), n
         else:
@@ -102,7 +107,7 @@ def test_mixed_sweep_files(tmp_path):
     ctx = Context(0)
     try:
         spec = hex2spec_device(ctx, mesh)
-        out = SlopeOutput(str(tmp_path), "slope_out", mesh, spec, CTL, SRF)
+        out = SlopeOutput(str(tmp_path), "slope_out", mesh, spec, CTL, SRF, save=SAVE_ALL)
         slope3d_mixed(ctx, mesh, spec=spec, srf=SRF, ncol=2, output=out, **CTL)
         out.close()
     finally:
@@ -111,11 +116,11 @@ def test_mixed_sweep_files(tmp_path):
     assert sorted(os.listdir(tmp_path)) == names
     for n in names:
         ref, mine = z["file:" + n].tobytes(), (tmp_path / n).read_bytes()
-        if n.endswith(".dis"):
-            a = np.frombuffer(mine[244:], np.float32)
-            b = np.frombuffer(ref[244:], np.float32)
+        if n.endswith(FLOAT_FIELDS):
+            a = np.frombuffer(mine[244:], np.float32).astype(np.float64)
+            b = np.frombuffer(ref[244:], np.float32).astype(np.float64)
             assert mine[:244] == ref[:244]
-            assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(b)
+            assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(b), n
         elif n.endswith("_summary"):
             va, vb = _summary_values(mine.decode()), _summary_values(ref.decode())
             assert len(va) == len(vb)
@@ -127,3 +132,49 @@ def test_mixed_sweep_files(tmp_path):
                 assert abs(a[3] - b[3]) <= 1e-7 * b[3] and abs(a[4] - b[4]) <= 1e-7 * b[4]
         else:
             assert mine == ref, n
+
+
+def test_device_nodal_fields_match_oracle():
+    """sgeo_slope_nodal_fields on an injected stress state against the
+    oracle's nodal fields: the nodal stress bit for bit (same sums in the
+    same order, then one rounding to FP32), the principal and normal
+    stresses equal or one FP32 spacing away (device sin), on a mesh with
+    every node valency a slope mesh has and with hydrostatic elements
+    (dsbar at its floor, Lode angle 0)."""
+    import oracle as O
+    from specfem3d_geotech_amd.mesh import slope_mesh
+    from specfem3d_geotech_amd.slope import DeviceSlope
+    from specfem3d_geotech_amd.solver import K_F64, DeviceMesh
+
+    mesh = slope_mesh(10, 3, 6)
+    ctx = Context(0)
+    try:
+        spec = hex2spec_device(ctx, mesh)
+        gam, ym, nu, phi, coh, psi = mesh.material_arrays()
+        rng = np.random.default_rng(3)
+        stress = rng.normal(size=(spec.nelmt, 27, 6)) * 100.0
+        hyd = rng.random(spec.nelmt) < 0.2
+        stress[hyd, :, 3:] = 0.0
+        stress[hyd, :, 1] = stress[hyd, :, 0]
+        stress[hyd, :, 2] = stress[hyd, :, 0]
+        dm = DeviceMesh(ctx, spec, len(gam))
+        try:
+            ext, _ = dm.build_stiffness(gam, ym, nu, kprec=K_F64)
+            st = DeviceSlope(dm, ext)
+            try:
+                st.begin_increment(ym, nu, phi, coh, psi, 1.0, stress0=stress)
+                f = st.nodal_fields()
+            finally:
+                st.close()
+        finally:
+            dm.close()
+    finally:
+        ctx.close()
+    o = O.nodal_fields(spec, stress)
+    print("nnode", spec.nnode, "hydrostatic elements", int(hyd.sum()), flush=True)
+    assert np.array_equal(f["stress"], o["stress"].astype(np.float32))
+    for k in ("psigma", "nsigma"):
+        r = o[k].astype(np.float32)
+        assert np.all(np.abs(f[k].astype(np.float64) - r) <=
+                      np.spacing(np.abs(r)).astype(np.float64)), k
+    assert not np.any(f["vmeps"])

@@ -8,11 +8,13 @@
   character for character; in the rest the compiler picks another last digit
   among the shortest decimal strings that convert back to the same double,
   a choice its runtime does not document.
-* The files of a two-SRF sweep of the 4x2x3 slope with displacement output
-  (tests/golden/output_slope_4x2x3.npz, written by the compiled reference):
-  the EnSight case and geometry files and the FP32 displacement files byte
-  for byte, the summary file value for value, from the oracle's
-  displacements (which equal the reference's bit for bit).
+* The files of a two-SRF sweep of the 4x2x3 slope with every result field
+  the input file can ask for (tests/golden/output_slope_4x2x3.npz, written
+  by the compiled reference): the EnSight case and geometry files and the
+  FP32 displacement, nodal stress, principal stress, normal stress, stress
+  concentration factor and plastic strain files byte for byte, the summary
+  file value for value, from the oracle's displacements and stress state
+  (which equal the reference's bit for bit) and its nodal fields.
 """
 import os
 
@@ -20,9 +22,11 @@ import numpy as np
 
 import oracle as O
 from conftest import golden
-from specfem3d_geotech_amd.mesh import HexMesh, Material
+from specfem3d_geotech_amd.mesh import HexMesh, Material, slope_mesh
 from specfem3d_geotech_amd.output import (SlopeOutput, case_slope, fortran_list,
                                           fortran_real)
+
+SAVE_ALL = ("disp", "stress", "psigma", "nsigma", "scf", "vmeps")
 from specfem3d_geotech_amd.spec import hex2spec
 
 
@@ -70,10 +74,11 @@ def test_sweep_files_match_reference(tmp_path):
     spec = hex2spec(mesh)
     ctl = dict(cg_tol=1e-8, cg_maxiter=5000, nl_tol=5e-4, nl_maxiter=300)
     srf = [1.0, 1.2]
-    out = SlopeOutput(str(tmp_path), "slope_out", mesh, spec, ctl, srf)
+    out = SlopeOutput(str(tmp_path), "slope_out", mesh, spec, ctl, srf, save=SAVE_ALL)
     for i, s in enumerate(srf):  # SRF trials are independent (src/slope.F90:248-282)
         r = O.slope3d(spec, mesh, srf=[s], **ctl)
-        out.srf_done(i + 1, r["rows"][0], r["x"])
+        out.srf_done(i + 1, r["rows"][0], r["x"],
+                     fields=lambda r=r: O.nodal_fields(spec, r["stress"]))
     out.close()
     names = [str(n) for n in z["file_names"]]
     assert sorted(os.listdir(tmp_path)) == names
@@ -94,3 +99,29 @@ def test_sweep_files_match_reference(tmp_path):
                         assert x == y, (a, b)
         else:
             assert mine == ref, n
+
+
+def test_nodal_fields_against_closed_forms():
+    """The oracle's nodal fields against what the mathematics fixes: the
+    nodal stress is the plain mean of the element values at the node (a
+    node's sum order cannot matter for equal values), and the principal
+    stresses are the eigenvalues of the symmetric tensor, ascending, with
+    the normal stress their extreme mean."""
+    mesh = slope_mesh(2, 1, 2)
+    spec = hex2spec(mesh)
+    rng = np.random.default_rng(5)
+    st = rng.normal(size=(spec.nelmt, 27, 6)) * 100.0
+    f = O.nodal_fields(spec, st)
+    g = np.asarray(spec.g_num)
+    acc = np.zeros((spec.nnode, 6)); cnt = np.zeros(spec.nnode)
+    np.add.at(acc, g.ravel(), st.reshape(-1, 6)); np.add.at(cnt, g.ravel(), 1)
+    assert np.allclose(f["stress"], acc / cnt[:, None], rtol=1e-13, atol=1e-11)
+    s = f["stress"]
+    # stress components: xx, yy, zz, xy, yz, zx (src/math_library.f90:160-184)
+    T = np.stack([np.stack([s[:, 0], s[:, 3], s[:, 5]], -1),
+                  np.stack([s[:, 3], s[:, 1], s[:, 4]], -1),
+                  np.stack([s[:, 5], s[:, 4], s[:, 2]], -1)], -2)
+    ev = np.linalg.eigvalsh(T)
+    assert np.allclose(f["psigma"], ev, rtol=1e-9, atol=1e-8)
+    assert np.all(np.diff(f["psigma"], axis=1) >= 0)
+    assert np.allclose(f["nsigma"], 0.5 * (ev[:, 0] + ev[:, 2]), rtol=1e-9, atol=1e-8)
