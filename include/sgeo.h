@@ -158,6 +158,12 @@ int sgeo_mesh_bytes(sgeo_mesh *mesh, size_t *bytes);
  * point) pairs at a node (src/slope.F90:203-210).  The reference computes
  * the same in its drivers; it has no separate interface. */
 int sgeo_mesh_set_nodes(sgeo_mesh *mesh, int32_t nnode, const int32_t *g_num);
+/* The stress, principal and normal stress fields (as sgeo_slope_nodal_fields)
+ * of a host stress state stress_elmt (6*27*nelmt, reference layout) on this
+ * mesh: the excavation's step 0 plots the in-situ stress this way
+ * (src/excavation.F90:303-323). */
+int sgeo_mesh_nodal_fields(sgeo_mesh *mesh, const double *stress_elmt, float *stress,
+                           float *psigma, float *nsigma);
 /* pcg_solver (src/solver.f90:14-79) on a resident mesh: its selected
  * operator and inverted diagonal; u (host, neq+1) warm start, overwritten;
  * f (host, neq+1).  cg_iter and errcode as sgeo_pcg_solver. */
@@ -223,16 +229,25 @@ int sgeo_slope_run_srf(sgeo_slope *slope, const int32_t *gdof_node,
 int sgeo_slope_download(sgeo_slope *slope, double *x, double *bodyload,
                         double *evpt, double *stress_elmt, uint8_t *flags);
 /* The nodal fields save_data writes for the trial's final stress state
- * (src/slope.F90:418-439, src/postprocess.f90:128-248), computed on the
- * device and rounded to FP32 as the EnSight files hold them, in component
- * blocks of nnode values: stress (6 blocks) the mean of stress_elmt over a
- * node's elements summed in element order, psigma (3 blocks) the principal
- * stresses ascending, nsigma = (psigma1 + psigma3) / 2, vmeps the nodal
- * mean of the excavation's per-point plastic strain (zero for a slope
- * trial, which does not accumulate it).  Any pointer may be NULL; needs
- * sgeo_mesh_set_nodes. */
+ * (src/slope.F90:418-439, src/excavation.F90:655-687, src/postprocess.f90:
+ * 128-248), computed on the device and rounded to FP32 as the EnSight files
+ * hold them, in component blocks of nnode values: stress (6 blocks) the
+ * mean of stress_elmt over a node's elements summed in element order,
+ * psigma (3 blocks) the principal stresses ascending, nsigma = (psigma1 +
+ * psigma3) / 2, scf and vmeps the stress concentration factor and the
+ * plastic strain.  A slope trial updates neither (scf = 1e32, vmeps = 0,
+ * src/slope.F90:249, 366-376); an excavation stage gives what
+ * sgeo_slope_accumulate_nodal gathered over its increments.  Any pointer
+ * may be NULL; needs sgeo_mesh_set_nodes. */
 int sgeo_slope_nodal_fields(sgeo_slope *slope, float *stress, float *psigma,
-                            float *nsigma, float *vmeps);
+                            float *nsigma, float *scf, float *vmeps);
+/* After each load increment of an excavation stage (src/excavation.F90:
+ * 599-609): per node, adds the increment's per-point plastic strain
+ * sqrt(2/3 evpt.evpt) and lowers scf to the points' stress concentration
+ * factor, in element order; first != 0 starts the stage from 0 and 1e32
+ * (src/excavation.F90:335).  Needs an increment begun with
+ * sgeo_slope_begin_increment and sgeo_mesh_set_nodes. */
+int sgeo_slope_accumulate_nodal(sgeo_slope *slope, int32_t first);
 
 /* ---- multi-trial mixed-precision engine ---------------------------------
  * Up to 64 (a power of two) strength-reduction trials on one resident mesh,
@@ -288,9 +303,9 @@ int sgeo_batch_advance(sgeo_batch *batch, double cg_tol, int32_t cg_maxiter,
                        int32_t *memo_ready);
 int sgeo_batch_download(sgeo_batch *batch, int32_t col, double *x,
                         uint8_t *flags);
-/* sgeo_slope_nodal_fields for the trial held by column col. */
+/* sgeo_slope_nodal_fields for the slope trial held by column col. */
 int sgeo_batch_nodal_fields(sgeo_batch *batch, int32_t col, float *stress, float *psigma,
-                            float *nsigma, float *vmeps);
+                            float *nsigma, float *scf, float *vmeps);
 /* Plastic iterations done so far by each column's current trial (ncol
  * values; 0 where no trial runs), as of the last body pass.  Lets a
  * scheduler see a trial heading for nl_maxiter (a failing SRF) before it

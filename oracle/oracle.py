@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+from types import SimpleNamespace
 
 import numpy as np
 
@@ -47,7 +48,7 @@ def lib():
         _LIB.orc_plastic_body.restype = C.c_double
         _LIB.orc_plastic_body_excav.argtypes = [C.c_int32, C.c_int32, i32, d, i32,
                                                 i32, d, d, d, d, d, C.c_double,
-                                                d, d, d, d, C.c_int, i32, d]
+                                                d, d, d, d, C.c_int, i32, d, d]
         _LIB.orc_plastic_body_excav.restype = C.c_double
         _LIB.orc_slope3d.argtypes = ([C.c_int32] * 3 + [d, i32, i32, i32, C.c_int]
                                      + [d] * 6 + [C.c_int, C.c_double, C.c_int32,
@@ -280,7 +281,11 @@ def excavation3d(spec, mesh, *, cg_tol, cg_maxiter, nl_tol, nl_maxiter, srf,
                  on_pcg=None):
     """The reference's staged excavation (src/excavation.F90:13-700), one SRF
     (the reference supports only nsrf = 1 there).  Returns dict(rows=[(stage,
-    cg_tot, nl_tot, uxmax, umax, fmax)], stress0, nodalu, cg_log)."""
+    cg_tot, nl_tot, uxmax, umax, fmax)], stress0, nodalu, cg_log, fields):
+    fields[k] holds what save_data receives for stage k (0 = the in-situ
+    state, src/excavation.F90:296-323; then :655-687): node_intact (0-based
+    ids), nodalu (nnode_intact, 3), scf, vmeps (nnode_intact,) and the
+    nodal fields of the stress (nodal_fields) at those nodes."""
     if s0_type != 0:
         raise NotImplementedError("s0_type 1 (overburden) not restated")
     gam, ym, nu, phi, coh, psi = mesh.material_arrays()
@@ -306,6 +311,10 @@ def excavation3d(spec, mesh, *, cg_tol, cg_maxiter, nl_tol, nl_maxiter, srf,
     ismat = np.ones(nmat, bool)
     evpt = np.zeros((E, 27, 6))
     rows = []
+    every = np.arange(spec.nnode)
+    fields = [dict(node_intact=every, nodalu=np.zeros((spec.nnode, 3)),
+                   scf=np.zeros(spec.nnode), vmeps=np.zeros(spec.nnode),
+                   **nodal_fields(spec, stress))]
     for i_ex in range(nexcav):
         id0 = i_ex * nexcavid[i_ex]
         ids = np.asarray(excavid[id0:id0 + nexcavid[i_ex]])
@@ -332,6 +341,9 @@ def excavation3d(spec, mesh, *, cg_tol, cg_maxiter, nl_tol, nl_maxiter, srf,
         cg_tot = nl_tot = 0
         fmax = 0.0
         nl_iter = 0
+        vmeps = np.zeros(spec.nnode)
+        scf = np.full(spec.nnode, 1.0e32)  # inftol (src/global.f90:22)
+        g_i32 = _i32(g_num)
         for _ in range(ninc):
             bodyload = np.zeros(neq + 1)
             evpt[:] = 0.0
@@ -355,7 +367,8 @@ def excavation3d(spec, mesh, *, cg_tol, cg_maxiter, nl_tol, nl_maxiter, srf,
                     neq, len(intact), _p(intact, C.c_int32), _p(_f64(spec.corners)),
                     _p(ge, C.c_int32), _p(_i32(mat), C.c_int32), _p(_f64(ym)),
                     _p(_f64(nuf)), _p(_f64(phif)), _p(_f64(cohf)), _p(_f64(psif)), dt,
-                    _p(x), _p(evpt), _p(stress), _p(bodyload), int(last), None, None)
+                    _p(x), _p(evpt), _p(stress), _p(bodyload), int(last),
+                    _p(g_i32, C.c_int32), _p(vmeps), _p(scf))
                 if last:
                     break
             nl_tot += nl_iter
@@ -364,7 +377,15 @@ def excavation3d(spec, mesh, *, cg_tol, cg_maxiter, nl_tol, nl_maxiter, srf,
         un = nodalu[alive]
         rows.append((i_ex + 1, cg_tot, nl_tot, float(np.abs(un).max()),
                      float(np.sqrt((un * un).sum(axis=1)).max()), float(fmax)))
+        node_intact = np.nonzero(alive)[0]
+        sub = SimpleNamespace(nelmt=len(intact), nnode=spec.nnode, g_num=g_num[intact])
+        nf = nodal_fields(sub, stress[intact])
+        val = np.bincount(g_num[intact].reshape(-1), minlength=spec.nnode)
+        fields.append(dict(node_intact=node_intact, nodalu=nodalu[node_intact].copy(),
+                           scf=scf[node_intact],
+                           vmeps=(vmeps / np.maximum(val, 1))[node_intact],
+                           **{k: v[node_intact] for k, v in nf.items()}))
         if nl_iter == nl_maxiter:
             break
     return dict(rows=rows, stress0=stress0, nodalu=nodalu, cg_log=cg_log,
-                stress=stress)
+                stress=stress, fields=fields)

@@ -575,7 +575,8 @@ static double body_element(const tables_t *T, const double *xc,
                            const int32_t *g, double E, double nuf, double phif,
                            double cohf, double psif, double dt, const double *x,
                            double *evpt, double *stress, int conv_or_last,
-                           double *bload, uint8_t *flags, double *vm_pt) {
+                           double *bload, uint8_t *flags, double *vm_pt,
+                           double *sf_pt) {
   double cmat[36], deriv[3 * NG], b[NST * NED], eld[NED];
   orc_compute_cmat(E, nuf, cmat);
   for (int j = 0; j < NED; j++) eld[j] = x[g[j]];
@@ -637,6 +638,11 @@ static double body_element(const tables_t *T, const double *xc,
         for (int s = 0; s < NST; s++) d = d + ev[s] * ev[s];
         vm_pt[i] = sqrt((2.0 / 3.0) * d);
       }
+      if (sf_pt) { /* src/excavation.F90:604-608 */
+        const double phir = phif * DEG2RAD;
+        sf_pt[i] = (sigm * sin(phir) - cohf * cos(phir)) /
+                   (-dsbar * (cos(theta) / sqrt(3.0) - sin(theta) * sin(phir) / 3.0));
+      }
       for (int s = 0; s < NST; s++) st[s] = eff[s];
     }
   }
@@ -662,7 +668,7 @@ double orc_plastic_body(int32_t neq, int32_t nelmt, const double *xc,
                              evpt + 6 * NG * (size_t)e,
                              stress_elmt + 6 * NG * (size_t)e, conv_or_last,
                              bl + (size_t)NED * e,
-                             flags ? flags + NG * (size_t)e : NULL, NULL);
+                             flags ? flags + NG * (size_t)e : NULL, NULL, NULL);
     if (fe > fmax) fmax = fe;
   }
   if (!conv_or_last) {
@@ -687,11 +693,12 @@ double orc_plastic_body_excav(int32_t neq, int32_t nel_act,
                               const double *psif, double dt, const double *x,
                               double *evpt, double *stress_elmt,
                               double *bodyload, int conv_or_last,
-                              const int32_t *g_num, double *vmeps) {
+                              const int32_t *g_num, double *vmeps, double *scf) {
   (void)neq;
   const tables_t *T = tables();
   double *bl = (double *)malloc(sizeof(double) * NED * (size_t)nel_act);
   double *vm = (double *)malloc(sizeof(double) * NG * (size_t)nel_act);
+  double *sf = (double *)malloc(sizeof(double) * NG * (size_t)nel_act);
   double fmax = 0.0;
 #pragma omp parallel for schedule(static) reduction(max : fmax)
   for (int32_t a = 0; a < nel_act; a++) {
@@ -701,7 +708,8 @@ double orc_plastic_body_excav(int32_t neq, int32_t nel_act,
                              ym[m], nuf[m], phif[m], cohf[m], psif[m], dt, x,
                              evpt + 6 * NG * (size_t)e,
                              stress_elmt + 6 * NG * (size_t)e, conv_or_last,
-                             bl + (size_t)NED * a, NULL, vm + NG * (size_t)a);
+                             bl + (size_t)NED * a, NULL, vm + NG * (size_t)a,
+                             sf + NG * (size_t)a);
     if (fe > fmax) fmax = fe;
   }
   if (!conv_or_last) {
@@ -712,18 +720,20 @@ double orc_plastic_body_excav(int32_t neq, int32_t nel_act,
       for (int j = 0; j < NED; j++) tmp[j] = bodyload[g[j]] + b[j];
       for (int j = 0; j < NED; j++) bodyload[g[j]] = tmp[j];
     }
-  } else if (vmeps && g_num) {
+  } else if (g_num) { /* element then point order (src/excavation.F90:546-609) */
     for (int32_t a = 0; a < nel_act; a++) {
       int32_t e = elmt[a];
       for (int i = 0; i < NG; i++) {
         int32_t n = g_num[NG * (size_t)e + i];
-        vmeps[n] = vmeps[n] + vm[NG * (size_t)a + i];
+        if (vmeps) vmeps[n] = vmeps[n] + vm[NG * (size_t)a + i];
+        if (scf && sf[NG * (size_t)a + i] < scf[n]) scf[n] = sf[NG * (size_t)a + i];
       }
     }
   }
   bodyload[0] = 0.0;
   free(bl);
   free(vm);
+  free(sf);
   return fmax;
 }
 

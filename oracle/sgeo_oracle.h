@@ -95,7 +95,9 @@ double orc_plastic_body(int32_t neq, int32_t nelmt, const double *xc,
 /* Excavation variant of the body (src/excavation.F90:546-616): elements
  * elmt[0..nel_act-1] (0-based ids into the full arrays), effective stress
  * adds water (none here) then stress_elmt, and on conv_or_last also
- * accumulates vmeps per node (g_num 0-based node ids, 27 per element). */
+ * accumulates vmeps per node and lowers scf per node to the points' stress
+ * concentration factor (g_num 0-based node ids, 27 per element; vmeps and
+ * scf may be NULL). */
 double orc_plastic_body_excav(int32_t neq, int32_t nel_act,
                               const int32_t *elmt, const double *xc,
                               const int32_t *gdof_act, const int32_t *mat_id,
@@ -104,7 +106,7 @@ double orc_plastic_body_excav(int32_t neq, int32_t nel_act,
                               const double *psif, double dt, const double *x,
                               double *evpt, double *stress_elmt,
                               double *bodyload, int conv_or_last,
-                              const int32_t *g_num, double *vmeps);
+                              const int32_t *g_num, double *vmeps, double *scf);
 
 /* elastic_stress (src/postprocess.f90:40-79): stress(:,i,e) = C B u_e at
  * every GLL point of elements elmt[0..nel-1] (NULL: 0..nel-1), written at

@@ -129,6 +129,8 @@ struct sgeo_slope {
   double *d_body = nullptr, *d_ext = nullptr;
   double *d_evpt = nullptr, *d_stress = nullptr, *d_BL = nullptr;
   double* d_vm = nullptr;  // excavation: per-point von Mises plastic strain
+  double* d_sf = nullptr;  // excavation: per-point stress concentration factor
+  double* d_nacc = nullptr;  // excavation: nodal sums of vm, minima of sf (2 x nnode)
   uint8_t* d_flags = nullptr;
   MatParams* d_mp = nullptr;
   std::vector<MatParams> h_mp;
@@ -366,35 +368,66 @@ extern "C" int sgeo_mesh_set_nodes(sgeo_mesh* m, int32_t nnode, const int32_t* g
 }
 
 // Nodal result fields of one trial's stress state (device, FP64 [e][27][6])
-// and, when given, its per-point plastic strain; FP32 outputs in component
-// blocks, any of them may be NULL.  vmeps without a per-point source is
-// zero: the slope driver never accumulates it (src/slope.F90:366-368).
-static int nodal_fields(sgeo_mesh* m, const double* d_stress, const double* d_vm,
-                        float* sig, float* psig, float* nsig, float* vmeps) {
+// and, for an excavation stage, its accumulated plastic strain and stress
+// concentration factor (d_acc, 2 x nnode, NULL otherwise); FP32 outputs in
+// component blocks, any of them may be NULL.  Without d_acc, vmeps is zero
+// and scf is inftol: the slope driver updates neither (src/slope.F90:249,
+// 366-376).
+static int nodal_fields(sgeo_mesh* m, const double* d_stress, const double* d_acc,
+                        float* sig, float* psig, float* nsig, float* scf, float* vmeps) {
   if (!m->d_nrowptr)
     return fail(SGEO_ESTATE, "no node map: call sgeo_mesh_set_nodes first");
   const size_t nn = (size_t)m->nnode;
   HIPCHK(hipSetDevice(m->ctx->device));
   hipStream_t s = m->ctx->stream;
   float* d_out = nullptr;
-  TRY(dalloc(&d_out, nn * 11));
-  float *ds = d_out, *dp = d_out + 6 * nn, *dn = d_out + 9 * nn, *dv = d_out + 10 * nn;
-  launch_nodal_fields(s, m->nnode, m->d_nrowptr, m->d_nslots, d_stress, d_vm,
-                      sig ? ds : nullptr, psig ? dp : nullptr, nsig ? dn : nullptr,
-                      vmeps && d_vm ? dv : nullptr);
+  TRY(dalloc(&d_out, nn * 12));
+  float *ds = d_out, *dp = d_out + 6 * nn, *dn = d_out + 9 * nn, *dv = d_out + 10 * nn,
+        *dc = d_out + 11 * nn;
+  launch_nodal_fields(s, m->nnode, m->d_nrowptr, m->d_nslots, d_stress, sig ? ds : nullptr,
+                      psig ? dp : nullptr, nsig ? dn : nullptr);
+  if (d_acc)
+    launch_nodal_acc_out(s, m->nnode, m->d_nrowptr, d_acc, vmeps ? dv : nullptr,
+                         scf ? dc : nullptr);
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess && sig)
-    e = hipMemcpyAsync(sig, ds, nn * 6 * 4, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess && psig)
-    e = hipMemcpyAsync(psig, dp, nn * 3 * 4, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess && nsig) e = hipMemcpyAsync(nsig, dn, nn * 4, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess && vmeps && d_vm)
-    e = hipMemcpyAsync(vmeps, dv, nn * 4, hipMemcpyDeviceToHost, s);
+  struct Out {
+    float* host;
+    const float* dev;
+    size_t n;
+  } outs[] = {{sig, ds, 6 * nn}, {psig, dp, 3 * nn}, {nsig, dn, nn},
+              {d_acc ? vmeps : nullptr, dv, nn}, {d_acc ? scf : nullptr, dc, nn}};
+  for (const Out& o : outs)
+    if (e == hipSuccess && o.host)
+      e = hipMemcpyAsync(o.host, o.dev, o.n * 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   hipFree(d_out);
   if (e != hipSuccess) return fail(SGEO_EHIP, "nodal fields: %s", hipGetErrorString(e));
-  if (vmeps && !d_vm) std::memset(vmeps, 0, nn * 4);
+  if (!d_acc) {
+    if (vmeps) std::memset(vmeps, 0, nn * 4);
+    if (scf)
+      for (size_t n = 0; n < nn; n++) scf[n] = 1.0e32f;  // real(inftol)
+  }
   return 0;
+}
+
+// The nodal fields of a host stress state (E,27,6) on this mesh: the
+// in-situ stress plotted as the excavation's step 0 (src/excavation.F90:
+// 303-323).
+extern "C" int sgeo_mesh_nodal_fields(sgeo_mesh* m, const double* stress_elmt, float* sig,
+                                      float* psig, float* nsig) {
+  if (!m || !stress_elmt) return fail(SGEO_EARG, "sgeo_mesh_nodal_fields: null argument");
+  const size_t ns = (size_t)m->nelmt * 27 * 6;
+  HIPCHK(hipSetDevice(m->ctx->device));
+  double* d_st = nullptr;
+  TRY(dalloc(&d_st, ns));
+  hipError_t e = hipMemcpyAsync(d_st, stress_elmt, ns * 8, hipMemcpyHostToDevice,
+                                m->ctx->stream);
+  int rc = e == hipSuccess
+               ? nodal_fields(m, d_st, nullptr, sig, psig, nsig, nullptr, nullptr)
+               : fail(SGEO_EHIP, "sgeo_mesh_nodal_fields: %s", hipGetErrorString(e));
+  hipStreamSynchronize(m->ctx->stream);
+  hipFree(d_st);
+  return rc;
 }
 
 extern "C" int sgeo_mesh_bytes(sgeo_mesh* m, size_t* bytes) {
@@ -847,7 +880,8 @@ extern "C" int sgeo_slope_destroy(sgeo_slope* t) {
   hipStreamSynchronize(t->mesh->ctx->stream);
   void* ptrs[] = {t->d_x,      t->d_oldx, t->d_load,  t->d_body,
                   t->d_ext,    t->d_evpt, t->d_stress, t->d_BL,
-                  t->d_flags,  t->d_mp,   t->d_vm};
+                  t->d_flags,  t->d_mp,   t->d_vm,     t->d_sf,
+                  t->d_nacc};
   for (void* p : ptrs)
     if (p) hipFree(p);
   delete t;
@@ -902,6 +936,7 @@ extern "C" int sgeo_slope_begin_increment(sgeo_slope* t, const double* ym,
   const size_t n1 = (size_t)m->neq + 1;
   const size_t ns = (size_t)m->nelmt * 27 * 6;
   if (!t->d_vm) TRY(dalloc(&t->d_vm, (size_t)m->nelmt * 27));
+  if (!t->d_sf) TRY(dalloc(&t->d_sf, (size_t)m->nelmt * 27));
   HIPCHK(hipMemcpyAsync(t->d_mp, t->h_mp.data(), m->nmat * sizeof(MatParams),
                         hipMemcpyHostToDevice, s));
   if (stress0)
@@ -912,6 +947,7 @@ extern "C" int sgeo_slope_begin_increment(sgeo_slope* t, const double* ym,
   HIPCHK(hipMemsetAsync(t->d_oldx, 0, n1 * 8, s));
   HIPCHK(hipMemsetAsync(t->d_flags, 0, (size_t)m->nelmt * 27, s));
   HIPCHK(hipMemsetAsync(t->d_vm, 0, (size_t)m->nelmt * 27 * 8, s));
+  HIPCHK(hipMemsetAsync(t->d_sf, 0, (size_t)m->nelmt * 27 * 8, s));
   HIPCHK(hipStreamSynchronize(s));
   t->excav = 1;
   t->begun = true;
@@ -1051,6 +1087,7 @@ extern "C" int sgeo_slope_plastic_iter(sgeo_slope* t, double cg_tol,
   b.is_last = is_last;
   b.excav = t->excav;
   b.vm_pt = t->d_vm;
+  b.sf_pt = t->d_sf;
   launch_plastic_body(s, b);
   // bodyload(egdof) += bload unless converged/last (src/slope.F90:380-384);
   // the kernel itself returns when nl_isconv is set on the device
@@ -1141,11 +1178,26 @@ extern "C" int sgeo_slope_download(sgeo_slope* t, double* x, double* bodyload,
   return 0;
 }
 
+extern "C" int sgeo_slope_accumulate_nodal(sgeo_slope* t, int32_t first) {
+  if (!t || !t->excav || !t->d_vm || !t->d_sf)
+    return fail(SGEO_ESTATE, "sgeo_slope_accumulate_nodal: no excavation increment");
+  sgeo_mesh* m = t->mesh;
+  if (!m->d_nrowptr)
+    return fail(SGEO_ESTATE, "no node map: call sgeo_mesh_set_nodes first");
+  HIPCHK(hipSetDevice(m->ctx->device));
+  if (!t->d_nacc) TRY(dalloc(&t->d_nacc, 2 * (size_t)m->nnode));
+  launch_nodal_accumulate(m->ctx->stream, m->nnode, m->d_nrowptr, m->d_nslots, t->d_vm,
+                          t->d_sf, t->d_nacc, first ? 1 : 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(m->ctx->stream));
+  return 0;
+}
+
 extern "C" int sgeo_slope_nodal_fields(sgeo_slope* t, float* stress, float* psigma,
-                                       float* nsigma, float* vmeps) {
+                                       float* nsigma, float* scf, float* vmeps) {
   if (!t) return fail(SGEO_EARG, "null slope");
-  return nodal_fields(t->mesh, t->d_stress, t->excav ? t->d_vm : nullptr, stress, psigma,
-                      nsigma, vmeps);
+  return nodal_fields(t->mesh, t->d_stress, t->excav ? t->d_nacc : nullptr, stress, psigma,
+                      nsigma, scf, vmeps);
 }
 
 // --------------------------------------------------------------------------
@@ -1815,11 +1867,12 @@ extern "C" int sgeo_batch_download(sgeo_batch* b, int32_t col, double* x,
 }
 
 extern "C" int sgeo_batch_nodal_fields(sgeo_batch* b, int32_t col, float* stress,
-                                       float* psigma, float* nsigma, float* vmeps) {
+                                       float* psigma, float* nsigma, float* scf,
+                                       float* vmeps) {
   if (!b || col < 0 || col >= b->B) return fail(SGEO_EARG, "bad column");
   const size_t ns = (size_t)b->mesh->nelmt * 27 * 6;
   return nodal_fields(b->mesh, b->d_stress + (size_t)col * ns, nullptr, stress, psigma,
-                      nsigma, vmeps);
+                      nsigma, scf, vmeps);
 }
 
 extern "C" int sgeo_batch_stats(sgeo_batch* b, int64_t* inner_iters,

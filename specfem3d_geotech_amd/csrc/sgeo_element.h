@@ -158,12 +158,14 @@ struct MatParams {
 // Strain, stress, yield and viscoplastic update at one GLL point
 // (src/slope.F90:329-377; src/excavation.F90:561-611).  ue: element
 // displacement (81).  Updates ev (evpt, 6) in place; returns f and writes
-// eff (effective stress) and devp = C evp (zero when f < 0).
+// eff (effective stress), devp = C evp (zero when f < 0) and, when inv is
+// given, the invariants sigm, dsbar, theta of eff.
 __device__ __forceinline__ double point_update(int i, const double jinv[3][3],
                                                const double* ue,
                                                const MatParams& mm,
                                                const double* st, double* ev,
-                                               double* eff, double* devp) {
+                                               double* eff, double* devp,
+                                               double* inv = nullptr) {
   const Cmat cm = cmat_of(mm.ym, mm.nuf);
   double eps[6] = {0, 0, 0, 0, 0, 0};
   for (int n = 0; n < 27; n++) {  // eps = matmul(bmat, eld)
@@ -186,6 +188,11 @@ __device__ __forceinline__ double point_update(int i, const double jinv[3][3],
   for (int s = 0; s < 6; s++) eff[s] = sig[s] + st[s];
   double sigm, dsbar, theta;
   stress_invariant(eff, sigm, dsbar, theta);
+  if (inv) {  // the invariants of the effective stress, for the caller
+    inv[0] = sigm;
+    inv[1] = dsbar;
+    inv[2] = theta;
+  }
   // mohcouf (src/plastic_library.f90:101-106)
   const double deg2rad = 3.141592653589793 / 180.0;
   const double phir = mm.phif * deg2rad;

@@ -40,6 +40,7 @@ struct BodyArgs {
   int ncols = 1;
   int excav = 0;
   double* vm_pt = nullptr;         // [col][nel][27] (excavation)
+  double* sf_pt = nullptr;         // [col][nel][27] (excavation)
 };
 
 unsigned vec_reduce_blocks(int64_t n1);
@@ -101,8 +102,13 @@ void launch_excav_load(hipStream_t s, int32_t nelmt, const double* corners,
                        const int32_t* mat_id, const double* gam, const double* stress,
                        double* Y);
 void launch_nodal_fields(hipStream_t s, int32_t nnode, const int32_t* rowptr,
-                         const int32_t* slots, const double* stress, const double* vm,
-                         float* sig, float* psig, float* nsig, float* vmo);
+                         const int32_t* slots, const double* stress, float* sig,
+                         float* psig, float* nsig);
+void launch_nodal_accumulate(hipStream_t s, int32_t nnode, const int32_t* rowptr,
+                             const int32_t* slots, const double* vm, const double* sf,
+                             double* acc, int first);
+void launch_nodal_acc_out(hipStream_t s, int32_t nnode, const int32_t* rowptr,
+                          const double* acc, float* vmo, float* scfo);
 
 // ---- multi-trial mixed-precision engine (sgeo_batch.hip) -----------------
 // Per-column states of the mixed-precision PCG.  The inner iteration runs in

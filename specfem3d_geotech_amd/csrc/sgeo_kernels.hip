@@ -152,7 +152,7 @@ __global__ __launch_bounds__(BODY_THREADS) void k_plastic_body(
     double* __restrict__ BL, uint8_t* __restrict__ flags,
     PlasticScalars* __restrict__ ps, int is_last,
     const int32_t* __restrict__ is_last_col, const int32_t* __restrict__ cols,
-    int excav, double* __restrict__ vm_pt) {
+    int excav, double* __restrict__ vm_pt, double* __restrict__ sf_pt) {
   __shared__ double s_x[BODY_EPB][81];
   __shared__ double s_xc[BODY_EPB][24];
   __shared__ PointOut s_pt[BODY_EPB][27];
@@ -193,7 +193,8 @@ __global__ __launch_bounds__(BODY_THREADS) void k_plastic_body(
     double* st = stress + ((so + a) * 27 + i) * 6;
     double evl[6], stl[6], eff[6], devp[6];
     for (int s = 0; s < 6; s++) { evl[s] = ev[s]; stl[s] = st[s]; }
-    const double f = point_update(i, jinv, s_x[el], mm, stl, evl, eff, devp);
+    double inv[3];
+    const double f = point_update(i, jinv, s_x[el], mm, stl, evl, eff, devp, inv);
     fmax = f > 0.0 ? f : 0.0;  // fmax starts at zero (src/slope.F90:286)
     const bool plastic = f >= 0.0;
     if (flags) flags[(so + a) * 27 + i] = plastic ? 1 : 0;
@@ -212,6 +213,13 @@ __global__ __launch_bounds__(BODY_THREADS) void k_plastic_body(
         double d = 0.0;
         for (int s = 0; s < 6; s++) d = d + evl[s] * evl[s];
         vm_pt[(so + a) * 27 + i] = sqrt((2.0 / 3.0) * d);
+      }
+      if (excav && sf_pt) {  // stress concentration factor, src/excavation.F90:604-608
+        const double phir = mm.phif * (3.141592653589793 / 180.0);
+        const double snph = sin(phir);
+        sf_pt[(so + a) * 27 + i] =
+            (inv[0] * snph - mm.cohf * cos(phir)) /
+            (-inv[1] * (cos(inv[2]) / sqrt(3.0) - sin(inv[2]) * snph / 3.0));
       }
       for (int s = 0; s < 6; s++) st[s] = eff[s];  // stress_elmt = effsigma
     }
@@ -397,28 +405,21 @@ void launch_elastic_stress(hipStream_t s, int32_t nelmt, const double* corners,
 
 // --------------------------------------------------------------------------
 // nodal result fields (src/slope.F90:418-437, src/postprocess.f90:146-183):
-// thread per node.  The point values of the elements at the node (`slots`
-// = e*27+i, ascending in e) are summed from zero in that order and divided
-// by the node's valency in FP64; the principal stresses follow from the
+// thread per node.  The stresses at the node's points (`slots` = e*27+i,
+// ascending in e) are summed from zero in that order and divided by the
+// node's valency in FP64; the principal stresses follow from the
 // invariants, ascending, and the normal stress on the plane of maximum
 // shear is their mean.  Outputs are rounded to FP32, component blocks of
 // nnode values, as the EnSight writers store them.  Any output may be null.
 // --------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_nodal_fields(
     int32_t nnode, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ slots,
-    const double* __restrict__ stress, const double* __restrict__ vm,
-    float* __restrict__ sig, float* __restrict__ psig, float* __restrict__ nsig,
-    float* __restrict__ vmo) {
+    const double* __restrict__ stress, float* __restrict__ sig, float* __restrict__ psig,
+    float* __restrict__ nsig) {
   const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (n >= nnode) return;
   const int32_t q0 = rowptr[n], q1 = rowptr[n + 1];
   const double val = (double)(q1 - q0);  // node_valency
-  if (vm && vmo) {
-    double v = 0.0;
-    for (int32_t q = q0; q < q1; q++) v = v + vm[slots[q]];
-    vmo[n] = (float)(v / val);
-  }
-  if (!stress || !(sig || psig || nsig)) return;
   double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int32_t q = q0; q < q1; q++) {
     const double* p = stress + (int64_t)slots[q] * 6;
@@ -452,12 +453,61 @@ __global__ __launch_bounds__(256) void k_nodal_fields(
   if (nsig) nsig[n] = (float)(0.5 * (ps[0] + ps[2]));
 }
 
-void launch_nodal_fields(hipStream_t s, int32_t nnode, const int32_t* rowptr,
-                         const int32_t* slots, const double* stress, const double* vm,
-                         float* sig, float* psig, float* nsig, float* vmo) {
+// Running nodal sums of the excavation's per-point plastic strain and
+// minima of its stress concentration factor over a stage's load increments
+// (src/excavation.F90:599-609): per node, the previous value is carried
+// through the node's points in element order, as the reference's element
+// loop does.  first != 0 starts from 0 and inftol (src/excavation.F90:335).
+__global__ __launch_bounds__(256) void k_nodal_accumulate(
+    int32_t nnode, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ slots,
+    const double* __restrict__ vm, const double* __restrict__ sf,
+    double* __restrict__ acc, int first) {
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (n >= nnode) return;
+  double v = first ? 0.0 : acc[n];
+  double m = first ? 1.0e32 : acc[nnode + n];
+  for (int32_t q = rowptr[n]; q < rowptr[n + 1]; q++) {
+    const int32_t p = slots[q];
+    v = v + vm[p];
+    if (sf[p] < m) m = sf[p];
+  }
+  acc[n] = v;
+  acc[nnode + n] = m;
+}
+
+// FP32 outputs of the accumulated fields: vmeps = sum / valency, scf.
+__global__ __launch_bounds__(256) void k_nodal_acc_out(int32_t nnode,
+                                                       const int32_t* __restrict__ rowptr,
+                                                       const double* __restrict__ acc,
+                                                       float* __restrict__ vmo,
+                                                       float* __restrict__ scfo) {
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (n >= nnode) return;
+  if (vmo) vmo[n] = (float)(acc[n] / (double)(rowptr[n + 1] - rowptr[n]));
+  if (scfo) scfo[n] = (float)acc[nnode + n];
+}
+
+void launch_nodal_accumulate(hipStream_t s, int32_t nnode, const int32_t* rowptr,
+                             const int32_t* slots, const double* vm, const double* sf,
+                             double* acc, int first) {
   if (nnode <= 0) return;
+  hipLaunchKernelGGL(k_nodal_accumulate, dim3(nblk(nnode, 256)), dim3(256), 0, s, nnode,
+                     rowptr, slots, vm, sf, acc, first);
+}
+
+void launch_nodal_acc_out(hipStream_t s, int32_t nnode, const int32_t* rowptr,
+                          const double* acc, float* vmo, float* scfo) {
+  if (nnode <= 0) return;
+  hipLaunchKernelGGL(k_nodal_acc_out, dim3(nblk(nnode, 256)), dim3(256), 0, s, nnode,
+                     rowptr, acc, vmo, scfo);
+}
+
+void launch_nodal_fields(hipStream_t s, int32_t nnode, const int32_t* rowptr,
+                         const int32_t* slots, const double* stress, float* sig,
+                         float* psig, float* nsig) {
+  if (nnode <= 0 || !(sig || psig || nsig)) return;
   hipLaunchKernelGGL(k_nodal_fields, dim3(nblk(nnode, 256)), dim3(256), 0, s, nnode,
-                     rowptr, slots, stress, vm, sig, psig, nsig, vmo);
+                     rowptr, slots, stress, sig, psig, nsig);
 }
 
 void launch_excav_load(hipStream_t s, int32_t nelmt, const double* corners,
@@ -474,7 +524,7 @@ void launch_plastic_body(hipStream_t s, const BodyArgs& b) {
                      dim3(BODY_THREADS), 0, s, b.nel, b.elmt, b.corners, b.gdof,
                      b.mat_id, b.mp, b.nmat, b.x, b.xs, b.evpt, b.stress, b.BL,
                      b.flags, b.ps, b.is_last, b.is_last_col, b.cols, b.excav,
-                     b.vm_pt);
+                     b.vm_pt, b.sf_pt);
 }
 
 }  // namespace sgeo

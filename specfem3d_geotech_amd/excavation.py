@@ -70,10 +70,13 @@ def excavation3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
                  nexcavid: Sequence[int], excavid: Sequence[int], ninc: int,
                  phinu: bool = False, s0_type: int = 0, kprec: int = K_F64,
                  kfmt: int = KFMT_FULL, spec: Optional[SpecMesh] = None,
-                 progress=None) -> ExcavationResult:
+                 progress=None, output=None) -> ExcavationResult:
     """Staged excavation on the GPU for one SRF.  The reference runs
     excavation with nsrf = 1 only (it reallocates and closes per-SRF state
-    inside its SRF loop); the first SRF is used."""
+    inside its SRF loop); the first SRF is used.  `output`
+    (output.ExcavationOutput) receives the in-situ step and every stage's
+    row, displacement and nodal fields (computed on the device) for the
+    reference's result files."""
     if s0_type != 0:
         raise NotImplementedError("initial stress type 1 (overburden) is not built")
     spec = spec or hex2spec_device(ctx, mesh)
@@ -94,6 +97,8 @@ def excavation3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
         cg_log.append(it)
         x[0] = 0.0
         stress = dm.elastic_stress(ym, nu, x)
+        if output is not None:  # step 0 (src/excavation.F90:296-323)
+            output.in_situ(fields=lambda: dm.nodal_fields(stress))
     finally:
         dm.close()
     s = float(srf[0])
@@ -148,16 +153,21 @@ def excavation3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
                 st = trial.download()
                 live = gdof_node != 0
                 nodalu[live] += st["x"][gdof_node[live]]
+                if output is not None and output.wants_fields:
+                    trial.accumulate_nodal(first=inc == 0)
             stress[intact] = st["stress"]
+            un = nodalu[alive]
+            rows.append(dict(stage=i_ex + 1, cg_tot=cg_tot, nl_tot=nl_tot,
+                             uxmax=float(np.abs(un).max()),
+                             umax=float(np.sqrt((un * un).sum(axis=1)).max()),
+                             fmax=float(fmax)))
+            if output is not None:  # src/excavation.F90:641-687
+                output.stage_done(rows[-1], nodalu, np.nonzero(alive)[0], intact,
+                                  fields=trial.nodal_fields)
         finally:
             if trial is not None:
                 trial.close()
             di.close()
-        un = nodalu[alive]
-        rows.append(dict(stage=i_ex + 1, cg_tot=cg_tot, nl_tot=nl_tot,
-                         uxmax=float(np.abs(un).max()),
-                         umax=float(np.sqrt((un * un).sum(axis=1)).max()),
-                         fmax=float(fmax)))
         if progress:
             progress(rows[-1])
         if nl_iter == nl_maxiter:  # src/excavation.F90:694

@@ -275,48 +275,24 @@ def nodal_displacement(spec: SpecMesh, x: np.ndarray) -> np.ndarray:
 # a sweep's files
 # ----------------------------------------------------------------------------
 
-class SlopeOutput:
-    """The files the reference writes for an SRF sweep, in `out_path` with
-    file head `head`.  `save` lists the result fields as the input file's
-    `save:` keys (disp, stress, psigma, nsigma, scf, vmeps); the default is
-    the displacement alone (`disp`).  Writes run on a background thread;
-    `close` waits for them and appends the elapsed-time lines."""
+class _Files:
+    """Background writer and the parts every run's files share."""
 
-    def __init__(self, out_path: str, head: str, mesh: HexMesh, spec: SpecMesh,
-                 control: Dict, srf: Sequence[float], disp: bool = True,
-                 save: Optional[Sequence[str]] = None):
+    def __init__(self, out_path: str, head: str, spec: SpecMesh,
+                 save: Optional[Sequence[str]], disp: bool):
         self.out_path, self.head, self.spec = out_path, head, spec
-        self.srf = list(srf)
         self.save = _save_set(save, disp)
         self.disp = "disp" in self.save
         # fields that need the trial's stress state
-        self.wants_fields = bool(self.save & {"stress", "psigma", "nsigma"})
-        self.nl_maxiter = int(control["nl_maxiter"])
-        self._pending: Dict[int, tuple] = {}
-        self._next = 1
-        self._stopped = False
+        self.wants_fields = bool(self.save & {"stress", "psigma", "nsigma", "scf",
+                                              "vmeps"})
         self.t0 = time.perf_counter()
         os.makedirs(out_path, exist_ok=True)
         self._q: "queue.Queue" = queue.Queue()
         self._err: Optional[BaseException] = None
         self._th = threading.Thread(target=self._run, daemon=True)
         self._th.start()
-        p = self.path
-        self._put(p(f"{head}_original.case"), case_original(head).encode())
-        self._put(p(f"{head}_original.geo"), geo_original(mesh))
-        self._put(p(f"{head}.case"), case_slope(head, self.srf, save=self.save).encode())
-        self._put(p(f"{head}.geo"), geo_gll(spec))
-        summ = (fortran_list("--------------------------------------------")
-                + fortran_list("Result summary produced by SPECFEM3D_GEOTECH")
-                + fortran_list("--------------------------------------------")
-                + fortran_list("CG_MAXITER, CG_TOL, NL_MAXITER, NL_TOL")
-                + fortran_list(int(control["cg_maxiter"]), float(control["cg_tol"]),
-                               int(control["nl_maxiter"]), float(control["nl_tol"]))
-                + fortran_list("Number of SRFs")
-                + fortran_list(len(self.srf))
-                + fortran_list("SRF, CGITER, NLITER, UXMAX, UMAX, fmax"))
-        self._summary = p(f"{head}_summary")
-        self._put(self._summary, summ.encode())
+        self._summary = self.path(f"{head}_summary")
 
     def path(self, name: str) -> str:
         return os.path.join(self.out_path, name)
@@ -336,49 +312,45 @@ class SlopeOutput:
             except BaseException as e:  # reported at close
                 self._err = e
 
-    def srf_done(self, i_srf: int, row: Dict, x: Optional[np.ndarray], fields=None):
-        """Summary row of SRF number i_srf (1-based) and its field files
-        (src/slope.F90:414-439, src/postprocess.f90:128-248).  `fields` is
-        the trial's nodal result fields (stress, psigma, nsigma: a dict, or
-        a callable returning one, called here only if a saved field needs
-        it).  Trials may end in any order (SRFs run in parallel); rows are
-        written in SRF order, and nothing after the first SRF whose plastic
-        loop reached nl_maxiter, where the reference's sweep stops
-        (src/slope.F90:441)."""
-        f = None
-        if self.wants_fields:
-            if fields is None:
-                raise ValueError("the saved fields need the trial's nodal fields")
-            f = fields() if callable(fields) else fields
-        self._pending[i_srf] = (row, x if self.disp else None, f)
-        while self._next in self._pending and not self._stopped:
-            row, x, f = self._pending.pop(self._next)
-            self._put(self._summary, fortran_list(
-                float(row["srf"]), int(row["cg_tot"]), int(row["nl_tot"]),
-                float(row["uxmax"]), float(row["umax"]), float(row["fmax"])).encode(),
-                append=True)
-            self._write_fields(self._next, x, f)
-            if int(row["nl_tot"]) >= self.nl_maxiter:
-                self._stopped = True
-            self._next += 1
+    def _summary_head(self, control: Dict) -> str:
+        """src/semgeotech.F90:317-321, then the drivers' control lines
+        (src/slope.F90:214-217, src/excavation.F90:262-266)."""
+        return (fortran_list("--------------------------------------------")
+                + fortran_list("Result summary produced by SPECFEM3D_GEOTECH")
+                + fortran_list("--------------------------------------------")
+                + fortran_list("CG_MAXITER, CG_TOL, NL_MAXITER, NL_TOL")
+                + fortran_list(int(control["cg_maxiter"]), float(control["cg_tol"]),
+                               int(control["nl_maxiter"]), float(control["nl_tol"])))
 
-    def _write_fields(self, step: int, x, f):
-        tw = step_width(len(self.srf))
+    def _fields_of(self, fields):
+        if not self.wants_fields:
+            return None
+        if fields is None:
+            raise ValueError("the saved fields need the trial's nodal fields")
+        return fields() if callable(fields) else fields
+
+    def _write_step(self, step: int, tw: int, u, f, sel=None):
+        """save_data (src/postprocess.f90:128-248): one file per saved field
+        of output step `step`; u (nnode, 3) the nodal displacement, f the
+        nodal fields, sel the nodes written (all when None)."""
         name = lambda suffix: self.path(f"{self.head}_step{step:0{tw}d}.{suffix}")
-        nn = self.spec.nnode
+        take = (lambda a: a) if sel is None else (lambda a: np.asarray(a)[sel])
+        nn = self.spec.nnode if sel is None else len(sel)
         for key, _, _, suffix, destag in FIELDS:
             if key not in self.save:
                 continue
             if key == "disp":
-                data = pernode_vector(destag, nodal_displacement(self.spec, x))
+                data = pernode_vector(destag, u)
             elif key in ("stress", "psigma"):
-                data = pernode_vector(destag, f[key])
+                data = pernode_vector(destag, take(f[key]))
             elif key == "nsigma":
-                data = pernode_scalar(destag, f["nsigma"])
-            elif key == "scf":  # never updated by the slope driver (src/slope.F90:249, 376)
-                data = pernode_scalar(destag, np.full(nn, INFTOL))
-            else:  # vmeps: not accumulated by the slope driver (src/slope.F90:366-368)
-                data = pernode_scalar(destag, np.zeros(nn))
+                data = pernode_scalar(destag, take(f["nsigma"]))
+            elif key == "scf":
+                data = pernode_scalar(destag, take(f["scf"]) if "scf" in f
+                                      else np.full(nn, INFTOL))
+            else:
+                data = pernode_scalar(destag, take(f["vmeps"]) if "vmeps" in f
+                                      else np.zeros(nn))
             self._put(name(suffix), data)
 
     def close(self):
@@ -394,3 +366,140 @@ class SlopeOutput:
         self._th.join()
         if self._err is not None:
             raise self._err
+
+
+class SlopeOutput(_Files):
+    """The files the reference writes for an SRF sweep, in `out_path` with
+    file head `head`.  `save` lists the result fields as the input file's
+    `save:` keys (disp, stress, psigma, nsigma, scf, vmeps); the default is
+    the displacement alone (`disp`).  Writes run on a background thread;
+    `close` waits for them and appends the elapsed-time lines."""
+
+    def __init__(self, out_path: str, head: str, mesh: HexMesh, spec: SpecMesh,
+                 control: Dict, srf: Sequence[float], disp: bool = True,
+                 save: Optional[Sequence[str]] = None):
+        super().__init__(out_path, head, spec, save, disp)
+        self.srf = list(srf)
+        self.nl_maxiter = int(control["nl_maxiter"])
+        self._pending: Dict[int, tuple] = {}
+        self._next = 1
+        self._stopped = False
+        p = self.path
+        self._put(p(f"{head}_original.case"), case_original(head).encode())
+        self._put(p(f"{head}_original.geo"), geo_original(mesh))
+        self._put(p(f"{head}.case"), case_slope(head, self.srf, save=self.save).encode())
+        self._put(p(f"{head}.geo"), geo_gll(spec))
+        summ = (self._summary_head(control)
+                + fortran_list("Number of SRFs")
+                + fortran_list(len(self.srf))
+                + fortran_list("SRF, CGITER, NLITER, UXMAX, UMAX, fmax"))
+        self._put(self._summary, summ.encode())
+
+    def srf_done(self, i_srf: int, row: Dict, x: Optional[np.ndarray], fields=None):
+        """Summary row of SRF number i_srf (1-based) and its field files
+        (src/slope.F90:414-439, src/postprocess.f90:128-248).  `fields` is
+        the trial's nodal result fields (a dict, or a callable returning
+        one, called here only if a saved field needs it).  Trials may end in
+        any order (SRFs run in parallel); rows are written in SRF order, and
+        nothing after the first SRF whose plastic loop reached nl_maxiter,
+        where the reference's sweep stops (src/slope.F90:441)."""
+        f = self._fields_of(fields)
+        self._pending[i_srf] = (row, x if self.disp else None, f)
+        while self._next in self._pending and not self._stopped:
+            row, x, f = self._pending.pop(self._next)
+            self._put(self._summary, fortran_list(
+                float(row["srf"]), int(row["cg_tot"]), int(row["nl_tot"]),
+                float(row["uxmax"]), float(row["umax"]), float(row["fmax"])).encode(),
+                append=True)
+            u = nodal_displacement(self.spec, x) if x is not None else None
+            self._write_step(self._next, step_width(len(self.srf)), u, f)
+            if int(row["nl_tot"]) >= self.nl_maxiter:
+                self._stopped = True
+            self._next += 1
+
+
+def case_excavation(head: str, nexcav: int, save: Sequence[str], ptail: str = "") -> str:
+    """The excavation's case file (src/semgeotech.F90:126-266): one geometry
+    per step, steps 0 (in situ) .. nexcav, time values 0, 1, ..."""
+    nt = nexcav + 1
+    tw = step_width(nt)
+    s = "FORMAT\ntype:  ensight gold\n\nGEOMETRY\n"
+    s += f"model:    {1:10d} {head}_step{'*' * tw}{ptail}.geo\n\n"
+    s += "VARIABLE\n"
+    for key, kind, name, suffix, _ in FIELDS:
+        if key in save:
+            s += f"{kind}: {1:10d} {name} {head}_step{'*' * tw}{ptail}.{suffix}\n\n"
+    s += "TIME\n"
+    s += f"time set:{1:10d}\n"
+    s += f"number of steps:{nt:10d}\n"
+    s += f"filename start number:{0:10d}\n"
+    s += f"filename increment:{1:10d}\n"
+    s += "time values: "
+    s += "".join(_e12_5(float(i)) + "\n" for i in range(nt))
+    return s
+
+
+def geo_intact(spec: SpecMesh, node_intact: np.ndarray, elmt_intact: np.ndarray) -> bytes:
+    """A stage's geometry (src/excavation.F90:397-430): the intact nodes,
+    renumbered 1.. in ascending order (nmir), and the intact elements as
+    8 hexa8 cells each."""
+    node_intact = np.asarray(node_intact, dtype=np.int64)
+    nmir = np.zeros(spec.nnode, np.int64)
+    nmir[node_intact] = np.arange(1, len(node_intact) + 1)
+    g = np.asarray(spec.g_num, dtype=np.int64)[np.asarray(elmt_intact, dtype=np.int64)]
+    cells = nmir[g[:, CELLS27]].astype(np.int32).reshape(-1, 8)
+    coord = np.asarray(spec.g_coord)[node_intact]
+    return b"".join([_geo_header(len(node_intact), coord), _s80("hexa8"),
+                     _i32(cells.shape[0]), np.ascontiguousarray(cells).tobytes()])
+
+
+class ExcavationOutput(_Files):
+    """The files the reference writes for a staged excavation (one SRF, the
+    only case the reference runs): the hex8 original, the case file, step 0
+    (the whole GLL mesh and the in-situ stress) and per stage the intact
+    mesh and its fields at the intact nodes, and the summary
+    (src/excavation.F90:262-291, 296-323, 397-430, 641-687)."""
+
+    def __init__(self, out_path: str, head: str, mesh: HexMesh, spec: SpecMesh,
+                 control: Dict, srf: Sequence[float], nexcav: int,
+                 save: Optional[Sequence[str]] = None):
+        super().__init__(out_path, head, spec, save, True)
+        self.tw = step_width(nexcav + 1)
+        p = self.path
+        self._put(p(f"{head}_original.case"), case_original(head).encode())
+        self._put(p(f"{head}_original.geo"), geo_original(mesh))
+        self._put(p(f"{head}.case"), case_excavation(head, nexcav, self.save).encode())
+        self._put(p(f"{head}_step{0:0{self.tw}d}.geo"), geo_gll(spec))
+        summ = (self._summary_head(control)
+                + fortran_list("Number of SRFs")
+                + fortran_list(len(srf))
+                + fortran_list("SRF")
+                + fortran_list(float(srf[0]))
+                + fortran_list("Number of excavation stages")
+                + fortran_list(int(nexcav))
+                + fortran_list("STEP, CGITER, NLITER, UXMAX, UMAX, fmax"))
+        self._put(self._summary, summ.encode())
+
+    def in_situ(self, fields=None):
+        """Step 0: zero displacement, scf and vmeps, the in-situ stress
+        fields at every node (src/excavation.F90:296-323)."""
+        f = self._fields_of(fields)
+        nn = self.spec.nnode
+        if f is not None:
+            f = dict(f, scf=np.zeros(nn), vmeps=np.zeros(nn))
+        self._write_step(0, self.tw, np.zeros((nn, 3)), f)
+
+    def stage_done(self, row: Dict, nodalu: np.ndarray, node_intact: np.ndarray,
+                   elmt_intact: np.ndarray, fields=None):
+        """Stage row (src/excavation.F90:646-648), the stage's geometry and
+        its fields at the intact nodes (:655-687); nodalu (nnode, 3) is the
+        displacement summed over the stages so far, fields the stage's
+        nodal fields on the whole node range."""
+        k = int(row["stage"])
+        f = self._fields_of(fields)
+        self._put(self.path(f"{self.head}_step{k:0{self.tw}d}.geo"),
+                  geo_intact(self.spec, node_intact, elmt_intact))
+        self._put(self._summary, fortran_list(
+            k, int(row["cg_tot"]), int(row["nl_tot"]), float(row["uxmax"]),
+            float(row["umax"]), float(row["fmax"])).encode(), append=True)
+        self._write_step(k, self.tw, np.asarray(nodalu)[node_intact], f, sel=node_intact)

@@ -74,6 +74,13 @@ class DeviceSlope:
               "sgeo_slope_download")
         return dict(x=x, bodyload=body, evpt=evpt, stress=stress, flags=flags)
 
+    def accumulate_nodal(self, first: bool):
+        """Excavation: gathers this increment's plastic strain and stress
+        concentration factor at the nodes (sgeo_slope_accumulate_nodal)."""
+        self.dmesh.ensure_nodes()
+        check(L.lib().sgeo_slope_accumulate_nodal(self.h, int(first)),
+              "sgeo_slope_accumulate_nodal")
+
     def nodal_fields(self) -> Dict[str, np.ndarray]:
         """The nodal result fields of the trial's stress state, computed on
         the device (sgeo_slope_nodal_fields)."""

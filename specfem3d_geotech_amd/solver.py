@@ -105,17 +105,24 @@ def pcg_solver(ctx: Context, neq: int, nelmt: int, k: np.ndarray, u: np.ndarray,
     return u, it.value, ec.value, tag.raw[:250].decode().rstrip()
 
 
-def nodal_fields_call(fn, what: str, nnode: int, *args) -> Dict[str, np.ndarray]:
+def nodal_fields_call(fn, what: str, nnode: int, *args,
+                      with_scf_vmeps: bool = True) -> Dict[str, np.ndarray]:
     """Runs one of the sgeo_*_nodal_fields entry points: FP32 stress
-    (nnode, 6), psigma (nnode, 3), nsigma and vmeps (nnode,), as the
-    reference's EnSight files hold them."""
+    (nnode, 6), psigma (nnode, 3), nsigma, and scf and vmeps (nnode,), as
+    the reference's EnSight files hold them."""
     sig = np.zeros((6, nnode), np.float32)
     ps = np.zeros((3, nnode), np.float32)
     ns = np.zeros(nnode, np.float32)
-    vm = np.zeros(nnode, np.float32)
     fp = lambda a: ptr(a, C.c_float)
-    check(fn(*args, fp(sig), fp(ps), fp(ns), fp(vm)), what)
-    return dict(stress=sig.T, psigma=ps.T, nsigma=ns, vmeps=vm)
+    out = dict(stress=sig.T, psigma=ps.T, nsigma=ns)
+    if with_scf_vmeps:
+        sc = np.zeros(nnode, np.float32)
+        vm = np.zeros(nnode, np.float32)
+        check(fn(*args, fp(sig), fp(ps), fp(ns), fp(sc), fp(vm)), what)
+        out.update(scf=sc, vmeps=vm)
+    else:
+        check(fn(*args, fp(sig), fp(ps), fp(ns)), what)
+    return out
 
 
 class DeviceMesh:
@@ -153,6 +160,14 @@ class DeviceMesh:
             check(L.lib().sgeo_mesh_set_nodes(self.h, self.spec.nnode, ptr(g, C.c_int32)),
                   "sgeo_mesh_set_nodes")
             self._nodes_set = True
+
+    def nodal_fields(self, stress) -> Dict[str, np.ndarray]:
+        """Nodal stress, principal and normal stress of a host stress state
+        (nelmt, 27, 6) on this mesh, computed on the device."""
+        self.ensure_nodes()
+        st = f64(stress)
+        return nodal_fields_call(L.lib().sgeo_mesh_nodal_fields, "sgeo_mesh_nodal_fields",
+                                 self.spec.nnode, self.h, ptr(st), with_scf_vmeps=False)
 
     def elastic_stress(self, ym, nu, x):
         """(nelmt, 27, 6) elastic stress at the GLL points from x."""

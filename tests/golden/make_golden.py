@@ -305,6 +305,37 @@ def output_case():
     print("output", names)
 
 
+def output_excavation_case():
+    """The reference's own output files for a small pit excavated in two
+    stages with two load increments each, every result field saved: the
+    in-situ step 0 and each stage's geometry of the intact mesh, its fields
+    at the intact nodes, and the summary (src/excavation.F90:284-700,
+    src/semgeotech.F90:125-266)."""
+    soil = Material(gam=20.0, ym=1e5, nu=0.3, phi=25.0, coh=15.0)
+    mesh = pit_mesh(6, 4, 4, lx=12.0, ly=8.0, lz=8.0, pit=(3.0, 9.0, 2.0, 6.0),
+                    nstage=2, stage_depth=2.0, soil=soil)
+    work = tempfile.mkdtemp(prefix="golden_exout_")
+    sem = write_reference_input(mesh, work, "pit_out", cg_tol=1e-8, cg_maxiter=5000,
+                                nl_tol=1e-5, nl_maxiter=100, srf=[1.0],
+                                extra_control=", nexcav=2, excavid=2 3, ninc=2",
+                                save=SAVE_ALL)
+    r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {REF_BIN} {sem}"],
+                       cwd=work, capture_output=True, text=True)
+    outdir = os.path.join(work, "output")
+    names = sorted(os.listdir(outdir))
+    if "pit_out_summary" not in names:
+        raise RuntimeError(f"reference run failed: {r.stdout[-2000:]} {r.stderr}")
+    out = dict(mesh_arrays(mesh))
+    out["file_names"] = np.array(names)
+    out["excav"] = np.array([2, 2, 2, 3], np.int32)  # nexcav, ninc, excavid
+    for f in names:
+        out["file:" + f] = np.frombuffer(open(os.path.join(outdir, f), "rb").read(),
+                                         dtype=np.uint8)
+    np.savez_compressed(os.path.join(HERE, "output_pit_6x4x4.npz"), **out)
+    shutil.rmtree(work)
+    print("output_excavation", names)
+
+
 def mesh_arrays(m):
     gam, ym, nu, phi, coh, psi = m.material_arrays()
     return dict(coord=m.coord, conn=m.conn, mat_id=m.mat_id,
@@ -442,6 +473,7 @@ if __name__ == "__main__":
          "excavation_example": lambda: excavation_example_case(
              sys.argv[2] if len(sys.argv) > 2 else None),
          "excavation_spread": excavation_example_spread,
-         "list_format": list_format_case, "output": output_case}[sys.argv[1]]()
+         "list_format": list_format_case, "output": output_case,
+         "output_excavation": output_excavation_case}[sys.argv[1]]()
     else:
         main()

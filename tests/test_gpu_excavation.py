@@ -136,3 +136,60 @@ def test_reference_example_excavation_3d(ctx_seq):
                 assert abs(r[keys[k]] - rr[k]) <= 2.0 * sp, (s, keys[k], sp)
     n_elastic = 1 + int(ref[0, 2])  # the gravity solve and stage 1's calls
     assert res.cg_log[:n_elastic] == [int(v) for v in z["cg_log"][:n_elastic]]
+
+
+def test_excavation_output_files(ctx_seq, tmp_path):
+    """The result files of a two-stage pit with two increments per stage and
+    every field saved, written from the device run (nodal fields, plastic
+    strain sums and stress concentration minima computed on the GPU),
+    against the compiled reference's files: case and geometry files byte
+    for byte; summary rows with cg_tot and nl_tot exact, displacements
+    within 1e-10, fmax within 1e-8; each FP32 field value equal to the
+    reference's or one FP32 spacing away (values near zero within 1e-9 of
+    the field's largest)."""
+    import os
+
+    from specfem3d_geotech_amd.mesh import HexMesh, Material
+    from specfem3d_geotech_amd.output import ExcavationOutput
+
+    z = golden("output_pit_6x4x4")
+    mats = [Material(gam=r[0], ym=r[1], nu=r[2], phi=r[3], coh=r[4], psi=r[5])
+            for r in z["materials"]]
+    mesh = HexMesh(coord=z["coord"], conn=z["conn"], mat_id=z["mat_id"], materials=mats,
+                   bc={"ux": z["bc_ux"], "uy": z["bc_uy"], "uz": z["bc_uz"]})
+    ctl = dict(cg_tol=1e-8, cg_maxiter=5000, nl_tol=1e-5, nl_maxiter=100)
+    nexcav, ninc, *excavid = [int(v) for v in z["excav"]]
+    save = ("disp", "stress", "psigma", "nsigma", "scf", "vmeps")
+    spec = hex2spec(mesh)
+    out = ExcavationOutput(str(tmp_path), "pit_out", mesh, spec, ctl, [1.0], nexcav,
+                           save=save)
+    r = excavation3d(ctx_seq, mesh, srf=[1.0], nexcav=nexcav, nexcavid=[1] * nexcav,
+                     excavid=excavid, ninc=ninc, spec=spec, output=out, **ctl)
+    out.close()
+    print(r.rows, flush=True)
+    names = [str(n) for n in z["file_names"]]
+    assert sorted(os.listdir(tmp_path)) == names
+    for n in names:
+        ref, mine = z["file:" + n].tobytes(), (tmp_path / n).read_bytes()
+        if n.endswith("_summary"):
+            la = mine.decode().split("ELAPSED")[0].split()
+            lb = ref.decode().split("ELAPSED")[0].split()
+            assert len(la) == len(lb)
+            rows_a, rows_b = la[-6 * nexcav:], lb[-6 * nexcav:]
+            assert la[:-6 * nexcav] == lb[:-6 * nexcav]
+            for k in range(nexcav):
+                a = [float(v) for v in rows_a[6 * k:6 * k + 6]]
+                b = [float(v) for v in rows_b[6 * k:6 * k + 6]]
+                assert a[:3] == b[:3], (a, b)
+                assert abs(a[3] - b[3]) <= 1e-10 * b[3] and abs(a[4] - b[4]) <= 1e-10 * b[4]
+                assert abs(a[5] - b[5]) <= 1e-8 * max(1.0, abs(b[5])), (a, b)
+        elif n.endswith(".geo") or n.endswith(".case"):
+            assert mine == ref, n
+        else:
+            assert mine[:244] == ref[:244], n
+            a = np.frombuffer(mine[244:], np.float32).astype(np.float64)
+            b = np.frombuffer(ref[244:], np.float32).astype(np.float64)
+            tol = np.maximum(np.spacing(np.abs(b).astype(np.float32)).astype(np.float64),
+                             1e-9 * np.abs(b).max())
+            assert a.shape == b.shape and np.all(np.abs(a - b) <= tol), \
+                (n, float(np.abs(a - b).max()))

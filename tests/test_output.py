@@ -23,8 +23,8 @@ import numpy as np
 import oracle as O
 from conftest import golden
 from specfem3d_geotech_amd.mesh import HexMesh, Material, slope_mesh
-from specfem3d_geotech_amd.output import (SlopeOutput, case_slope, fortran_list,
-                                          fortran_real)
+from specfem3d_geotech_amd.output import (ExcavationOutput, SlopeOutput, case_slope,
+                                          fortran_list, fortran_real)
 
 SAVE_ALL = ("disp", "stress", "psigma", "nsigma", "scf", "vmeps")
 from specfem3d_geotech_amd.spec import hex2spec
@@ -125,3 +125,65 @@ def test_nodal_fields_against_closed_forms():
     assert np.allclose(f["psigma"], ev, rtol=1e-9, atol=1e-8)
     assert np.all(np.diff(f["psigma"], axis=1) >= 0)
     assert np.allclose(f["nsigma"], 0.5 * (ev[:, 0] + ev[:, 2]), rtol=1e-9, atol=1e-8)
+
+
+def _full(nnode, sel, a):
+    out = np.zeros((nnode,) + np.shape(a)[1:])
+    out[sel] = a
+    return out
+
+
+def test_excavation_files_match_reference(tmp_path):
+    """A two-stage pit with two load increments per stage, every field
+    saved (tests/golden/output_pit_6x4x4.npz, written by the compiled
+    reference): case, step geometries of the intact mesh and every field
+    file byte for byte -- the in-situ step 0 and, per stage, the cumulative
+    displacement, the stress fields, the plastic strain summed over the
+    increments and the stress concentration factor -- from the oracle's
+    staged run; the summary value for value."""
+    z = golden("output_pit_6x4x4")
+    mats = [Material(gam=r[0], ym=r[1], nu=r[2], phi=r[3], coh=r[4], psi=r[5])
+            for r in z["materials"]]
+    mesh = HexMesh(coord=z["coord"], conn=z["conn"], mat_id=z["mat_id"], materials=mats,
+                   bc={"ux": z["bc_ux"], "uy": z["bc_uy"], "uz": z["bc_uz"]})
+    spec = hex2spec(mesh)
+    ctl = dict(cg_tol=1e-8, cg_maxiter=5000, nl_tol=1e-5, nl_maxiter=100)
+    nexcav, ninc, *excavid = [int(v) for v in z["excav"]]
+    r = O.excavation3d(spec, mesh, srf=[1.0], nexcav=nexcav, nexcavid=[1] * nexcav,
+                       excavid=excavid, ninc=ninc, **ctl)
+    out = ExcavationOutput(str(tmp_path), "pit_out", mesh, spec, ctl, [1.0], nexcav,
+                           save=SAVE_ALL)
+    nn = spec.nnode
+    f0 = r["fields"][0]
+    out.in_situ({k: f0[k] for k in ("stress", "psigma", "nsigma")})
+    g_num = np.asarray(spec.g_num)
+    for k, row in enumerate(r["rows"]):
+        f = r["fields"][k + 1]
+        sel = f["node_intact"]
+        alive = np.zeros(nn, bool); alive[sel] = True
+        intact = np.nonzero(alive[g_num].all(axis=1) &
+                            ~np.isin(np.asarray(spec.mat_id), excavid[:k + 1]))[0]
+        out.stage_done(dict(stage=row[0], cg_tot=row[1], nl_tot=row[2], uxmax=row[3],
+                            umax=row[4], fmax=row[5]),
+                       _full(nn, sel, f["nodalu"]), sel, intact,
+                       {key: _full(nn, sel, f[key])
+                        for key in ("stress", "psigma", "nsigma", "scf", "vmeps")})
+    out.close()
+    names = [str(n) for n in z["file_names"]]
+    assert sorted(os.listdir(tmp_path)) == names
+    for n in names:
+        ref = z["file:" + n].tobytes()
+        mine = (tmp_path / n).read_bytes()
+        if n.endswith("_summary"):
+            ra, ma = ref.decode().split("\n")[:-2], mine.decode().split("\n")[:-2]
+            assert len(ra) == len(ma)
+            for a, b in zip(ma, ra):
+                ta, tb = a.split(), b.split()
+                assert len(ta) == len(tb), (a, b)
+                for x, y in zip(ta, tb):
+                    try:
+                        assert _val(x) == _val(y), (a, b)
+                    except ValueError:
+                        assert x == y, (a, b)
+        else:
+            assert mine == ref, n
