@@ -387,10 +387,13 @@ __global__ __launch_bounds__(256) void k_bresid(
 }
 
 // Reductions of the chunk partials, in a fixed order independent of B:
-// stage 1 combines groups of BGROUP consecutive chunks per column (thread
-// per (group, column), column fastest: coalesced), stage 2 (one block per
-// column) combines the groups and advances the column's state machine.
-#define BGROUP 64
+// stage 1 combines groups of G consecutive chunks per column (thread per
+// (group, column), column fastest: coalesced), stage 2 (one block per
+// column) combines the groups and advances the column's state machine.  G
+// depends only on the chunk count (batch_group): 64 on large systems, 16
+// when there are few chunks, so a small system's stage 1 is one round of
+// loads and its stage 2 stays short.
+#define BGROUP_MIN 16
 
 __device__ __forceinline__ int fin_states(int what) {
   return what == BFIN_RESID ? (BCOL_BIT(BCOL_START) | BCOL_BIT(BCOL_REPLACE))
@@ -402,31 +405,31 @@ __device__ __forceinline__ int fin_states(int what) {
 __device__ __forceinline__ bool fin_max0(int what) { return what == BFIN_INNER; }
 __device__ __forceinline__ bool fin_max1(int what) { return what == BFIN_ALPHA; }
 
-__global__ __launch_bounds__(256) void k_bgroup(int64_t nch, int B,
+__global__ __launch_bounds__(256) void k_bgroup(int64_t nch, int B, int G,
                                                 const double* __restrict__ part,
                                                 double* __restrict__ part2,
                                                 const int32_t* __restrict__ cst,
                                                 int what) {
-  const int64_t ng = (nch + BGROUP - 1) / BGROUP;
+  const int64_t ng = (nch + G - 1) / G;
   const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (q >= ng * B) return;
   const int j = (int)(q % B);
   const int64_t g = q / B;
   if (!wanted(cst, j, fin_states(what))) return;
   const bool max0 = fin_max0(what), max1 = fin_max1(what);
-  const int64_t b0 = g * BGROUP, b1 = b0 + BGROUP < nch ? b0 + BGROUP : nch;
+  const int64_t b0 = g * G, b1 = b0 + G < nch ? b0 + G : nch;
   double v0 = 0.0, v1 = 0.0;
   int64_t b = b0;
-  // eight partials loaded together, then combined in chunk order
-  for (; b + 8 <= b1; b += 8) {
-    double a0[8], a1[8];
+  // sixteen partials loaded together, then combined in chunk order
+  for (; b + 16 <= b1; b += 16) {
+    double a0[16], a1[16];
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < 16; k++) {
       a0[k] = part[(0 * nch + b + k) * B + j];
       a1[k] = part[(1 * nch + b + k) * B + j];
     }
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < 16; k++) {
       v0 = max0 ? fmax(v0, a0[k]) : v0 + a0[k];
       v1 = max1 ? fmax(v1, a1[k]) : v1 + a1[k];
     }
@@ -441,7 +444,7 @@ __global__ __launch_bounds__(256) void k_bgroup(int64_t nch, int B,
   part2[(1 * ng + g) * B + j] = v1;
 }
 
-__global__ __launch_bounds__(256) void k_bfinalize(int64_t nch, int B,
+__global__ __launch_bounds__(256) void k_bfinalize(int64_t nch, int B, int G,
                                                    const double* __restrict__ part2,
                                                    int32_t* __restrict__ cst,
                                                    BCol* __restrict__ bc,
@@ -450,9 +453,10 @@ __global__ __launch_bounds__(256) void k_bfinalize(int64_t nch, int B,
   const int j = blockIdx.x;
   const int st = cst[j];
   if (!wanted(cst, j, fin_states(what))) return;  // uniform per block
-  const int64_t ng = (nch + BGROUP - 1) / BGROUP;
+  const int64_t ng = (nch + G - 1) / G;
   const bool max0 = fin_max0(what), max1 = fin_max1(what);
   double v0 = 0.0, v1 = 0.0;
+#pragma unroll 4
   for (int64_t b = threadIdx.x; b < ng; b += 256) {
     const double a0 = part2[(0 * ng + b) * B + j];
     const double a1 = part2[(1 * ng + b) * B + j];
@@ -551,6 +555,7 @@ __global__ __launch_bounds__(256) void k_buerr_finalize(
   const int j = blockIdx.x;
   if (!active[j]) return;
   double vd = 0.0, vx = 0.0;
+#pragma unroll 4
   for (int64_t b = threadIdx.x; b < nch; b += 256) {
     vd = fmax(vd, part[(0 * nch + b) * B + j]);
     vx = fmax(vx, part[(1 * nch + b) * B + j]);
@@ -634,9 +639,12 @@ static inline int64_t nchunks_host(int64_t n1) {
   return (n1 + ck - 1) / ck;
 }
 
+// stage-1 group size of the reductions (a function of the chunk count only)
+static inline int batch_group(int64_t nch) { return nch < 16384 ? 16 : 64; }
+
 int64_t batch_partials(int64_t n1) {
   const int64_t nch = nchunks_host(n1);
-  return 2 * nch + 2 * ((nch + BGROUP - 1) / BGROUP);  // chunk + group partials
+  return 2 * nch + 2 * ((nch + BGROUP_MIN - 1) / BGROUP_MIN);  // chunk + group partials
 }
 
 static inline unsigned chunk_blocks(int64_t n1, int B) {
@@ -726,11 +734,12 @@ void launch_bresid(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
 void launch_bfinalize(hipStream_t s, int64_t n1, int B, double* part,
                       int32_t* cst, BCol* bc, int what) {
   const int64_t nch = nchunks_host(n1);
-  const int64_t ng = (nch + BGROUP - 1) / BGROUP;
+  const int G = batch_group(nch);
+  const int64_t ng = (nch + G - 1) / G;
   double* part2 = part + 2 * nch * B;
   hipLaunchKernelGGL(k_bgroup, dim3((unsigned)((ng * B + 255) / 256)), dim3(256), 0,
-                     s, nch, B, part, part2, cst, what);
-  hipLaunchKernelGGL(k_bfinalize, dim3(B), dim3(256), 0, s, nch, B, part2, cst, bc,
+                     s, nch, B, G, part, part2, cst, what);
+  hipLaunchKernelGGL(k_bfinalize, dim3(B), dim3(256), 0, s, nch, B, G, part2, cst, bc,
                      what);
 }
 
