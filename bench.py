@@ -80,6 +80,10 @@ def parse():
                     help="mixed mode: comma-separated SRF list")
     ap.add_argument("--trials", type=int, default=64)
     ap.add_argument("--analyses", type=int, default=None)
+    ap.add_argument("--ref-seconds", type=int, default=20,
+                    help="analysis mode: seconds of the compiled reference's CPU sample")
+    ap.add_argument("--ref-procs", type=int, default=16,
+                    help="analysis mode: concurrent reference processes (one core each)")
     ap.add_argument("--rate", type=float, default=None,
                     help="analysis mode: Poisson arrival rate (1/s) of requests")
     ap.add_argument("--srf-parallel", type=int, default=1,
@@ -418,6 +422,8 @@ def mode_analysis(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
         latency_s=dict(p50=percentile(lat, 50), p99=percentile(lat, 99), max=max(lat)),
         analyses=rows_out, kernel=dict(name=kname, launches=nl,
                                        avg_ms=kms / max(nl, 1)), **extra)
+    cgs = [r.get("cg_tot", r.get("cg_total", 0)) for r in rows_out]
+    out["cg_iterations_per_analysis"] = sum(cgs) / max(len(cgs), 1)
     return out
 
 
@@ -518,6 +524,13 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     from specfem3d_geotech_amd import workloads
+
+    # the compiled reference on the CPU runs first, as child processes of a
+    # process that has not touched the GPU yet
+    ref_cpu = None
+    if a.mode == "analysis" and not a.no_cpu_baseline:
+        wl0 = workloads.get(a.workload)
+        ref_cpu = reference_cpu_sample(wl0, wl0.make(), a.ref_seconds, a.ref_procs)
     from specfem3d_geotech_amd.solver import Context
 
     if a.mode == "serve":
@@ -546,7 +559,83 @@ def main():
         ctx.close()
     if not a.no_cpu_baseline and a.mode not in ("analysis", "mixed"):
         res["cpu_baseline"] = cpu_baseline(spec, mesh, a.cpu_sample_iters)
+    if ref_cpu is not None:
+        # analyses/hour of the reference on these cores, from its measured
+        # PCG rate and this run's CG iterations per analysis; both the
+        # plastic body's share of the reference's time and (in the batched
+        # form) the elastic solves the engine reuses are left out, which
+        # favours the reference
+        cg_per = res.get("cg_iterations_per_analysis")
+        if cg_per:
+            ref_cpu["value"] = ref_cpu["procs"] * ref_cpu["cg_per_s_per_process"] * 3600.0 \
+                / cg_per
+            ref_cpu["unit"] = "FOS analyses/hour"
+            ref_cpu["cg_iterations_per_analysis"] = cg_per
+        res["cpu_baseline"] = ref_cpu
     print(json.dumps(res), flush=True)
+
+
+def reference_cpu_sample(wl, mesh, seconds, procs):
+    """The compiled reference (oracle/_ref/semgeotech_tap: the reference's own
+    Fortran sources built with amdflang -O2, plus a link-time tap that logs
+    each pcg_solver call's cg_iter and wall time) on this workload's input:
+    `procs` independent serial processes, one core each, stopped after
+    `seconds`.  Returns each process's PCG rate (CG iterations per second of
+    solver time, completed calls only), or None when the binary is absent."""
+    import shutil
+    import subprocess
+    import tempfile
+    from specfem3d_geotech_amd.mesh import write_reference_input
+    binp = os.path.join(ROOT, "oracle", "_ref", "semgeotech_tap")
+    if not os.path.exists(binp):
+        log("reference CPU sample skipped: oracle/_ref/semgeotech_tap is not built")
+        return None
+    ctl = dict(wl.control)
+    # the reference's parser keeps 80 characters of a value list
+    # (src/string_library.f90:547): the input lists the sweep's first 8 SRFs,
+    # more than a sample of this length reaches
+    srf = list(ctl["srf"])[:8]
+    work = tempfile.mkdtemp(prefix="refcpu_")
+    jobs = []
+    for i in range(procs):
+        d = os.path.join(work, f"p{i}")
+        sem = write_reference_input(mesh, d, "ref", cg_tol=ctl["cg_tol"],
+                                    cg_maxiter=ctl["cg_maxiter"], nl_tol=ctl["nl_tol"],
+                                    nl_maxiter=ctl["nl_maxiter"], srf=srf)
+        env = dict(os.environ, SGEO_TAP_DIR=d, SGEO_TAP_MODE="time", OMP_NUM_THREADS="1")
+        cmd = f"ulimit -s unlimited; exec timeout -s KILL {int(seconds)} {binp} {sem}"
+        jobs.append((subprocess.Popen(["bash", "-c", cmd], cwd=d, env=env,
+                                      stdout=subprocess.DEVNULL,
+                                      stderr=subprocess.DEVNULL), d))
+    log(f"reference CPU sample: {procs} processes for {seconds} s")
+    rates, neq = [], None
+    for p, d in jobs:
+        p.wait()
+        cg = t = 0.0
+        try:
+            with open(os.path.join(d, "pcg_times.txt")) as fh:
+                for ln in fh:
+                    _, it, ec, sec = ln.split()
+                    if int(ec) == 0:
+                        cg += int(it)
+                        t += float(sec)
+        except OSError:
+            pass
+        if t > 0:
+            rates.append(cg / t)
+    if not rates:
+        log("reference CPU sample: no PCG call completed")
+        return None
+    shutil.rmtree(work, ignore_errors=True)
+    from specfem3d_geotech_amd.spec import hex2spec
+    neq = hex2spec(mesh).neq
+    r = sum(rates) / len(rates)
+    return dict(kind="reference", cores=procs, procs=procs,
+                cg_per_s_per_process=r, dof_iterations_per_s=procs * r * neq,
+                sample=f"{procs} serial processes of the compiled reference "
+                       f"(amdflang -O2, its own sources) on the {mesh.nelmt}-element "
+                       f"input, {seconds} s each; completed pcg_solver calls timed by "
+                       "a link-time tap")
 
 
 def cpu_baseline(spec, mesh, iters):
