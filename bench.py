@@ -318,8 +318,11 @@ def mode_pcg(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
                     k_storage=f"{kf} {kp} (S={S})", reduction=red,
                     copy_kernel_gbs=round(copy_gbs, 1), ingest_s=round(prep_s, 3),
                     k_build_s=round(kbuild_s, 4)),
-        roofline=roofline(("k_ebe_packed" if kf == "packed" else "k_ebe_elem")
-                          + ("<double>" if kp == "f64" else "<float>"), kbytes, n, ms,
+        roofline=roofline({("packed", "f64"): "k_ebe_packed_pipe",
+                           ("packed", "f32"): "k_ebe_packed<float>",
+                           ("full", "f64"): "k_ebe_elem<double>",
+                           ("full", "f32"): "k_ebe_elem<float>"}.get((kf, kp), kf),
+                          kbytes, n, ms,
                           dict(bytes_model=f"E*({S}*{bk} + 81*4 + 2*81*8)")))
 
 
@@ -407,7 +410,7 @@ def mode_analysis(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
                                  cg_total=int(sum(res.cg_log))))
         dt_s = time.perf_counter() - t0
         extra = {}
-        kname = "k_ebe_packed<double>"
+        kname = "k_ebe_packed_pipe"
     nl, kms = ctx.profile(0)
     out = dict(
         metric=METRIC, value=nan * 3600.0 / dt_s, unit="FOS analyses/hour", n_gpus=1,

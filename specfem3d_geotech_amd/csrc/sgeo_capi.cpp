@@ -566,7 +566,8 @@ static void ebe_elem(sgeo_mesh* m, const double* p, double* Y, int skip_mode) {
     if (m->kprec == SGEO_K_F32)
       launch_ebe_packed_f32(s, m->nelmt, m->d_KP32, m->d_gdof, p, Y, sc, skip_mode);
     else
-      launch_ebe_packed_f64(s, m->nelmt, m->d_KP64, m->d_gdof, p, Y, sc, skip_mode);
+      launch_ebe_packed_f64(s, m->nelmt, m->ctx->n_cu, m->d_KP64, m->d_gdof, p, Y, sc,
+                            skip_mode);
   } else if (m->kprec == SGEO_K_F32) {
     launch_ebe_elem_f32k(s, m->nelmt, m->d_K32, m->d_gdof, p, Y, sc, skip_mode);
   } else {

@@ -55,7 +55,7 @@ void launch_ebe_elem_f64(hipStream_t s, int32_t nelmt, const double* K,
 void launch_ebe_elem_f32k(hipStream_t s, int32_t nelmt, const float* K,
                           const int32_t* gdof, const double* p, double* Y,
                           const PcgScalars* sc, int skip_mode);
-void launch_ebe_packed_f64(hipStream_t s, int32_t nelmt, const double* KP,
+void launch_ebe_packed_f64(hipStream_t s, int32_t nelmt, int n_cu, const double* KP,
                            const int32_t* gdof, const double* p, double* Y,
                            const PcgScalars* sc, int skip_mode);
 void launch_ebe_packed_f32(hipStream_t s, int32_t nelmt, const float* KP,

@@ -108,6 +108,83 @@ __global__ __launch_bounds__(128) void k_ebe_packed(
   Y[e * 81 + r] = a;
 }
 
+// Persistent, pipelined form of k_ebe_packed<double>: a few 128-thread
+// blocks per CU walk the elements.  While element e is multiplied from LDS,
+// the next element's packed K is in flight into registers (16-byte
+// non-temporal loads) and its gathered p into a register, with gdof read
+// one element further ahead; both reach LDS after the product, behind the
+// block's barrier.  Row r sums c ascending as k_ebe_packed does: the same
+// bits.
+__global__ __launch_bounds__(128) void k_ebe_packed_pipe(
+    int32_t nelmt, const double* __restrict__ KP, const int32_t* __restrict__ gdof,
+    const double* __restrict__ p, double* __restrict__ Y,
+    const PcgScalars* __restrict__ sc, int skip_mode) {
+  if (skip_mode == 1 && sc->done) return;
+  if (skip_mode == 2 && !sc->u_nonzero) return;
+  typedef double v2d __attribute__((ext_vector_type(2)));
+  constexpr int NV = SGEO_KPACK / 2;        // 16-byte pieces per element
+  constexpr int PT = (NV + 127) / 128;      // per thread
+  __shared__ __align__(16) double u[SGEO_KPACK];
+  __shared__ double pe[2][81];
+  const int t = threadIdx.x;
+  const int64_t stride = gridDim.x;
+  int64_t e = blockIdx.x;
+  v2d kr[PT];
+  int32_t gn = 0;  // gdof of the next element
+  if (e < nelmt) {
+    const v2d* src = reinterpret_cast<const v2d*>(KP + e * SGEO_KPACK);
+#pragma unroll
+    for (int i = 0; i < PT; i++) {
+      const int q = t + 128 * i;
+      if (q < NV) kr[i] = __builtin_nontemporal_load(src + q);
+    }
+    if (t < 81) pe[0][t] = p[gdof[e * 81 + t]];
+    if (t < 81 && e + stride < nelmt) gn = gdof[(e + stride) * 81 + t];
+  }
+  const int r = t;
+  const int rbase = r * (r + 1) / 2;
+  int cur = 0;
+  for (; e < nelmt; e += stride) {
+    v2d* dst = reinterpret_cast<v2d*>(u);
+#pragma unroll
+    for (int i = 0; i < PT; i++) {
+      const int q = t + 128 * i;
+      if (q < NV) dst[q] = kr[i];
+    }
+    __syncthreads();  // K of e and pe[cur] visible
+    const int64_t en = e + stride;
+    double pn = 0.0;
+    if (en < nelmt) {  // in flight during the product
+      const v2d* src = reinterpret_cast<const v2d*>(KP + en * SGEO_KPACK);
+#pragma unroll
+      for (int i = 0; i < PT; i++) {
+        const int q = t + 128 * i;
+        if (q < NV) kr[i] = __builtin_nontemporal_load(src + q);
+      }
+      if (t < 81) pn = p[gn];
+    }
+    int32_t gnn = 0;
+    if (t < 81 && en + stride < nelmt) gnn = gdof[(en + stride) * 81 + t];
+    if (t < 81) {
+      const double* pl = pe[cur];
+      double a = 0.0;
+#pragma unroll 9
+      for (int c = 0; c < 81; c++) {
+        const int idx = (r <= c) ? (c * (c + 1) / 2 + r) : (rbase + c);
+        a = a + u[idx] * pl[c];
+      }
+      Y[e * 81 + r] = a;
+      pe[cur ^ 1][t] = pn;
+    }
+    gn = gnn;
+    __syncthreads();  // u and pe[cur] free, pe[cur ^ 1] visible
+    cur ^= 1;
+  }
+}
+
+// blocks per CU of the persistent FP64 product (LDS 28 KB per block)
+#define EBE_PIPE_BLOCKS_PER_CU 5
+
 // full (reference layout) -> packed upper triangle, zero padding
 template <typename KT>
 __global__ void k_pack_upper(int32_t nelmt, const double* __restrict__ K,
@@ -491,12 +568,14 @@ void launch_ebe_elem_f32k(hipStream_t s, int32_t nelmt, const float* K,
                      nelmt, K, gdof, p, Y, sc, skip_mode);
 }
 
-void launch_ebe_packed_f64(hipStream_t s, int32_t nelmt, const double* KP,
+void launch_ebe_packed_f64(hipStream_t s, int32_t nelmt, int n_cu, const double* KP,
                            const int32_t* gdof, const double* p, double* Y,
                            const PcgScalars* sc, int skip_mode) {
   if (nelmt <= 0) return;
-  hipLaunchKernelGGL(k_ebe_packed<double>, dim3(nelmt), dim3(128), 0, s, nelmt,
-                     KP, gdof, p, Y, sc, skip_mode);
+  int64_t g = (int64_t)n_cu * EBE_PIPE_BLOCKS_PER_CU;
+  if (g > nelmt) g = nelmt;
+  hipLaunchKernelGGL(k_ebe_packed_pipe, dim3((unsigned)g), dim3(128), 0, s, nelmt, KP,
+                     gdof, p, Y, sc, skip_mode);
 }
 
 void launch_ebe_packed_f32(hipStream_t s, int32_t nelmt, const float* KP,

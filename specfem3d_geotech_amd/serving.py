@@ -112,7 +112,10 @@ class Server:
             return slope3d_mixed(ctx, mesh, ncol=self.slope_columns, defer_nl=defer,
                                  **req.control)
         from .excavation import excavation3d
-        return excavation3d(ctx, mesh, **req.control)
+        from .solver import KFMT_PACKED
+        # the packed FP64 operator: half the stiffness bytes of the full
+        # (reference-layout) one, the same arithmetic to rounding
+        return excavation3d(ctx, mesh, kfmt=KFMT_PACKED, **req.control)
 
     def _worker(self, req: Request, done: threading.Event):
         ctx = None
