@@ -305,23 +305,40 @@ __global__ __launch_bounds__(256) void k_bassemble_dot(
   }
   if (!c.valid || cst[c.j] != BCOL_INNER) return;
   double s = 0.0, m = 0.0;
-  for (int64_t g = c.g0; g < c.g1; g++) {
+  // kp[g] = the slots' Y values summed in slot order (the reference's
+  // element order); two equations per trip keep twice the loads in flight
+  auto gather = [&](int64_t g, float* v, int32_t& q0, int32_t& q1) {
+    q0 = g > 0 ? rowptr[g] : 0;
+    q1 = g > 0 ? rowptr[g + 1] : 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      v[k] = (q0 + k < q1) ? Y[(int64_t)slots[q0 + k] * B + c.j] : 0.0f;
+  };
+  auto finish = [&](int64_t g, const float* v, int32_t q0, int32_t q1) {
     float acc = 0.0f;
-    if (g > 0) {
-      const int32_t q0 = rowptr[g], q1 = rowptr[g + 1];
-      float v[8];
 #pragma unroll
-      for (int k = 0; k < 8; k++)
-        v[k] = (q0 + k < q1) ? Y[(int64_t)slots[q0 + k] * B + c.j] : 0.0f;
-#pragma unroll
-      for (int k = 0; k < 8; k++)
-        if (q0 + k < q1) acc = acc + v[k];
-      for (int32_t q = q0 + 8; q < q1; q++) acc = acc + Y[(int64_t)slots[q] * B + c.j];
-    }
+    for (int k = 0; k < 8; k++)
+      if (q0 + k < q1) acc = acc + v[k];
+    for (int32_t q = q0 + 8; q < q1; q++) acc = acc + Y[(int64_t)slots[q] * B + c.j];
     kp[g * B + c.j] = acc;
     const double pg = (double)p[g * B + c.j];
     s = s + pg * (double)acc;
     m = fmax(m, fabs(pg));
+  };
+  int64_t g = c.g0;
+  for (; g + 1 < c.g1; g += 2) {
+    float va[8], vb[8];
+    int32_t a0, a1, b0, b1;
+    gather(g, va, a0, a1);
+    gather(g + 1, vb, b0, b1);
+    finish(g, va, a0, a1);
+    finish(g + 1, vb, b0, b1);
+  }
+  if (g < c.g1) {
+    float va[8];
+    int32_t a0, a1;
+    gather(g, va, a0, a1);
+    finish(g, va, a0, a1);
   }
   const int64_t nch = nchunks(n1, ck);
   part[(0 * nch + c.chunk) * B + c.j] = s;
