@@ -100,6 +100,17 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
   // zero padding, once: k columns 81..83 (rows 0..80)
   for (int q = t; q < 3 * 81; q += 256) Kf[(81 + q / 81) * BMV_LDK + q % 81] = 0.0f;
   if (t >= 81 && t < BMV_KC) { sg[0][t] = 0; sg[1][t] = 0; }
+  // LDS addresses of this thread's packed entries q = t + 256 i (fixed for
+  // the kernel): K(r, c) at c*LDK + r and its mirror at r*LDK + c, both
+  // below 2^16, packed in one word
+  uint32_t kaddr[KPT];
+#pragma unroll
+  for (int i = 0; i < KPT; i++) {
+    const int q = t + 256 * i;
+    const int c = q < 3321 ? tri_col(q) : 0;
+    const int r = q < 3321 ? q - c * (c + 1) / 2 : 0;
+    kaddr[i] = (uint32_t)(c * BMV_LDK + r) | ((uint32_t)(r * BMV_LDK + c) << 16);
+  }
   // prologue: K and gdof of the first element
   float kreg[KPT];
   int32_t greg = 0;
@@ -124,12 +135,9 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
     // unpack the packed upper triangle (and its mirror) of element e
 #pragma unroll
     for (int i = 0; i < KPT; i++) {
-      const int q = t + 256 * i;
-      if (q < 3321) {
-        const int c = tri_col(q);
-        const int r = q - c * (c + 1) / 2;
-        Kf[c * BMV_LDK + r] = kreg[i];
-        Kf[r * BMV_LDK + c] = kreg[i];
+      if (t + 256 * i < 3321) {
+        Kf[kaddr[i] & 0xffffu] = kreg[i];
+        Kf[kaddr[i] >> 16] = kreg[i];
       }
     }
     __syncthreads();
