@@ -57,6 +57,8 @@ def lib():
                                                   C.c_int32, i32, d])
         _LIB.orc_slope3d.restype = C.c_int
         _LIB.orc_nodal_fields.argtypes = [C.c_int32, C.c_int32, i32, d, d, d, d]
+        _LIB.orc_overburden_stress.argtypes = [C.c_int32, i32, i32, d, d, C.c_double,
+                                               C.c_double, C.c_double, d]
         _LIB.orc_elastic_stress.argtypes = [C.c_int32, i32, d, i32, i32, d, d, d, d]
         _LIB.orc_excavation_load_nodal.argtypes = [C.c_int32, i32, d, i32, i32, d, d,
                                                    C.c_int32, d]
@@ -276,9 +278,19 @@ def renumber_gdof(gdof_node, node_alive):
     return out.reshape(g.shape), int(free.sum())
 
 
+def overburden_stress(spec, gam, z0, p0, k0):
+    """stress (E,27,6) of s0_type 1 (src/postprocess.f90:10-33)."""
+    st = np.zeros((spec.nelmt, 27, 6))
+    z = np.ascontiguousarray(np.asarray(spec.g_coord, dtype=np.float64)[:, 2])
+    lib().orc_overburden_stress(spec.nelmt, _p(_i32(spec.g_num), C.c_int32),
+                                _p(_i32(spec.mat_id), C.c_int32), _p(z), _p(_f64(gam)),
+                                float(z0), float(p0), float(k0), _p(st))
+    return st
+
+
 def excavation3d(spec, mesh, *, cg_tol, cg_maxiter, nl_tol, nl_maxiter, srf,
                  nexcav, nexcavid, excavid, ninc, phinu=False, s0_type=0,
-                 on_pcg=None):
+                 on_pcg=None, z0=0.0, s0=0.0, k0=0.0):
     """The reference's staged excavation (src/excavation.F90:13-700), one SRF
     (the reference supports only nsrf = 1 there).  Returns dict(rows=[(stage,
     cg_tot, nl_tot, uxmax, umax, fmax)], stress0, nodalu, cg_log, fields):
@@ -286,26 +298,29 @@ def excavation3d(spec, mesh, *, cg_tol, cg_maxiter, nl_tol, nl_maxiter, srf,
     state, src/excavation.F90:296-323; then :655-687): node_intact (0-based
     ids), nodalu (nnode_intact, 3), scf, vmeps (nnode_intact,) and the
     nodal fields of the stress (nodal_fields) at those nodes."""
-    if s0_type != 0:
-        raise NotImplementedError("s0_type 1 (overburden) not restated")
+    if s0_type not in (0, 1):
+        raise ValueError(f"s0_type {s0_type} not supported")
     gam, ym, nu, phi, coh, psi = mesh.material_arrays()
     nmat, E = len(gam), spec.nelmt
     g_num = np.asarray(spec.g_num)
     mat = np.asarray(spec.mat_id)
     gdof_node = np.asarray(spec.gdof).copy()
     cg_log = []
-    # initial stress: elastic solve under gravity on the whole mesh
-    km, dp, ext = stiffness_bodyload(spec, gam, nu, ym)
-    dinv = np.zeros_like(dp); dinv[1:] = 1.0 / dp[1:]
-    x, it, ec = pcg_solver(spec.neq, km, np.zeros(spec.neq + 1), ext, dinv,
-                           spec.gdof_elmt, cg_tol, cg_maxiter)
-    if ec != 0:
-        raise RuntimeError("ERROR: PCG solver doesn't converge!")
-    cg_log.append(it)
-    x[0] = 0.0
-    stress = elastic_stress(spec, ym, nu, x)
+    if s0_type == 0:
+        # initial stress: elastic solve under gravity on the whole mesh
+        km, dp, ext = stiffness_bodyload(spec, gam, nu, ym)
+        dinv = np.zeros_like(dp); dinv[1:] = 1.0 / dp[1:]
+        x, it, ec = pcg_solver(spec.neq, km, np.zeros(spec.neq + 1), ext, dinv,
+                               spec.gdof_elmt, cg_tol, cg_maxiter)
+        if ec != 0:
+            raise RuntimeError("ERROR: PCG solver doesn't converge!")
+        cg_log.append(it)
+        x[0] = 0.0
+        stress = elastic_stress(spec, ym, nu, x)
+        del km
+    else:  # overburden (src/excavation.F90:238-241)
+        stress = overburden_stress(spec, gam, z0, s0, k0)
     stress0 = stress.copy()
-    del km
     s = float(srf[0])
     nodalu = np.zeros((spec.nnode, 3))
     ismat = np.ones(nmat, bool)

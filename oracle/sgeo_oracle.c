@@ -896,6 +896,27 @@ void orc_nodal_fields(int32_t nelmt, int32_t nnode, const int32_t *g_num,
 }
 
 /* ------------------------------------------------------------------------ */
+/* overburden initial stress (src/postprocess.f90:10-33), s0_type 1         */
+/* ------------------------------------------------------------------------ */
+
+void orc_overburden_stress(int32_t nelmt, const int32_t *g_num, const int32_t *mat_id,
+                           const double *node_z, const double *gam, double z0,
+                           double p0, double k0, double *stress) {
+  /* stress_elmt starts at zero (src/excavation.F90:180); only the normal
+   * components are set */
+  memset(stress, 0, sizeof(double) * 6 * NG * (size_t)nelmt);
+  for (int64_t e = 0; e < nelmt; e++)
+    for (int i = 0; i < NG; i++) {
+      const double z = node_z[g_num[e * NG + i]];
+      const double szz = p0 - gam[mat_id[e] - 1] * fabs(z - z0); /* compression - */
+      double *st = stress + (e * NG + i) * 6;
+      st[2] = szz;
+      st[0] = k0 * szz;
+      st[1] = k0 * szz;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* excavation helpers (src/postprocess.f90:40-79;                           */
 /*                     src/excavation_library.f90:186-236)                  */
 /* ------------------------------------------------------------------------ */

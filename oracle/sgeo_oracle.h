@@ -150,6 +150,13 @@ int orc_slope3d(int32_t nelmt, int32_t nnode, int32_t neq, const double *xc,
                 double *fmax, double *x_out, int32_t *cg_log,
                 int32_t cg_log_cap, int32_t *n_cg_log, double *stress_out);
 
+/* Overburden initial stress (src/postprocess.f90:10-33): at every GLL point
+ * szz = p0 - gam(mat) |z - z0| with z the point's node z (node_z[g_num]),
+ * sxx = syy = k0 szz, the rest zero.  stress is (E,27,6). */
+void orc_overburden_stress(int32_t nelmt, const int32_t *g_num, const int32_t *mat_id,
+                           const double *node_z, const double *gam, double z0,
+                           double p0, double k0, double *stress);
+
 /* Nodal result fields of a trial's final stress state (src/slope.F90:
  * 203-210, 418-437; src/postprocess.f90:146-183): stress(E,27,6) is
  * stress_elmt; sig[n*6+c] the nodal mean in element order, psig[n*3+k]

@@ -4,7 +4,9 @@ path.
 The reference's stage logic stays on the host in its order: intact and void
 element sets, intact nodes, the renumbered equations (integer work,
 src/excavation_library.f90:17-111), the nodal displacement sum and the
-stage summary row.  Everything numeric runs on the GPU through libsgeo.so:
+stage summary row, and the overburden initial stress of s0_type 1 (a
+closed form per GLL point, src/postprocess.f90:10-33).  Everything else
+numeric runs on the GPU through libsgeo.so:
 - the initial gravity solve and the elastic stress (s0_type 0);
 - each stage's excavation load, built from the removed elements' stress;
 - each stage's stiffness, built on the intact elements with the reduced nu;
@@ -47,6 +49,22 @@ def renumber(gdof_node: np.ndarray, alive: np.ndarray):
     return out.reshape(g.shape), int(free.sum())
 
 
+def overburden_stress(spec: SpecMesh, gam: np.ndarray, z0: float, p0: float,
+                      k0: float) -> np.ndarray:
+    """Initial stress of s0_type 1 (src/postprocess.f90:10-33): at GLL point
+    i of element e, with z the z coordinate of its node, szz = p0 - gam |z -
+    z0| (compression negative), sxx = syy = k0 szz, shears zero; (E, 27, 6)
+    in the reference's stress order."""
+    z = np.asarray(spec.g_coord, dtype=np.float64)[np.asarray(spec.g_num), 2]
+    g = np.asarray(gam, dtype=np.float64)[np.asarray(spec.mat_id) - 1][:, None]
+    szz = p0 - g * np.abs(z - z0)
+    st = np.zeros((spec.nelmt, 27, 6))
+    st[:, :, 2] = szz
+    st[:, :, 0] = k0 * szz
+    st[:, :, 1] = k0 * szz
+    return st
+
+
 def sub_spec(spec: SpecMesh, elmt: np.ndarray, gdof_node: np.ndarray,
              gdof_elmt: np.ndarray, neq: int) -> SpecMesh:
     return SpecMesh(nelmt=int(elmt.size), nnode=spec.nnode, neq=neq,
@@ -70,33 +88,42 @@ def excavation3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
                  nexcavid: Sequence[int], excavid: Sequence[int], ninc: int,
                  phinu: bool = False, s0_type: int = 0, kprec: int = K_F64,
                  kfmt: int = KFMT_FULL, spec: Optional[SpecMesh] = None,
-                 progress=None, output=None) -> ExcavationResult:
+                 progress=None, output=None, z0: float = 0.0, s0: float = 0.0,
+                 k0: float = 0.0) -> ExcavationResult:
     """Staged excavation on the GPU for one SRF.  The reference runs
     excavation with nsrf = 1 only (it reallocates and closes per-SRF state
     inside its SRF loop); the first SRF is used.  `output`
     (output.ExcavationOutput) receives the in-situ step and every stage's
     row, displacement and nodal fields (computed on the device) for the
-    reference's result files."""
-    if s0_type != 0:
-        raise NotImplementedError("initial stress type 1 (overburden) is not built")
+    reference's result files.  s0_type 0 takes the in-situ stress from a
+    gravity solve, s0_type 1 from the overburden relation with datum z0,
+    stress s0 there and lateral ratio k0 (the input's stress0 line,
+    src/read_input.f90:236-250; k0 defaults to 0 as the reference leaves it
+    unset)."""
+    if s0_type not in (0, 1):
+        raise ValueError(f"s0_type {s0_type} not supported (src/excavation.F90:242)")
     spec = spec or hex2spec_device(ctx, mesh)
     gam, ym, nu, phi, coh, psi = mesh.material_arrays()
     nmat, E = len(gam), spec.nelmt
     g_num = np.asarray(spec.g_num)
     mat = np.asarray(spec.mat_id)
     cg_log: List[int] = []
-    # initial stress from the gravity solve on the whole mesh
-    # (src/excavation.F90:180-232)
+    # initial stress: the gravity solve on the whole mesh (s0_type 0,
+    # src/excavation.F90:180-237) or the overburden relation (s0_type 1,
+    # :238-241)
     dm = DeviceMesh(ctx, spec, nmat)
     try:
-        ext, _ = dm.build_stiffness(gam, ym, nu, kprec=K_F64)
-        dm.set_operator(kprec, kfmt)
-        x, it, ec = dm.solve(ext, None, cg_tol, cg_maxiter)
-        if ec != 0:
-            raise PCGNotConverged("ERROR: PCG solver doesn't converge!")
-        cg_log.append(it)
-        x[0] = 0.0
-        stress = dm.elastic_stress(ym, nu, x)
+        if s0_type == 0:
+            ext, _ = dm.build_stiffness(gam, ym, nu, kprec=K_F64)
+            dm.set_operator(kprec, kfmt)
+            x, it, ec = dm.solve(ext, None, cg_tol, cg_maxiter)
+            if ec != 0:
+                raise PCGNotConverged("ERROR: PCG solver doesn't converge!")
+            cg_log.append(it)
+            x[0] = 0.0
+            stress = dm.elastic_stress(ym, nu, x)
+        else:
+            stress = overburden_stress(spec, gam, z0, s0, k0)
         if output is not None:  # step 0 (src/excavation.F90:296-323)
             output.in_situ(fields=lambda: dm.nodal_fields(stress))
     finally:

@@ -305,6 +305,33 @@ def output_case():
     print("output", names)
 
 
+def excavation_overburden_case():
+    """The small pit with the overburden initial stress (stress0: type=1,
+    src/postprocess.f90:10-33; src/excavation.F90:238-241) instead of the
+    gravity solve: the reference's stage rows and every pcg_solver call's
+    cg_iter (tap, time mode)."""
+    soil = Material(gam=20.0, ym=1e5, nu=0.3, phi=25.0, coh=15.0)
+    mesh = pit_mesh(6, 4, 4, lx=12.0, ly=8.0, lz=8.0, pit=(3.0, 9.0, 2.0, 6.0),
+                    nstage=2, stage_depth=2.0, soil=soil)
+    work = tempfile.mkdtemp(prefix="golden_exob_")
+    s0 = dict(z0=8.0, s0=-5.0, k0=0.5)
+    sem = write_reference_input(
+        mesh, work, "pit_ob", cg_tol=1e-8, cg_maxiter=5000, nl_tol=1e-5, nl_maxiter=100,
+        srf=[1.0], extra_control=", nexcav=2, excavid=2 3, ninc=2",
+        extra_lines=f"stress0: type=1, z0={s0['z0']!r}, s0={s0['s0']!r}, k0={s0['k0']!r}\n")
+    run_tap(sem, work, mode="time")
+    summ = parse_stage_summary(find_summary(work))
+    times = np.loadtxt(os.path.join(work, "pcg_times.txt"), ndmin=2)
+    out = dict(mesh_arrays(mesh))
+    out.update(summary=summ, cg_log=times[:, 1].astype(np.int32),
+               control=np.array([1e-8, 5000, 1e-5, 100]), srf=np.array([1.0]),
+               excav=np.array([2, 2, 2, 3], np.int32),
+               stress0=np.array([s0["z0"], s0["s0"], s0["k0"]]))
+    np.savez_compressed(os.path.join(HERE, "pit_overburden_6x4x4.npz"), **out)
+    shutil.rmtree(work)
+    print("excavation_overburden", summ.tolist(), len(out["cg_log"]), "calls")
+
+
 def output_excavation_case():
     """The reference's own output files for a small pit excavated in two
     stages with two load increments each, every result field saved: the
@@ -474,6 +501,7 @@ if __name__ == "__main__":
              sys.argv[2] if len(sys.argv) > 2 else None),
          "excavation_spread": excavation_example_spread,
          "list_format": list_format_case, "output": output_case,
-         "output_excavation": output_excavation_case}[sys.argv[1]]()
+         "output_excavation": output_excavation_case,
+         "excavation_overburden": excavation_overburden_case}[sys.argv[1]]()
     else:
         main()

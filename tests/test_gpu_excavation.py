@@ -193,3 +193,23 @@ def test_excavation_output_files(ctx_seq, tmp_path):
                              1e-9 * np.abs(b).max())
             assert a.shape == b.shape and np.all(np.abs(a - b) <= tol), \
                 (n, float(np.abs(a - b).max()))
+
+
+def test_excavation_overburden_matches_reference(ctx_seq):
+    """The pit with the overburden initial stress (s0_type 1) on the device,
+    reference-order reductions: the compiled reference's stage rows (cg_tot
+    and nl_tot exactly, displacements within 1e-10, fmax within 1e-8) and
+    every PCG call's iteration count."""
+    z = golden("pit_overburden_6x4x4")
+    mesh = mesh_from_golden(z)
+    z0, s0, k0 = (float(v) for v in z["stress0"])
+    r = excavation3d(ctx_seq, mesh, s0_type=1, z0=z0, s0=s0, k0=k0, **_control(z))
+    ref = z["summary"]
+    assert len(r.rows) == ref.shape[0]
+    for row, rr in zip(r.rows, ref):
+        print(row, rr.tolist(), flush=True)
+        assert row["stage"] == rr[0] and row["cg_tot"] == rr[1] and row["nl_tot"] == rr[2]
+        assert abs(row["uxmax"] - rr[3]) <= 1e-10 * rr[3]
+        assert abs(row["umax"] - rr[4]) <= 1e-10 * rr[4]
+        assert abs(row["fmax"] - rr[5]) <= 1e-8 * max(1.0, abs(rr[5]))
+    assert r.cg_log == list(z["cg_log"])

@@ -97,3 +97,16 @@ def test_capi_library_loads_without_gpu():
     import ctypes
     lib = ctypes.CDLL(os.path.join(ROOT, "specfem3d_geotech_amd", "libsgeo.so"))
     assert hasattr(lib, "sgeo_pcg_solver")
+
+
+def test_overburden_stress_matches_oracle():
+    """The excavation driver's s0_type 1 initial stress (host, closed form)
+    equals the oracle's restatement bit for bit."""
+    from specfem3d_geotech_amd.excavation import overburden_stress
+    z = golden("pit_overburden_6x4x4")
+    mesh = mesh_from_golden(z)
+    spec = hex2spec(mesh)
+    gam = mesh.material_arrays()[0]
+    z0, s0, k0 = (float(v) for v in z["stress0"])
+    assert np.array_equal(overburden_stress(spec, gam, z0, s0, k0),
+                          O.overburden_stress(spec, gam, z0, s0, k0))

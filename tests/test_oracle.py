@@ -347,6 +347,40 @@ def test_excavation_matches_reference():
     assert np.array_equal(f, z["stage1_f"])
 
 
+def test_excavation_overburden_matches_reference():
+    """The pit with the overburden initial stress (stress0: type=1,
+    src/postprocess.f90:10-33) instead of the gravity solve: the oracle's
+    stage rows and every PCG call's iteration count equal the compiled
+    reference's."""
+    z = golden("pit_overburden_6x4x4")
+    mesh = mesh_from_golden(z)
+    spec = hex2spec(mesh)
+    z0, s0, k0 = (float(v) for v in z["stress0"])
+    r = O.excavation3d(spec, mesh, s0_type=1, z0=z0, s0=s0, k0=k0, **_excav_control(z))
+    ref = z["summary"]
+    assert len(r["rows"]) == ref.shape[0]
+    for row, rr in zip(r["rows"], ref):
+        print(row, rr.tolist())
+        assert list(row) == rr.tolist()
+    assert r["cg_log"] == list(z["cg_log"])
+
+
+def test_overburden_stress_closed_form():
+    """szz = s0 - gamma |z - z0| at every GLL point's node, sxx = syy = k0 szz,
+    shears zero: a linear stress profile under the datum."""
+    z = golden("pit_overburden_6x4x4")
+    mesh = mesh_from_golden(z)
+    spec = hex2spec(mesh)
+    gam = mesh.material_arrays()[0]
+    st = O.overburden_stress(spec, gam, 8.0, -5.0, 0.5)
+    zz = np.asarray(spec.g_coord)[np.asarray(spec.g_num), 2]
+    g = gam[np.asarray(spec.mat_id) - 1][:, None]
+    assert np.allclose(st[:, :, 2], -5.0 - g * np.abs(zz - 8.0), rtol=0, atol=1e-12)
+    assert np.array_equal(st[:, :, 0], 0.5 * st[:, :, 2])
+    assert np.array_equal(st[:, :, 1], st[:, :, 0])
+    assert not np.any(st[:, :, 3:])
+
+
 def test_excavation_load_is_equilibrium_of_removed_stress():
     """Closed-form check of excavation_load_nodal: with zero stress in the
     removed elements, the load is the weight of the removed soil (its vertical
