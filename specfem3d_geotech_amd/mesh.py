@@ -247,8 +247,8 @@ def write_reference_input(mesh: HexMesh, out_dir: str, head: str, *,
         fh.write("# material properties (id,domain,type,gamma,ym,nu,phi,coh,psi)\n")
         fh.write(f"{len(mesh.materials)}\n")
         for i, m in enumerate(mesh.materials):
-            fh.write(f"{i + 1}, 1, 0, {m.gam!r}, {m.ym!r}, {m.nu!r}, "
-                     f"{m.phi!r}, {m.coh!r}, {m.psi!r}\n")
+            v = [float(a) for a in (m.gam, m.ym, m.nu, m.phi, m.coh, m.psi)]
+            fh.write(f"{i + 1}, 1, 0, " + ", ".join(repr(a) for a in v) + "\n")
     for comp in ("ux", "uy", "uz"):
         with open(p(f"ssbc{comp}"), "w") as fh:
             pairs = mesh.bc[comp]
@@ -267,8 +267,8 @@ def write_reference_input(mesh: HexMesh, out_dir: str, head: str, *,
         fh.write(f"bc: uxfile='{head}_ssbcux', uyfile='{head}_ssbcuy', "
                  f"uzfile='{head}_ssbcuz'\n")
         fh.write(f"material: matfile='{head}_material_list', matpath='{out_dir}/'\n")
-        fh.write(f"control: cg_tol={cg_tol!r}, cg_maxiter={cg_maxiter}, "
-                 f"nl_tol={nl_tol!r}, nl_maxiter={nl_maxiter}, nsrf={len(srf)}, "
+        fh.write(f"control: cg_tol={float(cg_tol)!r}, cg_maxiter={int(cg_maxiter)}, "
+                 f"nl_tol={float(nl_tol)!r}, nl_maxiter={int(nl_maxiter)}, nsrf={len(srf)}, "
                  f"srf={srf_s}{extra_control}\n")
         if extra_lines:
             fh.write(extra_lines)
@@ -290,8 +290,11 @@ def read_reference_input(sem_path: str) -> tuple:
     """Read a serial `.sem` file and the mesh files it names.
 
     Supports the keys the slope/excavation paths use (preinfo, mesh, bc,
-    material, control); values follow src/read_input.f90:123-409.  Returns
-    (HexMesh, control dict)."""
+    material, control, stress0, save); values follow
+    src/read_input.f90:123-409.  Returns (HexMesh, control dict); the
+    control dict also holds `save` (the saved fields' keys) and `out_path`
+    (as written, relative to the working directory as the reference takes
+    it)."""
     base_dir = os.path.dirname(os.path.abspath(sem_path))
     lines: List[str] = []
     with open(sem_path) as fh:
@@ -365,6 +368,14 @@ def read_reference_input(sem_path: str) -> tuple:
                        ninc=int(ctl.get("ninc", "1")))
     st0 = kv.get("stress0", {})
     control["s0_type"] = int(float(st0.get("type", "0")))
+    if control["s0_type"] == 1:  # src/read_input.f90:243-248 (k0 unset: 0)
+        control.update(z0=float(st0["z0"]), s0=float(st0["s0"]),
+                       k0=float(st0.get("k0", "0")))
+    # result fields asked for (src/read_input.f90:380-403) and where they go
+    sv = kv.get("save", {})
+    control["save"] = [k for k in ("disp", "stress", "psigma", "nsigma", "scf", "vmeps")
+                       if sv.get(k, "0").strip() == "1"]
+    control["out_path"] = pre.get("out_path", "./output")
     mesh = HexMesh(coord=coord, conn=conn, mat_id=mat_id, materials=mats, bc=bc,
                    meta=dict(kind="file", path=sem_path))
     return mesh, control

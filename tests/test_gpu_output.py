@@ -178,3 +178,58 @@ def test_device_nodal_fields_match_oracle():
         assert np.all(np.abs(f[k].astype(np.float64) - r) <=
                       np.spacing(np.abs(r)).astype(np.float64)), k
     assert not np.any(f["vmeps"])
+
+
+def _compare_run_files(out_dir, z, nrow_fields=6):
+    """Every file of a run against the reference's: names, case/geometry
+    bytes, FP32 fields within one spacing, summary text before the rows
+    and the rows' numbers within the FP64 sweep bars."""
+    names = [str(n) for n in z["file_names"]]
+    assert sorted(os.listdir(out_dir)) == names
+    for n in names:
+        ref, mine = z["file:" + n].tobytes(), open(os.path.join(out_dir, n), "rb").read()
+        if n.endswith("_summary"):
+            va, vb = _summary_values(mine.decode()), _summary_values(ref.decode())
+            assert len(va) == len(vb)
+            for a, b in zip(va, vb):
+                if isinstance(b, float) and b != int(b):
+                    assert abs(a - b) <= 1e-8 * max(1.0, abs(b)), (a, b)
+                else:
+                    assert a == b, (a, b)
+        elif n.endswith(".geo") or n.endswith(".case"):
+            assert mine == ref, n
+        else:
+            assert mine[:244] == ref[:244], n
+            assert _fp32_close(mine[244:], ref[244:]), n
+
+
+def test_run_input_files_match_reference(tmp_path):
+    """driver.run_input on reference input files written for the two golden
+    runs (the 4x2x3 slope sweep and the two-stage pit, every field saved):
+    the result files land under the file's out_path and head and match the
+    compiled reference's."""
+    from specfem3d_geotech_amd.driver import run_input
+    from specfem3d_geotech_amd.mesh import write_reference_input
+
+    ctx = Context(0)
+    try:
+        ctx.set_reduction(REDUCE_SEQUENTIAL)
+        z = golden("output_slope_4x2x3")
+        sem = write_reference_input(_mesh(z), str(tmp_path / "s"), "slope_out",
+                                    out_path=str(tmp_path / "s" / "output"), srf=SRF,
+                                    save=SAVE_ALL, **CTL)
+        r = run_input(sem, ctx)
+        print("slope", r.rows, flush=True)
+        _compare_run_files(str(tmp_path / "s" / "output"), z)
+        z = golden("output_pit_6x4x4")
+        nexcav, ninc, *excavid = [int(v) for v in z["excav"]]
+        sem = write_reference_input(
+            _mesh(z), str(tmp_path / "e"), "pit_out", out_path=str(tmp_path / "e" / "output"),
+            cg_tol=1e-8, cg_maxiter=5000, nl_tol=1e-5, nl_maxiter=100, srf=[1.0],
+            extra_control=f", nexcav={nexcav}, excavid={' '.join(map(str, excavid))}, "
+                          f"ninc={ninc}", save=SAVE_ALL)
+        r = run_input(sem, ctx)
+        print("pit", r.rows, flush=True)
+        _compare_run_files(str(tmp_path / "e" / "output"), z)
+    finally:
+        ctx.close()

@@ -110,3 +110,33 @@ def test_overburden_stress_matches_oracle():
     z0, s0, k0 = (float(v) for v in z["stress0"])
     assert np.array_equal(overburden_stress(spec, gam, z0, s0, k0),
                           O.overburden_stress(spec, gam, z0, s0, k0))
+
+
+def test_reference_input_save_and_stress0_keys(tmp_path):
+    """The reader takes the save: keys and the stress0 line as the
+    reference's parser does (src/read_input.f90:236-250, 380-403)."""
+    m = slope_mesh(2, 1, 2)
+    sem = write_reference_input(m, str(tmp_path), "k", cg_tol=1e-8, cg_maxiter=50,
+                                nl_tol=1e-4, nl_maxiter=10, srf=[1.0],
+                                extra_lines="stress0: type=1, z0=8.0, s0=-5.0\n",
+                                save=("disp", "psigma", "vmeps"))
+    _, ctl = read_reference_input(sem)
+    assert ctl["save"] == ["disp", "psigma", "vmeps"]
+    assert ctl["s0_type"] == 1 and ctl["z0"] == 8.0 and ctl["s0"] == -5.0
+    assert ctl["k0"] == 0.0
+    assert ctl["out_path"].rstrip("/").endswith("output")
+
+
+def test_reference_input_numpy_scalars(tmp_path):
+    """Numpy scalars (as meshes rebuilt from .npz fixtures hold) are written
+    as plain numbers the reference's parser reads."""
+    m = slope_mesh(2, 1, 2, materials=[Material(np.float64(20.0), np.float64(1e5),
+                                                np.float64(0.3), np.float64(25.0),
+                                                np.float64(10.0))])
+    sem = write_reference_input(m, str(tmp_path), "n", cg_tol=np.float64(1e-8),
+                                cg_maxiter=np.int64(50), nl_tol=1e-4, nl_maxiter=10,
+                                srf=[np.float64(1.0)])
+    assert "np." not in open(sem).read()
+    m2, ctl = read_reference_input(sem)
+    assert m2.materials[0].ym == 1e5 and ctl["cg_tol"] == 1e-8
+    assert "np." not in open(str(tmp_path / "n_material_list")).read()
