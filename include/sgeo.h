@@ -303,9 +303,26 @@ int sgeo_batch_advance(sgeo_batch *batch, double cg_tol, int32_t cg_maxiter,
                        int32_t *memo_ready);
 int sgeo_batch_download(sgeo_batch *batch, int32_t col, double *x,
                         uint8_t *flags);
-/* sgeo_slope_nodal_fields for the slope trial held by column col. */
+/* sgeo_slope_nodal_fields for the trial held by column col (an excavation
+ * stage's scf and vmeps from sgeo_batch_accumulate_nodal). */
 int sgeo_batch_nodal_fields(sgeo_batch *batch, int32_t col, float *stress, float *psigma,
                             float *nsigma, float *scf, float *vmeps);
+/* Excavation load increment on column col, the mixed-precision form of
+ * sgeo_slope_begin_increment (src/excavation.F90:509-515): materials and
+ * dt, evpt, bodyload, x and oldx reset, the stress state set from
+ * stress0 (host, 6*27*nelmt, reference layout) or, with NULL, kept from
+ * the column's last increment; the body pass then also records each
+ * point's plastic strain and stress concentration factor at convergence
+ * (src/excavation.F90:599-609).  The increment's first PCG call solves the
+ * same system for every increment of a stage, so the engine's elastic
+ * memo serves increments after the first. */
+int sgeo_batch_begin_increment(sgeo_batch *batch, int32_t col, const double *ym,
+                               const double *nuf, const double *phif, const double *cohf,
+                               const double *psif, double dt, const double *stress0);
+/* Column col's stress_elmt (6*27*nelmt, reference layout) to the host. */
+int sgeo_batch_download_stress(sgeo_batch *batch, int32_t col, double *stress_elmt);
+/* sgeo_slope_accumulate_nodal for column col's excavation increment. */
+int sgeo_batch_accumulate_nodal(sgeo_batch *batch, int32_t col, int32_t first);
 /* Plastic iterations done so far by each column's current trial (ncol
  * values; 0 where no trial runs), as of the last body pass.  Lets a
  * scheduler see a trial heading for nl_maxiter (a failing SRF) before it

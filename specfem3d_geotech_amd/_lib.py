@@ -79,6 +79,10 @@ SIGNATURES = {
     "sgeo_slope_nodal_fields": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "sgeo_batch_nodal_fields": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "sgeo_slope_accumulate_nodal": (C.c_int, [vp, C.c_int32]),
+    "sgeo_batch_begin_increment": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                             C.c_double, C.POINTER(C.c_double)]),
+    "sgeo_batch_download_stress": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_double)]),
+    "sgeo_batch_accumulate_nodal": (C.c_int, [vp, C.c_int32, C.c_int32]),
     "sgeo_mesh_nodal_fields": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "sgeo_batch_nl_iters": (C.c_int, [vp, C.POINTER(C.c_int32)]),
     "sgeo_batch_set_geometry": (C.c_int, [vp, C.c_int32, C.c_int32]),

@@ -52,6 +52,28 @@ class BatchEngine:
                                        dt, ptr(x0a), int(x0_cg)),
               "sgeo_batch_begin")
 
+    def begin_increment(self, col, ym, nuf, phif, cohf, psif, dt, stress0=None):
+        """Excavation load increment on column `col`: state reset, the
+        stress state set from stress0 (nelmt, 27, 6) or kept."""
+        st = f64(stress0) if stress0 is not None else None
+        check(L.lib().sgeo_batch_begin_increment(
+            self.h, int(col), ptr(f64(ym)), ptr(f64(nuf)), ptr(f64(phif)),
+            ptr(f64(cohf)), ptr(f64(psif)), float(dt), ptr(st)),
+            "sgeo_batch_begin_increment")
+
+    def download_stress(self, col) -> np.ndarray:
+        st = np.zeros((self.dmesh.spec.nelmt, 27, 6))
+        check(L.lib().sgeo_batch_download_stress(self.h, int(col), ptr(st)),
+              "sgeo_batch_download_stress")
+        return st
+
+    def accumulate_nodal(self, col, first: bool):
+        """Excavation: this increment's plastic strain and stress
+        concentration factor gathered at the nodes of column `col`."""
+        self.dmesh.ensure_nodes()
+        check(L.lib().sgeo_batch_accumulate_nodal(self.h, int(col), int(first)),
+              "sgeo_batch_accumulate_nodal")
+
     def release(self, col):
         check(L.lib().sgeo_batch_release(self.h, col), "sgeo_batch_release")
 

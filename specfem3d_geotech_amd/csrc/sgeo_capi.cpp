@@ -1252,6 +1252,11 @@ struct sgeo_batch {
   int32_t* h_stage = nullptr;  // pinned staging of the small uploads
   unsigned stage_busy = 0;
   double step_us = 0.0;  // running mean of the wall time of one step
+  // excavation increments (sgeo_batch_begin_increment): per-point plastic
+  // strain and stress concentration factor [col][e][27], and per-column
+  // nodal accumulators [col][2][nnode]
+  double *d_vm = nullptr, *d_sf = nullptr, *d_nacc = nullptr;
+  std::vector<int32_t> nacc_valid;
 };
 
 static int ensure_packed(sgeo_mesh* m, int kprec) {
@@ -1368,7 +1373,8 @@ extern "C" int sgeo_batch_destroy(sgeo_batch* b) {
                   b->d_ext,   b->d_xel,    b->d_r,      b->d_p,     b->d_kp,
                   b->d_Y32,   b->d_dinv32, b->d_evpt,   b->d_stress, b->d_BL,
                   b->d_flags, b->d_mp,     b->d_ps,     b->d_bc,    b->d_cst,
-                  b->d_mask,  b->d_islast, b->d_cols,   b->d_part};
+                  b->d_mask,  b->d_islast, b->d_cols,   b->d_part,  b->d_vm,
+                  b->d_sf,    b->d_nacc};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (b->h_ps) hipHostFree(b->h_ps);
@@ -1516,6 +1522,87 @@ extern "C" int sgeo_batch_begin(sgeo_batch* b, int32_t col, const double* ym,
   b->seeded[col] = seeded;
   b->seed_cg[col] = seed_cg;
   b->cst[col] = BCOL_OFF;
+  if (!b->nacc_valid.empty()) b->nacc_valid[col] = 0;
+  return 0;
+}
+
+// An excavation load increment on column col (src/excavation.F90:509-515):
+// materials and dt, evpt, bodyload, x and oldx reset, the stress state set
+// from stress0 (host, 6*27*nelmt) or kept from the column's last increment;
+// the body then also records each point's plastic strain and stress
+// concentration factor at convergence (src/excavation.F90:599-609).  The
+// first PCG call solves the increment's load from x = 0, the same system
+// for every increment of the stage, so the engine's elastic memo serves it.
+extern "C" int sgeo_batch_begin_increment(sgeo_batch* b, int32_t col, const double* ym,
+                                          const double* nuf, const double* phif,
+                                          const double* cohf, const double* psif,
+                                          double dt, const double* stress0) {
+  if (!b || col < 0 || col >= b->B)
+    return fail(SGEO_EARG, "sgeo_batch_begin_increment: bad arguments");
+  sgeo_mesh* m = b->mesh;
+  HIPCHK(hipSetDevice(m->ctx->device));
+  hipStream_t s = m->ctx->stream;
+  const size_t ns = (size_t)m->nelmt * 27 * 6, np = (size_t)m->nelmt * 27;
+  if (!b->d_vm) {
+    TRY(dalloc(&b->d_vm, np * b->B, &b->bytes));
+    TRY(dalloc(&b->d_sf, np * b->B, &b->bytes));
+  }
+  // keep the stress across sgeo_batch_begin's reset
+  double* keep = nullptr;
+  if (!stress0) {
+    TRY(dalloc(&keep, ns));
+    HIPCHK(hipMemcpyAsync(keep, b->d_stress + col * ns, ns * 8, hipMemcpyDeviceToDevice,
+                          s));
+  }
+  int rc = sgeo_batch_begin(b, col, ym, nuf, phif, cohf, psif, dt, nullptr, 0);
+  hipError_t e = hipSuccess;
+  if (rc == 0)
+    e = stress0 ? hipMemcpyAsync(b->d_stress + col * ns, stress0, ns * 8,
+                                 hipMemcpyHostToDevice, s)
+                : hipMemcpyAsync(b->d_stress + col * ns, keep, ns * 8,
+                                 hipMemcpyDeviceToDevice, s);
+  if (rc == 0 && e == hipSuccess) e = hipMemsetAsync(b->d_vm + col * np, 0, np * 8, s);
+  if (rc == 0 && e == hipSuccess) e = hipMemsetAsync(b->d_sf + col * np, 0, np * 8, s);
+  if (rc == 0 && e == hipSuccess) e = hipStreamSynchronize(s);
+  if (keep) hipFree(keep);
+  if (rc != 0) return rc;
+  if (e != hipSuccess)
+    return fail(SGEO_EHIP, "sgeo_batch_begin_increment: %s", hipGetErrorString(e));
+  return 0;
+}
+
+// The column's stress_elmt (6*27*nelmt, reference layout) to the host.
+extern "C" int sgeo_batch_download_stress(sgeo_batch* b, int32_t col, double* stress) {
+  if (!b || col < 0 || col >= b->B || !stress) return fail(SGEO_EARG, "bad arguments");
+  sgeo_mesh* m = b->mesh;
+  HIPCHK(hipSetDevice(m->ctx->device));
+  const size_t ns = (size_t)m->nelmt * 27 * 6;
+  HIPCHK(hipMemcpyAsync(stress, b->d_stress + col * ns, ns * 8, hipMemcpyDeviceToHost,
+                        m->ctx->stream));
+  HIPCHK(hipStreamSynchronize(m->ctx->stream));
+  return 0;
+}
+
+// sgeo_slope_accumulate_nodal for column col of an excavation increment.
+extern "C" int sgeo_batch_accumulate_nodal(sgeo_batch* b, int32_t col, int32_t first) {
+  if (!b || col < 0 || col >= b->B) return fail(SGEO_EARG, "bad column");
+  if (!b->d_vm)
+    return fail(SGEO_ESTATE, "sgeo_batch_accumulate_nodal: no excavation increment");
+  sgeo_mesh* m = b->mesh;
+  if (!m->d_nrowptr)
+    return fail(SGEO_ESTATE, "no node map: call sgeo_mesh_set_nodes first");
+  HIPCHK(hipSetDevice(m->ctx->device));
+  const size_t nn = (size_t)m->nnode, np = (size_t)m->nelmt * 27;
+  if (!b->d_nacc) {
+    TRY(dalloc(&b->d_nacc, 2 * nn * b->B, &b->bytes));
+    b->nacc_valid.assign(b->B, 0);
+  }
+  launch_nodal_accumulate(m->ctx->stream, m->nnode, m->d_nrowptr, m->d_nslots,
+                          b->d_vm + col * np, b->d_sf + col * np, b->d_nacc + col * 2 * nn,
+                          first ? 1 : 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(m->ctx->stream));
+  b->nacc_valid[col] = 1;
   return 0;
 }
 
@@ -1703,6 +1790,9 @@ static int batch_body(sgeo_batch* b, const std::vector<int32_t>& cols, double nl
   ba.is_last_col = b->d_islast;
   ba.cols = b->d_cols;
   ba.ncols = (int)cols.size();
+  ba.excav = b->d_vm != nullptr;  // per-point vmeps and scf at convergence
+  ba.vm_pt = b->d_vm;
+  ba.sf_pt = b->d_sf;
   launch_plastic_body(s, ba);
   launch_bbody_assemble(s, b->n1, B, m->nelmt, m->d_rowptr, m->d_slots, b->d_BL,
                         b->d_body, b->d_mask, b->d_islast, b->d_ps);
@@ -1872,7 +1962,10 @@ extern "C" int sgeo_batch_nodal_fields(sgeo_batch* b, int32_t col, float* stress
                                        float* vmeps) {
   if (!b || col < 0 || col >= b->B) return fail(SGEO_EARG, "bad column");
   const size_t ns = (size_t)b->mesh->nelmt * 27 * 6;
-  return nodal_fields(b->mesh, b->d_stress + (size_t)col * ns, nullptr, stress, psigma,
+  const double* acc = (b->d_nacc && b->nacc_valid[col])
+                          ? b->d_nacc + (size_t)col * 2 * b->mesh->nnode
+                          : nullptr;
+  return nodal_fields(b->mesh, b->d_stress + (size_t)col * ns, acc, stress, psigma,
                       nsigma, scf, vmeps);
 }
 

@@ -49,6 +49,46 @@ def renumber(gdof_node: np.ndarray, alive: np.ndarray):
     return out.reshape(g.shape), int(free.sum())
 
 
+class _MixedStage:
+    """One excavation stage's trial on a one-column mixed-precision engine
+    (batch.BatchEngine), with the calls the stage loop makes on a
+    DeviceSlope.  DeviceSlope's download gives x and stress; here the
+    engine runs each increment's whole plastic loop (src/excavation.F90:
+    517-627) and reports its plastic iterations, CG iterations and fmax."""
+
+    def __init__(self, dmesh: DeviceMesh, extload: np.ndarray, inner_tol: float):
+        from .batch import BatchEngine
+        self.eng = BatchEngine(dmesh, extload, 1)
+        self.inner_tol = inner_tol
+
+    def begin_increment(self, ym, nuf, phif, cohf, psif, dt, stress0=None):
+        self.eng.begin_increment(0, ym, nuf, phif, cohf, psif, dt, stress0)
+
+    def run(self, cg_tol, cg_maxiter, nl_tol, nl_maxiter):
+        while True:
+            r = self.eng.advance(cg_tol, cg_maxiter, self.inner_tol, nl_tol, nl_maxiter)
+            if r["ended"][0]:
+                break
+        if r["errcode"][0] != 0:
+            raise PCGNotConverged("ERROR: PCG solver doesn't converge!")
+        return int(r["nl_tot"][0]), int(r["cg_tot"][0]), float(r["fmax"][0])
+
+    def x(self):
+        return self.eng.download(0)[0]
+
+    def stress(self):
+        return self.eng.download_stress(0)
+
+    def accumulate_nodal(self, first):
+        self.eng.accumulate_nodal(0, first)
+
+    def nodal_fields(self):
+        return self.eng.nodal_fields(0)
+
+    def close(self):
+        self.eng.close()
+
+
 def overburden_stress(spec: SpecMesh, gam: np.ndarray, z0: float, p0: float,
                       k0: float) -> np.ndarray:
     """Initial stress of s0_type 1 (src/postprocess.f90:10-33): at GLL point
@@ -89,19 +129,26 @@ def excavation3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
                  phinu: bool = False, s0_type: int = 0, kprec: int = K_F64,
                  kfmt: int = KFMT_FULL, spec: Optional[SpecMesh] = None,
                  progress=None, output=None, z0: float = 0.0, s0: float = 0.0,
-                 k0: float = 0.0) -> ExcavationResult:
+                 k0: float = 0.0, form: str = "fp64",
+                 inner_tol: float = 1e-5) -> ExcavationResult:
     """Staged excavation on the GPU for one SRF.  The reference runs
     excavation with nsrf = 1 only (it reallocates and closes per-SRF state
     inside its SRF loop); the first SRF is used.  `output`
     (output.ExcavationOutput) receives the in-situ step and every stage's
     row, displacement and nodal fields (computed on the device) for the
-    reference's result files.  s0_type 0 takes the in-situ stress from a
+    reference's result files.  form "fp64" runs each increment's plastic
+    loop on the FP64 drop-in (operator kprec/kfmt), "mixed" on a one-column
+    mixed-precision engine (FP32 inner CG, FP64 residuals to cg_tol, the
+    elastic memo serving every increment after a stage's first); the
+    in-situ gravity solve is FP64 in both.  s0_type 0 takes the in-situ stress from a
     gravity solve, s0_type 1 from the overburden relation with datum z0,
     stress s0 there and lateral ratio k0 (the input's stress0 line,
     src/read_input.f90:236-250; k0 defaults to 0 as the reference leaves it
     unset)."""
     if s0_type not in (0, 1):
         raise ValueError(f"s0_type {s0_type} not supported (src/excavation.F90:242)")
+    if form not in ("fp64", "mixed"):
+        raise ValueError(f"unknown form {form!r}")
     spec = spec or hex2spec_device(ctx, mesh)
     gam, ym, nu, phi, coh, psi = mesh.material_arrays()
     nmat, E = len(gam), spec.nelmt
@@ -158,31 +205,40 @@ def excavation3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
         trial = None
         try:
             di.build_stiffness(gam, ym, nuf, kprec=K_F64, want_extload=False)
-            di.set_operator(kprec, kfmt)
-            trial = DeviceSlope(di, extload / ninc)
+            if form == "mixed":
+                trial = _MixedStage(di, extload / ninc, inner_tol)
+            else:
+                di.set_operator(kprec, kfmt)
+                trial = DeviceSlope(di, extload / ninc)
             cg_tot = nl_tot = 0
             fmax = 0.0
             nl_iter = 0
             for inc in range(ninc):
                 trial.begin_increment(ym, nuf, phif, cohf, psif, dt,
                                       stress[intact] if inc == 0 else None)
-                for nl_iter in range(1, nl_maxiter + 1):
-                    r = trial.plastic_iter(cg_tol, cg_maxiter, nl_tol,
-                                           nl_iter == nl_maxiter)
-                    cg_log.append(r["cg_iter"])
-                    if r["errcode"] != 0:
-                        raise PCGNotConverged("ERROR: PCG solver doesn't converge!")
-                    cg_tot += r["cg_iter"]
-                    fmax = r["fmax"]
-                    if r["nl_isconv"] or nl_iter == nl_maxiter:
-                        break
+                if form == "mixed":  # the whole plastic loop on the engine
+                    nl_iter, cg_inc, fmax = trial.run(cg_tol, cg_maxiter, nl_tol,
+                                                      nl_maxiter)
+                    cg_tot += cg_inc
+                    x = trial.x()
+                else:
+                    for nl_iter in range(1, nl_maxiter + 1):
+                        r = trial.plastic_iter(cg_tol, cg_maxiter, nl_tol,
+                                               nl_iter == nl_maxiter)
+                        cg_log.append(r["cg_iter"])
+                        if r["errcode"] != 0:
+                            raise PCGNotConverged("ERROR: PCG solver doesn't converge!")
+                        cg_tot += r["cg_iter"]
+                        fmax = r["fmax"]
+                        if r["nl_isconv"] or nl_iter == nl_maxiter:
+                            break
+                    x = trial.x()
                 nl_tot += nl_iter
-                st = trial.download()
                 live = gdof_node != 0
-                nodalu[live] += st["x"][gdof_node[live]]
+                nodalu[live] += x[gdof_node[live]]
                 if output is not None and output.wants_fields:
                     trial.accumulate_nodal(first=inc == 0)
-            stress[intact] = st["stress"]
+            stress[intact] = trial.stress()
             un = nodalu[alive]
             rows.append(dict(stage=i_ex + 1, cg_tot=cg_tot, nl_tot=nl_tot,
                              uxmax=float(np.abs(un).max()),

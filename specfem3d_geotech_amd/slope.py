@@ -88,6 +88,20 @@ class DeviceSlope:
         return nodal_fields_call(L.lib().sgeo_slope_nodal_fields,
                                  "sgeo_slope_nodal_fields", self.dmesh.spec.nnode, self.h)
 
+    def x(self) -> np.ndarray:
+        """The trial's displacement x (neq+1)."""
+        x = np.zeros(self.dmesh.spec.neq + 1)
+        check(L.lib().sgeo_slope_download(self.h, ptr(x), None, None, None, None),
+              "sgeo_slope_download")
+        return x
+
+    def stress(self) -> np.ndarray:
+        """The trial's stress_elmt (nelmt, 27, 6)."""
+        st = np.zeros((self.dmesh.spec.nelmt, 27, 6))
+        check(L.lib().sgeo_slope_download(self.h, None, None, None, ptr(st), None),
+              "sgeo_slope_download")
+        return st
+
     def close(self):
         if self.h:
             L.lib().sgeo_slope_destroy(self.h)

@@ -213,3 +213,64 @@ def test_excavation_overburden_matches_reference(ctx_seq):
         assert abs(row["umax"] - rr[4]) <= 1e-10 * rr[4]
         assert abs(row["fmax"] - rr[5]) <= 1e-8 * max(1.0, abs(rr[5]))
     assert r.cg_log == list(z["cg_log"])
+
+
+@pytest.mark.parametrize("name,s0", [("pit_8x6x6", None), ("pit_overburden_6x4x4", 1)])
+def test_excavation_mixed_matches_reference(ctx, name, s0):
+    """The mixed-precision form of the staged excavation (one-column engine
+    per stage, FP32 inner CG with FP64 residuals to cg_tol) against the
+    compiled reference's stage rows: the same stages and plastic iteration
+    counts, CG counts within 2%, displacements within 1e-7 and fmax within
+    1e-5 (the solver tolerance carried through the yield checks)."""
+    z = golden(name)
+    mesh = mesh_from_golden(z)
+    kw = {}
+    if s0:
+        z0, p0, k0 = (float(v) for v in z["stress0"])
+        kw = dict(s0_type=1, z0=z0, s0=p0, k0=k0)
+    r = excavation3d(ctx, mesh, form="mixed", **kw, **_control(z))
+    ref = z["summary"]
+    assert len(r.rows) == ref.shape[0]
+    for row, rr in zip(r.rows, ref):
+        print(name, row, rr.tolist(), flush=True)
+        assert row["stage"] == rr[0] and row["nl_tot"] == rr[2]
+        assert abs(row["cg_tot"] - rr[1]) <= 0.02 * rr[1] + 2
+        assert abs(row["uxmax"] - rr[3]) <= 1e-7 * rr[3]
+        assert abs(row["umax"] - rr[4]) <= 1e-7 * rr[4]
+        assert abs(row["fmax"] - rr[5]) <= 1e-5 * max(1.0, abs(rr[5]))
+
+
+def test_excavation_mixed_output_files(ctx, tmp_path):
+    """Result files of the mixed-precision excavation (plastic strain and
+    stress concentration gathered per column on the engine) against the
+    reference's: geometry bytes equal, every field within 1e-6 relative L2."""
+    import os
+
+    from specfem3d_geotech_amd.mesh import HexMesh, Material
+    from specfem3d_geotech_amd.output import ExcavationOutput
+
+    z = golden("output_pit_6x4x4")
+    mats = [Material(gam=r[0], ym=r[1], nu=r[2], phi=r[3], coh=r[4], psi=r[5])
+            for r in z["materials"]]
+    mesh = HexMesh(coord=z["coord"], conn=z["conn"], mat_id=z["mat_id"], materials=mats,
+                   bc={"ux": z["bc_ux"], "uy": z["bc_uy"], "uz": z["bc_uz"]})
+    ctl = dict(cg_tol=1e-8, cg_maxiter=5000, nl_tol=1e-5, nl_maxiter=100)
+    nexcav, ninc, *excavid = [int(v) for v in z["excav"]]
+    spec = hex2spec(mesh)
+    out = ExcavationOutput(str(tmp_path), "pit_out", mesh, spec, ctl, [1.0], nexcav,
+                           save=("disp", "stress", "psigma", "nsigma", "scf", "vmeps"))
+    excavation3d(ctx, mesh, srf=[1.0], nexcav=nexcav, nexcavid=[1] * nexcav,
+                 excavid=excavid, ninc=ninc, spec=spec, output=out, form="mixed", **ctl)
+    out.close()
+    names = [str(n) for n in z["file_names"]]
+    assert sorted(os.listdir(tmp_path)) == names
+    for n in names:
+        ref, mine = z["file:" + n].tobytes(), (tmp_path / n).read_bytes()
+        if n.endswith(".geo") or n.endswith(".case"):
+            assert mine == ref, n
+        elif not n.endswith("_summary"):
+            a = np.frombuffer(mine[244:], np.float32).astype(np.float64)
+            b = np.frombuffer(ref[244:], np.float32).astype(np.float64)
+            assert mine[:244] == ref[:244] and a.shape == b.shape, n
+            nb = np.linalg.norm(b)
+            assert np.linalg.norm(a - b) <= 1e-6 * nb + 1e-12, (n, np.linalg.norm(a - b), nb)
