@@ -19,6 +19,7 @@
 import os
 
 import numpy as np
+import pytest
 
 import oracle as O
 from conftest import golden
@@ -187,3 +188,37 @@ def test_excavation_files_match_reference(tmp_path):
                         assert x == y, (a, b)
         else:
             assert mine == ref, n
+
+
+def test_partial_save_sets(tmp_path):
+    """Only the saved fields are written and listed in the case file: scf
+    alone (a slope trial's is 1e32 everywhere, src/slope.F90:249), and an
+    excavation with no field files at all."""
+    z = golden("output_slope_4x2x3")
+    mats = [Material(gam=r[0], ym=r[1], nu=r[2], phi=r[3], coh=r[4], psi=r[5])
+            for r in z["materials"]]
+    mesh = HexMesh(coord=z["coord"], conn=z["conn"], mat_id=z["mat_id"], materials=mats,
+                   bc={"ux": z["bc_ux"], "uy": z["bc_uy"], "uz": z["bc_uz"]})
+    spec = hex2spec(mesh)
+    ctl = dict(cg_tol=1e-8, cg_maxiter=5000, nl_tol=5e-4, nl_maxiter=300)
+    out = SlopeOutput(str(tmp_path / "s"), "h", mesh, spec, ctl, [1.0], save=("scf",))
+    nn = spec.nnode
+    out.srf_done(1, dict(srf=1.0, cg_tot=1, nl_tot=1, uxmax=0.0, umax=0.0, fmax=0.0),
+                 None, fields=dict(stress=np.zeros((nn, 6)), psigma=np.zeros((nn, 3)),
+                                   nsigma=np.zeros(nn)))
+    out.close()
+    names = sorted(os.listdir(tmp_path / "s"))
+    assert names == ["h.case", "h.geo", "h_original.case", "h_original.geo",
+                     "h_step1.scf", "h_summary"]
+    case = (tmp_path / "s" / "h.case").read_text()
+    assert "stress_concentration_factor h_step*.scf" in case and "displacement" not in case
+    v = np.frombuffer((tmp_path / "s" / "h_step1.scf").read_bytes()[244:], np.float32)
+    assert v.size == nn and np.all(v == np.float32(1e32))
+    out = ExcavationOutput(str(tmp_path / "e"), "p", mesh, spec, ctl, [1.0], 1, save=())
+    out.in_situ()
+    out.close()
+    assert sorted(os.listdir(tmp_path / "e")) == ["p.case", "p_original.case",
+                                                  "p_original.geo", "p_step0.geo",
+                                                  "p_summary"]
+    with pytest.raises(ValueError):
+        SlopeOutput(str(tmp_path / "x"), "x", mesh, spec, ctl, [1.0], save=("maxtau",))
