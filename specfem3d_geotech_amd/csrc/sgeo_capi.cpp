@@ -155,6 +155,22 @@ static int dalloc(T** p, size_t n, size_t* acc = nullptr) {
     if (_r != 0) return _r;  \
   } while (0)
 
+// Destroys a handle under construction when a create call returns early
+// with an error (every destroy function frees only what was allocated);
+// release() hands it to the caller on success.
+template <typename T, int (*Destroy)(T*)>
+struct Partial {
+  T* p;
+  ~Partial() {
+    if (p) Destroy(p);
+  }
+  T* release() {
+    T* q = p;
+    p = nullptr;
+    return q;
+  }
+};
+
 static int ensure_part(sgeo_ctx* c, size_t n) {
   if (n <= c->part_cap) return 0;
   if (c->d_part) hipFree(c->d_part);
@@ -176,6 +192,7 @@ extern "C" int sgeo_init(int device, sgeo_ctx** out) {
     return fail(SGEO_EARG, "sgeo_init: device %d of %d", device, ndev);
   HIPCHK(hipSetDevice(device));
   sgeo_ctx* c = new sgeo_ctx();
+  Partial<sgeo_ctx, sgeo_finalize> guard{c};
   c->device = device;
   HIPCHK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
   if (c->n_cu < 1) c->n_cu = 1;
@@ -188,7 +205,7 @@ extern "C" int sgeo_init(int device, sgeo_ctx** out) {
   HIPCHK(hipMemsetAsync(c->d_ps, 0, sizeof(PlasticScalars), c->stream));
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
-  *out = c;
+  *out = guard.release();
   return 0;
 }
 
@@ -282,6 +299,7 @@ extern "C" int sgeo_mesh_create(sgeo_ctx* c, int32_t nelmt, int32_t neq,
   }
   HIPCHK(hipSetDevice(c->device));
   sgeo_mesh* m = new sgeo_mesh();
+  Partial<sgeo_mesh, sgeo_mesh_destroy> guard{m};
   m->ctx = c;
   m->nelmt = nelmt;
   m->neq = neq;
@@ -313,7 +331,7 @@ extern "C" int sgeo_mesh_create(sgeo_ctx* c, int32_t nelmt, int32_t neq,
   }
   TRY(ensure_part(c, 4 * (size_t)vec_reduce_blocks(n1) + 64));
   HIPCHK(hipStreamSynchronize(s));  // host vectors die at return
-  *out = m;
+  *out = guard.release();
   return 0;
 }
 
@@ -856,6 +874,7 @@ extern "C" int sgeo_slope_create(sgeo_mesh* m, const double* extload,
   sgeo_ctx* c = m->ctx;
   HIPCHK(hipSetDevice(c->device));
   sgeo_slope* t = new sgeo_slope();
+  Partial<sgeo_slope, sgeo_slope_destroy> guard{t};
   t->mesh = m;
   const size_t n1 = (size_t)m->neq + 1;
   const size_t ns = (size_t)m->nelmt * 27 * 6;
@@ -871,7 +890,7 @@ extern "C" int sgeo_slope_create(sgeo_mesh* m, const double* extload,
   TRY(dalloc(&t->d_mp, (size_t)std::max(1, m->nmat)));
   HIPCHK(hipMemcpyAsync(t->d_ext, extload, n1 * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  *out = t;
+  *out = guard.release();
   return 0;
 }
 
@@ -1294,6 +1313,7 @@ extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extlo
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   sgeo_batch* b = new sgeo_batch();
+  Partial<sgeo_batch, sgeo_batch_destroy> guard{b};
   b->mesh = m;
   b->B = ncol;
   b->n1 = (int64_t)m->neq + 1;
@@ -1352,7 +1372,7 @@ extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extlo
   HIPCHK(hipMemsetAsync(b->d_x, 0, nv * 8, s));
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s));
-  *out = b;
+  *out = guard.release();
   return 0;
 }
 
