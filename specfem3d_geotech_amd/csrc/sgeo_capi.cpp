@@ -155,6 +155,22 @@ static int dalloc(T** p, size_t n, size_t* acc = nullptr) {
     if (_r != 0) return _r;  \
   } while (0)
 
+// Device temporaries of one call, freed when the call returns, on every
+// path (hipFree waits for the device).
+struct DevScratch {
+  std::vector<void*> ptrs;
+  ~DevScratch() {
+    for (void* p : ptrs)
+      if (p) hipFree(p);
+  }
+  template <typename T>
+  int alloc(T** p, size_t n) {
+    const int r = dalloc(p, n);
+    if (r == 0) ptrs.push_back((void*)*p);
+    return r;
+  }
+};
+
 // Destroys a handle under construction when a create call returns early
 // with an error (every destroy function frees only what was allocated);
 // release() hands it to the caller on success.
@@ -519,12 +535,13 @@ extern "C" int sgeo_mesh_build_stiffness(sgeo_mesh* m, const double* gam,
   const int64_t n1 = (int64_t)m->neq + 1;
   double *d_ym = nullptr, *d_nu = nullptr, *d_gam = nullptr, *d_kdiag = nullptr,
          *d_eld = nullptr, *d_ext = nullptr;
-  TRY(dalloc(&d_ym, m->nmat));
-  TRY(dalloc(&d_nu, m->nmat));
-  TRY(dalloc(&d_gam, m->nmat));
-  TRY(dalloc(&d_kdiag, (size_t)m->nelmt * 81));
-  TRY(dalloc(&d_eld, (size_t)m->nelmt * 81));
-  TRY(dalloc(&d_ext, (size_t)n1));
+  DevScratch tmp;
+  TRY(tmp.alloc(&d_ym, m->nmat));
+  TRY(tmp.alloc(&d_nu, m->nmat));
+  TRY(tmp.alloc(&d_gam, m->nmat));
+  TRY(tmp.alloc(&d_kdiag, (size_t)m->nelmt * 81));
+  TRY(tmp.alloc(&d_eld, (size_t)m->nelmt * 81));
+  TRY(tmp.alloc(&d_ext, (size_t)n1));
   HIPCHK(hipMemcpyAsync(d_ym, ym, m->nmat * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d_nu, nu, m->nmat * 8, hipMemcpyHostToDevice, s));
   if (gam) HIPCHK(hipMemcpyAsync(d_gam, gam, m->nmat * 8, hipMemcpyHostToDevice, s));
@@ -546,8 +563,6 @@ extern "C" int sgeo_mesh_build_stiffness(sgeo_mesh* m, const double* gam,
     HIPCHK(hipMemcpyAsync(dprecon_out, m->d_dinv, n1 * 8, hipMemcpyDeviceToHost, s));
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s));
-  hipFree(d_ym); hipFree(d_nu); hipFree(d_gam); hipFree(d_kdiag); hipFree(d_eld);
-  hipFree(d_ext);
   return 0;
 }
 
@@ -632,18 +647,16 @@ extern "C" int sgeo_hex2spec(sgeo_ctx* c, int32_t nelmt, int32_t nnode8,
   hipStream_t s = c->stream;
   double* d_coord = nullptr;
   int32_t *d_conn = nullptr, *d_bc = nullptr;
-  TRY(dalloc(&d_coord, (size_t)nnode8 * 3));
-  TRY(dalloc(&d_conn, (size_t)nelmt * 8));
-  TRY(dalloc(&d_bc, (size_t)std::max(nbc, 1) * 3));
+  DevScratch tmp;
+  TRY(tmp.alloc(&d_coord, (size_t)nnode8 * 3));
+  TRY(tmp.alloc(&d_conn, (size_t)nelmt * 8));
+  TRY(tmp.alloc(&d_bc, (size_t)std::max(nbc, 1) * 3));
   HIPCHK(hipMemcpyAsync(d_coord, coord8, (size_t)nnode8 * 24, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d_conn, conn, (size_t)nelmt * 32, hipMemcpyHostToDevice, s));
   if (nbc > 0)
     HIPCHK(hipMemcpyAsync(d_bc, bc, (size_t)nbc * 12, hipMemcpyHostToDevice, s));
   SpecDev d;
   const hipError_t e = hex2spec_device(s, nelmt, d_coord, d_conn, nbc, d_bc, tol, &d);
-  hipFree(d_coord);
-  hipFree(d_conn);
-  hipFree(d_bc);
   if (e != hipSuccess)
     return fail(SGEO_EHIP, "sgeo_hex2spec: %s", hipGetErrorString(e));
   sgeo_spec* sp = new sgeo_spec();
@@ -690,8 +703,9 @@ extern "C" int sgeo_calibrate_copy(sgeo_ctx* c, size_t bytes, int reps,
   HIPCHK(hipSetDevice(c->device));
   const size_t n2 = bytes / 16;
   double *a = nullptr, *b = nullptr;
-  TRY(dalloc(&a, n2 * 2));
-  TRY(dalloc(&b, n2 * 2));
+  DevScratch tmp;
+  TRY(tmp.alloc(&a, n2 * 2));
+  TRY(tmp.alloc(&b, n2 * 2));
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -706,8 +720,6 @@ extern "C" int sgeo_calibrate_copy(sgeo_ctx* c, size_t bytes, int reps,
   *gbs = 2.0 * (double)n2 * 16.0 * reps / (ms * 1e-3) / 1e9;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  hipFree(a);
-  hipFree(b);
   return 0;
 }
 
@@ -1966,9 +1978,12 @@ extern "C" int sgeo_batch_download(sgeo_batch* b, int32_t col, double* x,
     double* tmp = nullptr;
     TRY(dalloc(&tmp, (size_t)b->n1));
     launch_col_io(s, b->n1, b->B, col, b->d_x, nullptr, tmp);
-    HIPCHK(hipMemcpyAsync(x, tmp, b->n1 * 8, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(x, tmp, b->n1 * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     hipFree(tmp);
+    if (e != hipSuccess)
+      return fail(SGEO_EHIP, "sgeo_batch_download: %s", hipGetErrorString(e));
   }
   if (flags)
     HIPCHK(hipMemcpyAsync(flags, b->d_flags + (size_t)col * m->nelmt * 27,
