@@ -1008,8 +1008,9 @@ extern "C" int sgeo_mesh_solve(sgeo_mesh* m, double* u, const double* f,
   hipStream_t s = c->stream;
   const size_t n1 = (size_t)m->neq + 1;
   double *d_u = nullptr, *d_f = nullptr;
-  TRY(dalloc(&d_u, n1));
-  TRY(dalloc(&d_f, n1));
+  DevScratch tmp;
+  TRY(tmp.alloc(&d_u, n1));
+  TRY(tmp.alloc(&d_f, n1));
   HIPCHK(hipMemcpyAsync(d_u, u, n1 * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d_f, f, n1 * 8, hipMemcpyHostToDevice, s));
   const int rc = pcg_device(m, d_u, d_f, cg_tol, cg_maxiter, cg_iter, errcode);
@@ -1017,8 +1018,6 @@ extern "C" int sgeo_mesh_solve(sgeo_mesh* m, double* u, const double* f,
     HIPCHK(hipMemcpyAsync(u, d_u, n1 * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
   }
-  hipFree(d_u);
-  hipFree(d_f);
   return rc;
 }
 
@@ -1036,9 +1035,10 @@ extern "C" int sgeo_mesh_elastic_stress(sgeo_mesh* m, const double* ym,
   const size_t n1 = (size_t)m->neq + 1, ns = (size_t)m->nelmt * 27 * 6;
   const int nmat = std::max(1, m->nmat);
   double *d_x = nullptr, *d_st = nullptr, *d_mat = nullptr;
-  TRY(dalloc(&d_x, n1));
-  TRY(dalloc(&d_st, ns));
-  TRY(dalloc(&d_mat, 2 * (size_t)nmat));
+  DevScratch tmp;
+  TRY(tmp.alloc(&d_x, n1));
+  TRY(tmp.alloc(&d_st, ns));
+  TRY(tmp.alloc(&d_mat, 2 * (size_t)nmat));
   HIPCHK(hipMemcpyAsync(d_x, x, n1 * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d_mat, ym, nmat * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d_mat + nmat, nu, nmat * 8, hipMemcpyHostToDevice, s));
@@ -1047,9 +1047,6 @@ extern "C" int sgeo_mesh_elastic_stress(sgeo_mesh* m, const double* ym,
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(stress, d_st, ns * 8, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  hipFree(d_x);
-  hipFree(d_st);
-  hipFree(d_mat);
   return 0;
 }
 
@@ -1069,9 +1066,10 @@ extern "C" int sgeo_mesh_excavation_load(sgeo_mesh* m, const double* gam,
   const size_t ns = (size_t)m->nelmt * 27 * 6;
   const int nmat = std::max(1, m->nmat);
   double *d_st = nullptr, *d_gam = nullptr, *d_out = nullptr;
-  TRY(dalloc(&d_st, ns));
-  TRY(dalloc(&d_gam, (size_t)nmat));
-  TRY(dalloc(&d_out, (size_t)n1));
+  DevScratch tmp;
+  TRY(tmp.alloc(&d_st, ns));
+  TRY(tmp.alloc(&d_gam, (size_t)nmat));
+  TRY(tmp.alloc(&d_out, (size_t)n1));
   HIPCHK(hipMemcpyAsync(d_st, stress, ns * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d_gam, gam, nmat * 8, hipMemcpyHostToDevice, s));
   launch_excav_load(s, m->nelmt, m->d_corners, m->d_mat, d_gam, d_st, m->d_Y);
@@ -1079,9 +1077,6 @@ extern "C" int sgeo_mesh_excavation_load(sgeo_mesh* m, const double* gam,
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(load, d_out, n1 * 8, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  hipFree(d_st);
-  hipFree(d_gam);
-  hipFree(d_out);
   return 0;
 }
 
@@ -1527,9 +1522,10 @@ extern "C" int sgeo_batch_begin(sgeo_batch* b, int32_t col, const double* ym,
   launch_col_io(s, b->n1, b->B, col, b->d_body, nullptr, nullptr);
   launch_col_io(s, b->n1, b->B, col, b->d_oldx, nullptr, nullptr);
   double* d_x0 = nullptr;
+  DevScratch tmp;
   int seeded = 0, seed_cg = 0;
   if (x0) {
-    TRY(dalloc(&d_x0, (size_t)b->n1));
+    TRY(tmp.alloc(&d_x0, (size_t)b->n1));
     HIPCHK(hipMemcpyAsync(d_x0, x0, b->n1 * 8, hipMemcpyHostToDevice, s));
     launch_col_io(s, b->n1, b->B, col, b->d_x, d_x0, nullptr);
     seeded = 1;
@@ -1543,7 +1539,6 @@ extern "C" int sgeo_batch_begin(sgeo_batch* b, int32_t col, const double* ym,
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s));
-  if (d_x0) hipFree(d_x0);
   b->phase[col] = BPH_RUN;
   b->need_start[col] = 1;
   b->nl_iter[col] = 0;
@@ -1581,8 +1576,9 @@ extern "C" int sgeo_batch_begin_increment(sgeo_batch* b, int32_t col, const doub
   }
   // keep the stress across sgeo_batch_begin's reset
   double* keep = nullptr;
+  DevScratch tmp;
   if (!stress0) {
-    TRY(dalloc(&keep, ns));
+    TRY(tmp.alloc(&keep, ns));
     HIPCHK(hipMemcpyAsync(keep, b->d_stress + col * ns, ns * 8, hipMemcpyDeviceToDevice,
                           s));
   }
@@ -1596,7 +1592,6 @@ extern "C" int sgeo_batch_begin_increment(sgeo_batch* b, int32_t col, const doub
   if (rc == 0 && e == hipSuccess) e = hipMemsetAsync(b->d_vm + col * np, 0, np * 8, s);
   if (rc == 0 && e == hipSuccess) e = hipMemsetAsync(b->d_sf + col * np, 0, np * 8, s);
   if (rc == 0 && e == hipSuccess) e = hipStreamSynchronize(s);
-  if (keep) hipFree(keep);
   if (rc != 0) return rc;
   if (e != hipSuccess)
     return fail(SGEO_EHIP, "sgeo_batch_begin_increment: %s", hipGetErrorString(e));
