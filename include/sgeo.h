@@ -246,7 +246,9 @@ int sgeo_slope_nodal_fields(sgeo_slope *slope, float *stress, float *psigma,
  * sqrt(2/3 evpt.evpt) and lowers scf to the points' stress concentration
  * factor, in element order; first != 0 starts the stage from 0 and 1e32
  * (src/excavation.F90:335).  Needs an increment begun with
- * sgeo_slope_begin_increment and sgeo_mesh_set_nodes. */
+ * sgeo_slope_begin_increment and sgeo_mesh_set_nodes.  The first call, and
+ * the first after sgeo_mesh_set_nodes changed the node count, must pass
+ * first != 0 (SGEO_ESTATE otherwise). */
 int sgeo_slope_accumulate_nodal(sgeo_slope *slope, int32_t first);
 
 /* ---- multi-trial mixed-precision engine ---------------------------------
@@ -321,7 +323,8 @@ int sgeo_batch_begin_increment(sgeo_batch *batch, int32_t col, const double *ym,
                                const double *psif, double dt, const double *stress0);
 /* Column col's stress_elmt (6*27*nelmt, reference layout) to the host. */
 int sgeo_batch_download_stress(sgeo_batch *batch, int32_t col, double *stress_elmt);
-/* sgeo_slope_accumulate_nodal for column col's excavation increment. */
+/* sgeo_slope_accumulate_nodal for column col's excavation increment (the
+ * same first != 0 rule, per column). */
 int sgeo_batch_accumulate_nodal(sgeo_batch *batch, int32_t col, int32_t first);
 /* Plastic iterations done so far by each column's current trial (ncol
  * values; 0 where no trial runs), as of the last body pass.  Lets a

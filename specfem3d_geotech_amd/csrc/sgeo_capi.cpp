@@ -131,6 +131,8 @@ struct sgeo_slope {
   double* d_vm = nullptr;  // excavation: per-point von Mises plastic strain
   double* d_sf = nullptr;  // excavation: per-point stress concentration factor
   double* d_nacc = nullptr;  // excavation: nodal sums of vm, minima of sf (2 x nnode)
+  int32_t nacc_nnode = 0;    // node count d_nacc was sized for
+  int nacc_init = 0;         // d_nacc holds a started accumulation
   uint8_t* d_flags = nullptr;
   MatParams* d_mp = nullptr;
   std::vector<MatParams> h_mp;
@@ -409,7 +411,7 @@ extern "C" int sgeo_mesh_set_nodes(sgeo_mesh* m, int32_t nnode, const int32_t* g
 // 366-376).
 static int nodal_fields(sgeo_mesh* m, const double* d_stress, const double* d_acc,
                         float* sig, float* psig, float* nsig, float* scf, float* vmeps) {
-  if (!m->d_nrowptr)
+  if (!m->d_nrowptr || !m->d_nslots || m->nnode <= 0)
     return fail(SGEO_ESTATE, "no node map: call sgeo_mesh_set_nodes first");
   const size_t nn = (size_t)m->nnode;
   HIPCHK(hipSetDevice(m->ctx->device));
@@ -1209,21 +1211,35 @@ extern "C" int sgeo_slope_accumulate_nodal(sgeo_slope* t, int32_t first) {
   if (!t || !t->excav || !t->d_vm || !t->d_sf)
     return fail(SGEO_ESTATE, "sgeo_slope_accumulate_nodal: no excavation increment");
   sgeo_mesh* m = t->mesh;
-  if (!m->d_nrowptr)
+  if (!m->d_nrowptr || !m->d_nslots || m->nnode <= 0)
     return fail(SGEO_ESTATE, "no node map: call sgeo_mesh_set_nodes first");
   HIPCHK(hipSetDevice(m->ctx->device));
-  if (!t->d_nacc) TRY(dalloc(&t->d_nacc, 2 * (size_t)m->nnode));
+  if (t->d_nacc && t->nacc_nnode != m->nnode) {  // node map replaced since
+    HIPCHK(hipStreamSynchronize(m->ctx->stream));
+    hipFree(t->d_nacc);
+    t->d_nacc = nullptr;
+  }
+  if (!t->d_nacc) {
+    t->nacc_init = 0;
+    TRY(dalloc(&t->d_nacc, 2 * (size_t)m->nnode));
+    t->nacc_nnode = m->nnode;
+  }
+  if (!first && !t->nacc_init)
+    return fail(SGEO_ESTATE,
+                "sgeo_slope_accumulate_nodal: first=0 before an accumulation with first=1");
   launch_nodal_accumulate(m->ctx->stream, m->nnode, m->d_nrowptr, m->d_nslots, t->d_vm,
                           t->d_sf, t->d_nacc, first ? 1 : 0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(m->ctx->stream));
+  t->nacc_init = 1;
   return 0;
 }
 
 extern "C" int sgeo_slope_nodal_fields(sgeo_slope* t, float* stress, float* psigma,
                                        float* nsigma, float* scf, float* vmeps) {
   if (!t) return fail(SGEO_EARG, "null slope");
-  return nodal_fields(t->mesh, t->d_stress, t->excav ? t->d_nacc : nullptr, stress, psigma,
+  const bool acc = t->excav && t->nacc_init && t->nacc_nnode == t->mesh->nnode;
+  return nodal_fields(t->mesh, t->d_stress, acc ? t->d_nacc : nullptr, stress, psigma,
                       nsigma, scf, vmeps);
 }
 
@@ -1282,7 +1298,9 @@ struct sgeo_batch {
   // strain and stress concentration factor [col][e][27], and per-column
   // nodal accumulators [col][2][nnode]
   double *d_vm = nullptr, *d_sf = nullptr, *d_nacc = nullptr;
-  std::vector<int32_t> nacc_valid;
+  int32_t nacc_nnode = 0;            // node count d_nacc was sized for
+  std::vector<int32_t> nacc_valid;   // column's accumulators describe its state
+  std::vector<int32_t> nacc_init;    // column's accumulators hold a started sum
 };
 
 static int ensure_packed(sgeo_mesh* m, int kprec) {
@@ -1570,9 +1588,12 @@ extern "C" int sgeo_batch_begin_increment(sgeo_batch* b, int32_t col, const doub
   HIPCHK(hipSetDevice(m->ctx->device));
   hipStream_t s = m->ctx->stream;
   const size_t ns = (size_t)m->nelmt * 27 * 6, np = (size_t)m->nelmt * 27;
-  if (!b->d_vm) {
-    TRY(dalloc(&b->d_vm, np * b->B, &b->bytes));
-    TRY(dalloc(&b->d_sf, np * b->B, &b->bytes));
+  if (!b->d_vm || !b->d_sf) {  // both or neither: the body keys on the pair
+    double *vm = b->d_vm, *sf = b->d_sf;
+    if (!vm) TRY(dalloc(&vm, np * b->B, &b->bytes));
+    b->d_vm = vm;
+    if (!sf) TRY(dalloc(&sf, np * b->B, &b->bytes));
+    b->d_sf = sf;
   }
   // keep the stress across sgeo_batch_begin's reset
   double* keep = nullptr;
@@ -1613,23 +1634,35 @@ extern "C" int sgeo_batch_download_stress(sgeo_batch* b, int32_t col, double* st
 // sgeo_slope_accumulate_nodal for column col of an excavation increment.
 extern "C" int sgeo_batch_accumulate_nodal(sgeo_batch* b, int32_t col, int32_t first) {
   if (!b || col < 0 || col >= b->B) return fail(SGEO_EARG, "bad column");
-  if (!b->d_vm)
+  if (!b->d_vm || !b->d_sf)
     return fail(SGEO_ESTATE, "sgeo_batch_accumulate_nodal: no excavation increment");
   sgeo_mesh* m = b->mesh;
-  if (!m->d_nrowptr)
+  if (!m->d_nrowptr || !m->d_nslots || m->nnode <= 0)
     return fail(SGEO_ESTATE, "no node map: call sgeo_mesh_set_nodes first");
   HIPCHK(hipSetDevice(m->ctx->device));
   const size_t nn = (size_t)m->nnode, np = (size_t)m->nelmt * 27;
-  if (!b->d_nacc) {
-    TRY(dalloc(&b->d_nacc, 2 * nn * b->B, &b->bytes));
-    b->nacc_valid.assign(b->B, 0);
+  if (b->d_nacc && b->nacc_nnode != m->nnode) {  // node map replaced since
+    HIPCHK(hipStreamSynchronize(m->ctx->stream));
+    hipFree(b->d_nacc);
+    b->bytes -= 2 * (size_t)b->nacc_nnode * b->B * sizeof(double);
+    b->d_nacc = nullptr;
   }
+  if (!b->d_nacc) {
+    b->nacc_valid.assign(b->B, 0);
+    b->nacc_init.assign(b->B, 0);
+    TRY(dalloc(&b->d_nacc, 2 * nn * b->B, &b->bytes));
+    b->nacc_nnode = m->nnode;
+  }
+  if (!first && !b->nacc_init[col])
+    return fail(SGEO_ESTATE,
+                "sgeo_batch_accumulate_nodal: first=0 before an accumulation with first=1");
   launch_nodal_accumulate(m->ctx->stream, m->nnode, m->d_nrowptr, m->d_nslots,
                           b->d_vm + col * np, b->d_sf + col * np, b->d_nacc + col * 2 * nn,
                           first ? 1 : 0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(m->ctx->stream));
   b->nacc_valid[col] = 1;
+  b->nacc_init[col] = 1;
   return 0;
 }
 
@@ -1817,7 +1850,7 @@ static int batch_body(sgeo_batch* b, const std::vector<int32_t>& cols, double nl
   ba.is_last_col = b->d_islast;
   ba.cols = b->d_cols;
   ba.ncols = (int)cols.size();
-  ba.excav = b->d_vm != nullptr;  // per-point vmeps and scf at convergence
+  ba.excav = b->d_vm && b->d_sf;  // per-point vmeps and scf at convergence
   ba.vm_pt = b->d_vm;
   ba.sf_pt = b->d_sf;
   launch_plastic_body(s, ba);
@@ -1992,7 +2025,7 @@ extern "C" int sgeo_batch_nodal_fields(sgeo_batch* b, int32_t col, float* stress
                                        float* vmeps) {
   if (!b || col < 0 || col >= b->B) return fail(SGEO_EARG, "bad column");
   const size_t ns = (size_t)b->mesh->nelmt * 27 * 6;
-  const double* acc = (b->d_nacc && b->nacc_valid[col])
+  const double* acc = (b->d_nacc && b->nacc_nnode == b->mesh->nnode && b->nacc_valid[col])
                           ? b->d_nacc + (size_t)col * 2 * b->mesh->nnode
                           : nullptr;
   return nodal_fields(b->mesh, b->d_stress + (size_t)col * ns, acc, stress, psigma,
