@@ -87,12 +87,18 @@ int sgeo_profile(sgeo_ctx *ctx, int enable, int64_t *launches,
  * "ERROR: PCG solver doesn't converge!" on -1) follow the reference.
  * Device copies of k, dprecon and gdof_elmt are cached on the context and
  * reused while the host pointers, sizes and a strided content fingerprint
- * are unchanged. */
+ * are unchanged.  The fingerprint reads one value in 1024 of k: a caller
+ * that rewrites k, dprecon or gdof_elmt in place (same pointers) calls
+ * sgeo_pcg_solver_invalidate before the next solve. */
 int sgeo_pcg_solver(sgeo_ctx *ctx, int32_t neq, int32_t nelmt,
                     const double *k, double *u, const double *f,
                     const double *dprecon, const int32_t *gdof_elmt,
                     double cg_tol, int32_t cg_maxiter, int32_t *cg_iter,
                     int32_t *errcode, char *errtag);
+/* Drops the cached device copies sgeo_pcg_solver keeps (no reference
+ * counterpart: the reference reads its arrays at every call), so the next
+ * call uploads k, dprecon and gdof_elmt again. */
+int sgeo_pcg_solver_invalidate(sgeo_ctx *ctx);
 
 /* ---- ingest: GLL numbering on the device ---------------------------------
  * Replaces hex2spec + get_global + get_global_indirect_addressing

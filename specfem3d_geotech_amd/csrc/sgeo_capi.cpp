@@ -828,6 +828,19 @@ static void set_errtag(char* errtag, const char* msg) {
   memset(errtag + n, ' ', 250 - n);
 }
 
+// Drops the drop-in's cached device copies of k, dprecon and gdof_elmt.
+extern "C" int sgeo_pcg_solver_invalidate(sgeo_ctx* c) {
+  if (!c) return fail(SGEO_EARG, "null context");
+  if (c->cache) {
+    hipSetDevice(c->device);
+    sgeo_mesh_destroy(c->cache);
+  }
+  c->cache = nullptr;
+  c->cache_k = c->cache_g = c->cache_d = nullptr;
+  c->cache_sig = 0;
+  return 0;
+}
+
 extern "C" int sgeo_pcg_solver(sgeo_ctx* c, int32_t neq, int32_t nelmt,
                                const double* k, double* u, const double* f,
                                const double* dprecon, const int32_t* gdof,
@@ -849,10 +862,12 @@ extern "C" int sgeo_pcg_solver(sgeo_ctx* c, int32_t neq, int32_t nelmt,
   if (!c->cache || c->cache_k != k || c->cache_g != gdof ||
       c->cache_d != dprecon || c->cache_sig != sig ||
       c->cache->nelmt != nelmt || c->cache->neq != neq) {
-    if (c->cache) sgeo_mesh_destroy(c->cache);
-    c->cache = nullptr;
-    TRY(sgeo_mesh_create(c, nelmt, neq, nullptr, gdof, nullptr, 0, &c->cache));
-    TRY(sgeo_mesh_upload_stiffness(c->cache, k, dprecon, SGEO_K_F64));
+    TRY(sgeo_pcg_solver_invalidate(c));
+    sgeo_mesh* nm = nullptr;
+    TRY(sgeo_mesh_create(c, nelmt, neq, nullptr, gdof, nullptr, 0, &nm));
+    Partial<sgeo_mesh, sgeo_mesh_destroy> guard{nm};
+    TRY(sgeo_mesh_upload_stiffness(nm, k, dprecon, SGEO_K_F64));
+    c->cache = guard.release();  // keyed only once the upload succeeded
     c->cache_k = k;
     c->cache_g = gdof;
     c->cache_d = dprecon;
@@ -1145,9 +1160,14 @@ extern "C" int sgeo_slope_run_srf(sgeo_slope* t, const int32_t* gdof_node,
                                   int32_t nl_maxiter, int32_t* cg_tot,
                                   int32_t* nl_tot, double* uxmax, double* umax,
                                   double* fmax, int32_t* errcode) {
-  if (!t || !gdof_node || nl_maxiter < 1)
+  if (!t || !gdof_node || nl_maxiter < 1 || nnode < 0 || !cg_tot || !nl_tot || !uxmax ||
+      !umax || !fmax || !errcode)
     return fail(SGEO_EARG, "sgeo_slope_run_srf: bad arguments");
   sgeo_mesh* m = t->mesh;
+  for (int64_t q = 0; q < 3 * (int64_t)nnode; q++)
+    if (gdof_node[q] < 0 || gdof_node[q] > m->neq)
+      return fail(SGEO_EARG, "sgeo_slope_run_srf: gdof_node[%lld]=%d outside 0..%d",
+                  (long long)q, gdof_node[q], m->neq);
   int32_t tot = 0, nl = 0, ec = 0;
   double fm = 0.0;
   for (nl = 1; nl <= nl_maxiter; nl++) {

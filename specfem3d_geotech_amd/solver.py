@@ -76,6 +76,13 @@ class Context:
             pass
 
 
+def pcg_solver_invalidate(ctx: Context) -> None:
+    """Drops the device copies pcg_solver caches on the context (keyed on the
+    host arrays' addresses and a strided fingerprint of k); needed after k,
+    dprecon or gdof_elmt were rewritten in place."""
+    check(L.lib().sgeo_pcg_solver_invalidate(ctx.h), "sgeo_pcg_solver_invalidate")
+
+
 def pcg_solver(ctx: Context, neq: int, nelmt: int, k: np.ndarray, u: np.ndarray,
                f: np.ndarray, dprecon: np.ndarray, gdof_elmt: np.ndarray,
                cg_tol: float, cg_maxiter: int) -> Tuple[np.ndarray, int, int, str]:

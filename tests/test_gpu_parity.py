@@ -18,7 +18,8 @@ import oracle as O
 from conftest import control_from_golden, golden, mesh_from_golden
 from specfem3d_geotech_amd.slope import slope3d
 from specfem3d_geotech_amd.solver import (K_F64, REDUCE_SEQUENTIAL, Context,
-                                          DeviceMesh, pcg_solver)
+                                          DeviceMesh, pcg_solver,
+                                          pcg_solver_invalidate)
 from specfem3d_geotech_amd.spec import hex2spec
 
 pytestmark = pytest.mark.gpu
@@ -133,6 +134,36 @@ def test_seam1_pcg_calls_match_reference(ctx):
         worst = max(worst, e)
         assert e < tol, (c, e, tol)
     print(f"worst per-call relative L2: {worst:.3e} (tolerance {tol:.3e})")
+
+
+def test_seam1_in_place_edit_after_invalidate(ctx_seq):
+    """k rewritten in place (same buffer) at an entry the cache fingerprint
+    does not sample: after sgeo_pcg_solver_invalidate the solve uses the
+    edited k, as the oracle on the edited k does."""
+    z = golden("slope_4x2x3")
+    mesh = mesh_from_golden(z)
+    spec = hex2spec(mesh)
+    gam, ym, nu, *_ = mesh.material_arrays()
+    km, _, _ = O.stiffness_bodyload(spec, gam, nu, ym)
+    km = np.ascontiguousarray(km, dtype=np.float64)
+    ctl = control_from_golden(z)
+    args = (z["call_uin"][0], z["call_f"][0], z["dinv"], spec.gdof_elmt,
+            ctl["cg_tol"], ctl["cg_maxiter"])
+    u0, _, ec0, _ = pcg_solver(ctx_seq, spec.neq, spec.nelmt, km, *args)
+    flat = km.reshape(-1)
+    # a diagonal entry of a free equation (gdof != 0), off the sampled words
+    g = np.asarray(spec.gdof_elmt).reshape(spec.nelmt, 81)
+    free = [e * 6561 + 82 * c for e in range(spec.nelmt) for c in range(81)
+            if g[e, c] != 0 and (e * 6561 + 82 * c) % 1024 != 0]
+    q = free[len(free) // 2]
+    assert q != flat.size - 1
+    flat[q] *= 1.5
+    pcg_solver_invalidate(ctx_seq)
+    u1, _, ec1, _ = pcg_solver(ctx_seq, spec.neq, spec.nelmt, km, *args)
+    u2, _, ec2 = O.pcg_solver(spec.neq, km, *args)
+    assert ec0 == ec1 == ec2 == 0
+    assert rel_l2(u1, u2) < 1e-12
+    assert rel_l2(u1, u0) > 1e-9  # the edit changed the solution
 
 
 def test_seam1_nonconvergence_errcode(ctx):
