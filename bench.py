@@ -103,7 +103,12 @@ def parse():
                     help="sweep mode: MFMA product launch geometry 'blocks_per_cu,xcd_split' "
                          "instead of the autotuner's pick (e.g. 3,1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-iters", type=int, default=30)
+    ap.add_argument("--roofline-steps", type=int, default=64,
+                    help="sweep mode: steps of the HIP-event window after the timed one")
+    ap.add_argument("--cpu-sample-iters", type=int, default=8,
+                    help="CG iterations of each reference process in the CPU baseline")
+    ap.add_argument("--cpu-port-iters", type=int, default=20,
+                    help="CG iterations of the oracle restatement in the CPU baseline")
     return ap.parse_args()
 
 
@@ -166,6 +171,7 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
     from specfem3d_geotech_amd.solver import K_F64, KFMT_PACKED
     steps = a.steps or 1000
     warm = a.warmup if a.warmup is not None else 100
+    prof_steps = a.roofline_steps
     B = a.trials
     ctl = wl.control
     gam, ym, nu, phi, coh, psi = mesh.material_arrays()
@@ -228,20 +234,29 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
         log("autotune (MFMA product launch geometry): " + ", ".join(
             f"{c['blocks_per_cu']}/CU {'xcd' if c['xcd_split'] else 'flat'} {c['ms']:.3f} ms"
             for c in tune["candidates"]))
+        def window(nsteps):
+            s_a = eng.stats()
+            t_a = time.perf_counter()
+            while True:
+                left = nsteps - (eng.stats()["batched_steps"] - s_a["batched_steps"])
+                if left <= 0:
+                    break
+                handle(eng.advance(max_steps=left, **kw))
+            ctx.synchronize()
+            return time.perf_counter() - t_a, s_a, eng.stats()
+
+        # the timed window: the production path (steps replayed from HIP
+        # graphs, no per-launch events), exactly `steps` batched steps
+        ctx.profile(0)
+        ended0 = len(done_rows)
+        dt_s, s0, s1 = window(steps)
+        ended_in_window = len(done_rows) - ended0
+        log(f"  timed: {steps} steps in {dt_s:.3f} s")
+        # the roofline window right after: the same steps launched one by one
+        # with HIP events around every MFMA product on the library's stream
         ctx.profile(1)
-        s0 = eng.stats()
-        t0 = time.perf_counter()
-        while True:
-            left = steps - (eng.stats()["batched_steps"] - s0["batched_steps"])
-            if left <= 0:
-                break
-            handle(eng.advance(max_steps=left, **kw))
-            log(f"  {eng.stats()['batched_steps'] - s0['batched_steps']} steps, "
-                f"{time.perf_counter() - t0:.1f} s")
-        ctx.synchronize()
-        dt_s = time.perf_counter() - t0
+        dtp_s, sp0, sp1 = window(prof_steps)
         nl, kms = ctx.profile(0)
-        s1 = eng.stats()
         nbytes = eng.nbytes()
     finally:
         eng.close()
@@ -250,16 +265,24 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
     ncg = s1["cg_iterations"] - s0["cg_iterations"]
     b_eff = ncg / max(nsteps, 1)  # trials iterating per step
     per_trial_ms = dt_s / max(ncg, 1) * 1e3
-    # algorithmic bytes of one element-product launch at b_eff columns:
-    # packed FP32 K, gdof, the gathered P and the written Y of those columns
-    kbytes = E * (3328 * 4 + 81 * 4 + 2 * 81 * 4 * b_eff)
+    # algorithmic bytes of one element-product launch at the roofline
+    # window's columns: packed FP32 K, gdof, the gathered P and the written
+    # Y of the iterating columns
+    b_prof = (sp1["cg_iterations"] - sp0["cg_iterations"]) / max(
+        sp1["batched_steps"] - sp0["batched_steps"], 1)
+    kbytes = E * (3328 * 4 + 81 * 4 + 2 * 81 * 4 * b_prof)
     res = dict(
         metric=METRIC, value=N * ncg / dt_s, unit="DOF-iterations/s", n_gpus=1,
         steps=nsteps, warmup=warm, ms_per_step=dt_s / nsteps * 1e3,
         higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype="f32 inner CG (MFMA) + f64 dots/x/residual/yield", data=DATA,
-        config=dict(workload=wl.description + f"; {B} form-A strength realisations, "
-                    "each an SRF sweep 1.00:0.05:2.00", nelmt=E, neq=N, trials=B,
+        config=dict(workload=wl.description + f"; {B} form-A strength realisations "
+                    "resident, each column walking its own SRF sweep 1.00:0.05:2.00 in "
+                    f"steady state; the window is {nsteps} batched CG steps (each advances "
+                    "every iterating trial one CG iteration), with the plastic body passes, "
+                    "PCG starts and FP64 residuals that fall inside it; "
+                    f"{ended_in_window} SRF trials ended in the window",
+                    nelmt=E, neq=N, trials=B,
                     form="batched mixed precision", k_storage="packed f32 (S=3328) + "
                     "packed f64 for the residual", cg_tol=ctl["cg_tol"], inner_tol=1e-5,
                     nl_tol=ctl["nl_tol"], nl_maxiter=ctl["nl_maxiter"],
@@ -272,15 +295,20 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
                          operator="packed f64",
                          ebe_kernel_ms=round(ms64 / max(n64, 1), 4)),
         speedup_vs_fp64_per_trial_iteration=round(fp64_ms_iter / per_trial_ms, 2),
-        trials_ended_in_run=len(done_rows),
+        trials_ended_in_window=ended_in_window,
         autotune=dict(blocks_per_cu=tune["blocks_per_cu"], xcd_split=tune["xcd_split"],
                       candidates_ms={f"{c['blocks_per_cu']}/CU,"
                                      f"{'xcd' if c['xcd_split'] else 'flat'}":
                                      round(c["ms"], 4) for c in tune["candidates"]}),
+        timing=dict(headline="HIP-graph replays of the batched steps (production path), "
+                    "wall clock around exactly `steps` steps",
+                    roofline=f"{prof_steps} further steps launched one by one with HIP "
+                             "events around every k_bmv_mfma launch on the library's stream",
+                    roofline_window_ms_per_step=round(dtp_s / max(prof_steps, 1) * 1e3, 4)),
         roofline=roofline("k_bmv_mfma<4> (v_mfma_f32_16x16x4f32)", kbytes, nl, kms,
-                          dict(bytes_model="E*(3328*4 + 81*4 + 2*81*4*b_eff), "
-                               "b_eff = mean trials iterating per step",
-                               b_eff=round(b_eff, 2),
+                          dict(bytes_model="E*(3328*4 + 81*4 + 2*81*4*b), "
+                               "b = mean trials iterating per step of the roofline window",
+                               b=round(b_prof, 2),
                                traffic_source="profiles/bmv_mfma_c1_b64_traffic.md "
                                "(c1, 64 trials iterating)"),
                           traffic=measured_traffic("k_bmv_mfma<4>")
@@ -530,10 +558,13 @@ def main():
 
     # the compiled reference on the CPU runs first, as child processes of a
     # process that has not touched the GPU yet
-    ref_cpu = None
+    ref_cpu = cpu = None
     if a.mode == "analysis" and not a.no_cpu_baseline:
         wl0 = workloads.get(a.workload)
         ref_cpu = reference_cpu_sample(wl0, wl0.make(), a.ref_seconds, a.ref_procs)
+    if a.mode in ("sweep", "pcg") and not a.no_cpu_baseline:
+        cpu = cpu_baseline(workloads.get(a.workload).make(), a.cpu_sample_iters,
+                           a.ref_procs, a.cpu_port_iters)
     from specfem3d_geotech_amd.solver import Context
 
     if a.mode == "serve":
@@ -560,8 +591,8 @@ def main():
             res = mode_analysis(a, ctx, wl, mesh, spec, prep_s, copy_gbs)
     finally:
         ctx.close()
-    if not a.no_cpu_baseline and a.mode not in ("analysis", "mixed"):
-        res["cpu_baseline"] = cpu_baseline(spec, mesh, a.cpu_sample_iters)
+    if cpu is not None:
+        res["cpu_baseline"] = cpu
     if ref_cpu is not None:
         # analyses/hour of the reference on these cores, from its measured
         # PCG rate and this run's CG iterations per analysis; both the
@@ -641,30 +672,90 @@ def reference_cpu_sample(wl, mesh, seconds, procs):
                        "a link-time tap")
 
 
-def cpu_baseline(spec, mesh, iters):
-    """The oracle's C restatement (OpenMP element products, serial
-    element-order assembly) on the same operator: K build on the host, then
-    `iters` unconverged FP64 CG iterations of the elastic solve."""
+def cpu_baseline(mesh, iters, procs, port_iters):
+    """CPU baselines of the EBE-PCG on the benchmark's own mesh, on this
+    host's cores, before the GPU is touched.
+
+    kind "reference": the reference's own compiled pcg_solver (the
+    unmodified src/solver.f90 built by oracle/Makefile, driven by
+    oracle/_ref/ref_pcg_bench) on the elastic system K u = extload of this
+    mesh, `procs` concurrent serial processes (one core each, one shared
+    memory-mapped copy of K), `iters` unconverged iterations each.  K,
+    dprecon and extload come from the oracle's C restatement (OpenMP), which
+    equals the reference's stiffness_bodyload bit for bit; the reference's
+    own serial K build of a 144,000-element mesh takes minutes and is not
+    part of the timed solve.  Secondary (kind "port"): the oracle's C
+    restatement of pcg_solver (OpenMP element products) on the same K."""
+    import shutil
+    import subprocess
+    import tempfile
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
+    from specfem3d_geotech_amd.spec import hex2spec
     cores = min(os.cpu_count() or 1, 16)
     os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    spec = hex2spec(mesh)
     gam, ym, nu, *_ = mesh.material_arrays()
+    t0 = time.perf_counter()
     km, dp, ext = O.stiffness_bodyload(spec, gam, nu, ym)
     dinv = np.zeros_like(dp)
     dinv[1:] = 1.0 / dp[1:]
+    log(f"cpu baseline: K of {spec.nelmt} elements built on the host in "
+        f"{time.perf_counter() - t0:.1f} s")
+    out = None
+    binp = os.path.join(ROOT, "oracle", "_ref", "ref_pcg_bench")
+    if os.path.exists(binp):
+        base = "/dev/shm" if os.path.isdir("/dev/shm") else None
+        work = tempfile.mkdtemp(prefix="refpcg_", dir=base)
+        try:
+            km.tofile(os.path.join(work, "k.bin"))
+            ext.tofile(os.path.join(work, "f.bin"))
+            dinv.tofile(os.path.join(work, "dinv.bin"))
+            np.ascontiguousarray(spec.gdof_elmt, dtype=np.int32).tofile(
+                os.path.join(work, "gdof.bin"))
+            cmd = (f"ulimit -s unlimited; exec {binp} {work} {spec.neq} {spec.nelmt} "
+                   f"{iters}")
+            env = dict(os.environ, OMP_NUM_THREADS="1")
+            jobs = [subprocess.Popen(["bash", "-c", cmd], env=env, stdout=subprocess.PIPE,
+                                     stderr=subprocess.DEVNULL, text=True)
+                    for _ in range(procs)]
+            log(f"cpu baseline: the reference's pcg_solver, {procs} processes x {iters} "
+                "iterations")
+            secs = []
+            for j in jobs:
+                o, _ = j.communicate()
+                f = o.split()
+                if j.returncode == 0 and len(f) == 3 and int(f[0]) == iters + 1:
+                    secs.append(float(f[2]))
+        finally:
+            shutil.rmtree(work, ignore_errors=True)
+        if secs:
+            per = [spec.neq * iters / t for t in secs]
+            out = dict(value=sum(per), unit="DOF-iterations/s", cores=len(secs),
+                       kind="reference",
+                       sample=f"the reference's compiled pcg_solver (src/solver.f90, "
+                              f"amdflang -O2) on this {spec.nelmt}-element mesh's elastic "
+                              f"system, {len(secs)} concurrent serial processes x {iters} "
+                              "unconverged FP64 CG iterations",
+                       seconds_per_iteration=round(sum(secs) / len(secs) / iters, 4))
+    if out is None:
+        log("cpu baseline: oracle/_ref/ref_pcg_bench not built; the oracle's restatement only")
     t0 = time.perf_counter()
     _, it, ec = O.pcg_solver(spec.neq, km, np.zeros(spec.neq + 1), ext, dinv,
-                             spec.gdof_elmt, 1e-300, iters)
+                             spec.gdof_elmt, 1e-300, port_iters)
     dt = time.perf_counter() - t0
     del km
-    return dict(value=spec.neq * iters / dt, unit="DOF-iterations/s",
+    port = dict(value=spec.neq * port_iters / dt, unit="DOF-iterations/s",
                 cores=int(os.environ.get("OMP_NUM_THREADS", cores)), kind="port",
-                sample=f"{iters} unconverged FP64 CG iterations of the elastic solve on "
-                       f"the same {spec.nelmt}-element mesh (oracle C restatement, "
+                sample=f"{port_iters} unconverged FP64 CG iterations of the elastic solve "
+                       f"on the same {spec.nelmt}-element mesh (oracle C restatement, "
                        "OpenMP element products, serial element-order assembly)",
-                ms_per_iter=dt / iters * 1e3)
+                ms_per_iter=round(dt / port_iters * 1e3, 2))
+    if out is None:
+        return port
+    out["port"] = port
+    return out
 
 
 if __name__ == "__main__":

@@ -86,10 +86,12 @@ int sgeo_profile(sgeo_ctx *ctx, int enable, int64_t *launches,
  * (0 converged, -1 not converged) and errtag (250 chars, blank padded,
  * "ERROR: PCG solver doesn't converge!" on -1) follow the reference.
  * Device copies of k, dprecon and gdof_elmt are cached on the context and
- * reused while the host pointers, sizes and a strided content fingerprint
- * are unchanged.  The fingerprint reads one value in 1024 of k: a caller
- * that rewrites k, dprecon or gdof_elmt in place (same pointers) calls
- * sgeo_pcg_solver_invalidate before the next solve. */
+ * reused while the host pointers, sizes and a content fingerprint are
+ * unchanged.  The fingerprint reads every value of dprecon and gdof_elmt
+ * and 8192 evenly spaced values of k, so a rebuild of K that changes any
+ * element's diagonal (any change of E or nu) is seen through dprecon.  A
+ * caller that rewrites k in place without changing dprecon (same pointers)
+ * calls sgeo_pcg_solver_invalidate before the next solve. */
 int sgeo_pcg_solver(sgeo_ctx *ctx, int32_t neq, int32_t nelmt,
                     const double *k, double *u, const double *f,
                     const double *dprecon, const int32_t *gdof_elmt,
@@ -99,6 +101,7 @@ int sgeo_pcg_solver(sgeo_ctx *ctx, int32_t neq, int32_t nelmt,
  * counterpart: the reference reads its arrays at every call), so the next
  * call uploads k, dprecon and gdof_elmt again. */
 int sgeo_pcg_solver_invalidate(sgeo_ctx *ctx);
+
 
 /* ---- ingest: GLL numbering on the device ---------------------------------
  * Replaces hex2spec + get_global + get_global_indirect_addressing
@@ -300,7 +303,9 @@ int sgeo_batch_plastic_iter(sgeo_batch *batch, const int32_t *is_last,
  * starts its next call while the others keep iterating.  Returns when at
  * least one trial ended (nl_isconv, nl_maxiter reached, or a PCG call that
  * did not converge: errcode -1), when the elastic solution has just become
- * available (memo_ready), when no column runs, or after max_steps CG steps.
+ * available (memo_ready), when no column runs, or once max_steps CG steps
+ * ran (never more: max_steps >= 1 is the exact count of a call that none of
+ * the other conditions ends early).
  * ended[col] = 1 marks the trials that ended in this call, with their
  * summary row (src/slope.F90:415): nl_tot, cg_tot, fmax; the column keeps
  * its state (sgeo_batch_download) until the next begin. */

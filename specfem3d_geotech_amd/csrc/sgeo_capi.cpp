@@ -747,8 +747,10 @@ extern "C" int sgeo_mesh_matvec(sgeo_mesh* m, const double* p, double* kp) {
 // once when the device-side stop flag is set, so over-enqueued iterations
 // cost only their launches.  The host reads the flag once per chunk.
 static int pcg_device(sgeo_mesh* m, double* d_u, const double* d_f, double tol,
-                      int32_t maxiter, int32_t* cg_iter, int32_t* errcode) {
+                      int32_t maxiter, int32_t* cg_iter, int32_t* errcode,
+                      double* part = nullptr) {
   sgeo_ctx* c = m->ctx;
+  if (!part) part = c->d_part;
   hipStream_t s = c->stream;
   const int64_t n1 = (int64_t)m->neq + 1;
   if (maxiter < 1) {
@@ -757,12 +759,12 @@ static int pcg_device(sgeo_mesh* m, double* d_u, const double* d_f, double tol,
     *errcode = -1;
     return 0;
   }
-  launch_pcg_begin(s, n1, d_u, c->d_part, c->d_sc, tol, maxiter);
+  launch_pcg_begin(s, n1, d_u, part, c->d_sc, tol, maxiter);
   ebe_elem(m, d_u, m->d_Y, 2);
   launch_assemble(s, n1, m->d_rowptr, m->d_slots, m->d_Y, m->d_kp, c->d_sc, 2);
   const int seq = c->reduction == SGEO_REDUCE_SEQUENTIAL;
   launch_pcg_init(s, n1, d_f, m->d_kp, m->d_dinv, m->d_r, m->d_z, m->d_p,
-                  c->d_part, c->d_sc, seq);
+                  part, c->d_sc, seq);
   int32_t enq = 0, iter_prev = 0;
   int chunk = 8;
   c->ev_used = 0;
@@ -773,9 +775,9 @@ static int pcg_device(sgeo_mesh* m, double* d_u, const double* d_f, double tol,
       ebe_elem(m, m->d_p, m->d_Y, 1);
       TRY(prof_mark(c));
       launch_pcg_assemble_dot(s, n1, m->d_rowptr, m->d_slots, m->d_Y, m->d_p,
-                              m->d_kp, c->d_part, c->d_sc, seq);
+                              m->d_kp, part, c->d_sc, seq);
       launch_pcg_update(s, n1, m->d_p, m->d_kp, m->d_dinv, d_u, m->d_r,
-                        m->d_z, c->d_part, c->d_sc, seq);
+                        m->d_z, part, c->d_sc, seq);
       launch_pcg_pupdate(s, n1, m->d_z, m->d_p, c->d_sc);
     }
     enq += n;
@@ -855,10 +857,14 @@ extern "C" int sgeo_pcg_solver(sgeo_ctx* c, int32_t neq, int32_t nelmt,
   HIPCHK(hipSetDevice(c->device));
   const size_t nk = (size_t)nelmt * 6561;
   const int64_t n1 = (int64_t)neq + 1;
+  // Cache key: every value of dprecon and gdof_elmt (O(neq), a few ms on
+  // the largest meshes) and 8192 strided values of k.  Any change of a
+  // material's stiffness changes diag(K) and so dprecon (the reference's
+  // phinu path rebuilds both in place, src/slope.F90:259-270).
   uint64_t sig = 1469598103934665603ULL ^ ((uint64_t)nelmt << 32) ^ (uint64_t)neq;
   sig = fp_mix(sig, k, nk * 8, 8192);
-  sig = fp_mix(sig, dprecon, n1 * 8, 4096);
-  sig = fp_mix(sig, gdof, (size_t)nelmt * 81 * 4, 4096);
+  sig = fp_mix(sig, dprecon, n1 * 8, (size_t)n1);
+  sig = fp_mix(sig, gdof, (size_t)nelmt * 81 * 4, (size_t)nelmt * 81 / 2);
   if (!c->cache || c->cache_k != k || c->cache_g != gdof ||
       c->cache_d != dprecon || c->cache_sig != sig ||
       c->cache->nelmt != nelmt || c->cache->neq != neq) {
@@ -1686,11 +1692,18 @@ extern "C" int sgeo_batch_accumulate_nodal(sgeo_batch* b, int32_t col, int32_t f
   return 0;
 }
 
+// A released column stops at once: its device state is set to OFF too
+// (queued behind the steps already launched), so a column abandoned in the
+// middle of a PCG call costs no further vector passes or FP64 residuals.
 extern "C" int sgeo_batch_release(sgeo_batch* b, int32_t col) {
   if (!b || col < 0 || col >= b->B) return fail(SGEO_EARG, "bad column");
   b->phase[col] = BPH_IDLE;
   b->need_start[col] = 0;
   b->cst[col] = BCOL_OFF;
+  b->h_cst[col] = BCOL_OFF;
+  HIPCHK(hipSetDevice(b->mesh->ctx->device));
+  launch_set_i32(b->mesh->ctx->stream, b->d_cst + col, BCOL_OFF);
+  HIPCHK(hipGetLastError());
   return 0;
 }
 
@@ -1945,7 +1958,7 @@ extern "C" int sgeo_batch_advance(sgeo_batch* b, double cg_tol, int32_t cg_maxit
                                   int64_t max_steps, int32_t* ended, int32_t* nl_tot,
                                   int32_t* cg_tot, double* fmax, int32_t* errcode,
                                   int32_t* memo_ready) {
-  if (!b || !ended || nl_maxiter < 1)
+  if (!b || !ended || nl_maxiter < 1 || max_steps < 1)
     return fail(SGEO_EARG, "sgeo_batch_advance: bad arguments");
   sgeo_ctx* c = b->mesh->ctx;
   HIPCHK(hipSetDevice(c->device));
@@ -1972,8 +1985,12 @@ extern "C" int sgeo_batch_advance(sgeo_batch* b, double cg_tol, int32_t cg_maxit
     for (int j = 0; j < B; j++)
       running = running || (b->phase[j] == BPH_RUN && pcg_running(b->cst[j]));
     if (running) {
-      TRY(batch_steps(b, chunk, &running));
-      steps += chunk;
+      // never past max_steps: whole graphs of 32, 16 or 8 steps while they
+      // fit, the last few steps launched one by one
+      int64_t run = std::min<int64_t>(chunk, max_steps - steps);
+      run = run >= 32 ? 32 : run >= 16 ? 16 : run >= 8 ? 8 : run;
+      TRY(batch_steps(b, (int)run, &running));
+      steps += run;
       chunk = std::min(chunk * 2, 32);
     }
     // columns whose PCG call finished: body pass

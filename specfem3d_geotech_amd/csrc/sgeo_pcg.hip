@@ -550,6 +550,20 @@ __global__ __launch_bounds__(256) void k_uerr_finalize(
 }
 
 // --------------------------------------------------------------------------
+// small fills
+// --------------------------------------------------------------------------
+__global__ void k_fill_i32(int64_t n, int32_t* __restrict__ a, int32_t v) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x)
+    a[q] = v;
+}
+
+// *a = v, queued on the stream (a kernel, not a memset node)
+void launch_set_i32(hipStream_t s, int32_t* a, int32_t v) {
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int64_t)1, a, v);
+}
+
+// --------------------------------------------------------------------------
 // launch wrappers
 // --------------------------------------------------------------------------
 void launch_ebe_elem_f64(hipStream_t s, int32_t nelmt, const double* K,

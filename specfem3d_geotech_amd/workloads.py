@@ -41,7 +41,13 @@ def _c0():
 
 
 def _c1():
-    return slope_mesh(120, 20, 60, materials=[
+    return c1_like(120, 20, 60)
+
+
+def c1_like(nx: int, ny: int, nz: int):
+    """The features of c1 (two layers split at z = 10 m, 2% interior jitter,
+    seed 1) on a coarser nx x ny x nz grid of the same 2H:1V slope."""
+    return slope_mesh(nx, ny, nz, materials=[
         Material(gam=20.0, ym=1e5, nu=0.3, phi=25.0, coh=8.0, psi=0.0),
         Material(gam=20.0, ym=1e5, nu=0.3, phi=30.0, coh=20.0, psi=0.0)],
         layer_z=[10.0], jitter=0.02, seed=1)

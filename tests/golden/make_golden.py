@@ -399,6 +399,137 @@ def slope_case(name, mesh, control, keep_calls):
     print(name, "neq", neq, "calls", len(cg), "summary", summ.tolist())
 
 
+def _chunk_run(mesh, control, srfs, work, name):
+    """One reference run over the SRF list `srfs` (tap, full mode) in its own
+    directory; returns the process."""
+    os.makedirs(work, exist_ok=True)
+    sem = write_reference_input(mesh, work, name, **dict(control, srf=srfs))
+    env = dict(os.environ, SGEO_TAP_DIR=work, SGEO_TAP_MODE="full")
+    return subprocess.Popen(["bash", "-c", f"ulimit -s unlimited; exec {REF_TAP} {sem}"],
+                            cwd=work, env=env, stdout=subprocess.DEVNULL,
+                            stderr=subprocess.DEVNULL)
+
+
+def sweep_case(name, mesh, control, chunk=7):
+    """A full SRF sweep of `mesh` through the reference, run as independent
+    chunks of `chunk` SRFs in parallel: the reference's parser keeps 80
+    characters of a value list (src/string_library.f90:547), and every SRF
+    trial restarts from zero state (src/slope.F90:248-249, 281-282), so the
+    chunks' rows concatenate to the one-run sweep, truncated at the first
+    failing SRF (src/slope.F90:441).  Stored: the summary rows, every PCG
+    call's cg_iter, the final displacement x of every SRF (the last PCG
+    call's result), the plastic-point flags (f >= 0) of every SRF's final
+    body pass, packed 8 to a byte, and K's sha256, dinv and extload."""
+    srf = list(control["srf"])
+    chunks = [srf[i:i + chunk] for i in range(0, len(srf), chunk)]
+    base = tempfile.mkdtemp(prefix=f"golden_{name}_")
+    procs = [_chunk_run(mesh, control, c, os.path.join(base, f"c{i}"), name)
+             for i, c in enumerate(chunks)]
+    for p in procs:
+        p.wait()
+    rows, cg_log, xs, flags = [], [], [], []
+    neq = k_sha = dinv = ext = None
+    npt = 27 * mesh.nelmt
+    for i, c in enumerate(chunks):
+        work = os.path.join(base, f"c{i}")
+        summ = parse_summary(find_summary(work))
+        if i == 0:
+            neq, k, dinv, _ = read_static(work)
+            k_sha = hashlib.sha256(np.ascontiguousarray(k).tobytes()).digest()
+            del k
+        n1 = neq + 1
+        rec = 16 + 24 * n1
+        calls = np.memmap(os.path.join(work, "pcg_calls.bin"), dtype=np.uint8, mode="r")
+        ncall = calls.size // rec
+        cgs = np.array([calls[q * rec:q * rec + 16].view(np.int32)[2]
+                        for q in range(ncall)], np.int32)
+        if i == 0:
+            ext = calls[16 + 8 * n1:16 + 16 * n1].view(np.float64).copy()
+        moh = np.memmap(os.path.join(work, "mohcouf.bin"), dtype=np.float64, mode="r")
+        ends = np.cumsum(summ[:, 2].astype(int))
+        assert ends[-1] == ncall and moh.size == ncall * npt, (ends, ncall, moh.size)
+        for s, e in enumerate(ends):
+            q = e - 1
+            xs.append(calls[q * rec + 16 + 16 * n1:(q + 1) * rec].view(np.float64).copy())
+            flags.append(np.packbits(np.asarray(moh[q * npt:(q + 1) * npt]) >= 0.0))
+        rows.append(summ)
+        cg_log.append(cgs)
+        del calls, moh
+        if summ[-1, 2] == control["nl_maxiter"]:
+            break  # the sweep ends at its first failing SRF
+    shutil.rmtree(base)
+    summ = np.concatenate(rows)
+    out = dict(mesh_arrays(mesh))
+    out.update(
+        neq=np.int32(neq), dinv=dinv, extload=ext,
+        k_sha256=np.frombuffer(k_sha, dtype=np.uint8),
+        summary=summ, cg_log=np.concatenate(cg_log), x_final=np.stack(xs),
+        flags_final=np.stack(flags),
+        control=np.array([control["cg_tol"], control["cg_maxiter"], control["nl_tol"],
+                          control["nl_maxiter"]]),
+        srf=np.array(srf))
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+    print(name, "neq", neq, "calls", out["cg_log"].size, "summary", summ.tolist())
+
+
+def c0_case():
+    """The 1,152-element homogeneous slope of the CPU-runnable configuration
+    (24x4x12, c=10 kPa, phi=20 deg), its full SRF sweep 1.00:0.05:2.00."""
+    from specfem3d_geotech_amd import workloads
+    w = workloads.get("c0")
+    sweep_case("c0_sweep", w.make(), dict(w.control))
+
+
+def c1_like_case():
+    """A reduced mesh with the 144,000-element configuration's features: the
+    2H:1V slope at 30x5x15 elements, two layers split at z = 10 m (upper
+    c=8 kPa, phi=25 deg; lower c=20 kPa, phi=30 deg), interior nodes
+    jittered by U[-0.02,0.02] x local spacing (seed 1); full SRF sweep."""
+    from specfem3d_geotech_amd import workloads
+    w = workloads.get("c1")
+    sweep_case("c1_like_sweep", workloads.c1_like(30, 5, 15), dict(w.control))
+
+
+def form_a_case(n=64, jobs=6):
+    """Form-A strength realisations (lognormal c and tan phi, workloads.form_a,
+    seed 3) of the layered 8x4x6 slope, each swept over SRF 1.0:0.2:2.0 with
+    nl_maxiter 60 by the reference: the summary rows of every realisation
+    (rows of realisation i at offsets[i]:offsets[i+1])."""
+    from specfem3d_geotech_amd import workloads
+    layered = slope_mesh(8, 4, 6, materials=[
+        Material(gam=20.0, ym=1e5, nu=0.3, phi=25.0, coh=8.0),
+        Material(gam=20.0, ym=1e5, nu=0.3, phi=30.0, coh=20.0)],
+        layer_z=[10.0], jitter=0.02, seed=1)
+    ctl = dict(cg_tol=1e-8, cg_maxiter=5000, nl_tol=5e-4, nl_maxiter=60,
+               srf=[1.0, 1.2, 1.4, 1.6, 1.8, 2.0])
+    real = workloads.form_a(n, len(layered.materials))
+    base = tempfile.mkdtemp(prefix="golden_forma_")
+    pending, rows = list(range(n)), {}
+    running = []
+    while pending or running:
+        while pending and len(running) < jobs:
+            i = pending.pop(0)
+            c, p = real[i]
+            work = os.path.join(base, f"r{i}")
+            sem = write_reference_input(layered.with_strengths(c, p), work, "fa", **ctl)
+            env = dict(os.environ, SGEO_TAP_DIR=work, SGEO_TAP_MODE="time")
+            running.append((i, work, subprocess.Popen(
+                ["bash", "-c", f"ulimit -s unlimited; exec {REF_TAP} {sem}"], cwd=work,
+                env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)))
+        i, work, pr = running.pop(0)
+        pr.wait()
+        rows[i] = parse_summary(find_summary(work))
+    shutil.rmtree(base)
+    offs = np.cumsum([0] + [rows[i].shape[0] for i in range(n)])
+    out = dict(mesh_arrays(layered))
+    out.update(coh=np.array([r[0][0] for r in real]), phi=np.array([r[1][0] for r in real]),
+               rows=np.concatenate([rows[i] for i in range(n)]), offsets=offs,
+               control=np.array([ctl["cg_tol"], ctl["cg_maxiter"], ctl["nl_tol"],
+                                 ctl["nl_maxiter"]]), srf=np.array(ctl["srf"]))
+    np.savez_compressed(os.path.join(HERE, "form_a_8x4x6.npz"), **out)
+    print("form_a", n, "realisations;", out["rows"].shape[0], "rows")
+
+
 def validation1_case():
     """The reference's own example, SRF 1.0 only (the first of its list)."""
     work = tempfile.mkdtemp(prefix="golden_v1_")
@@ -502,6 +633,7 @@ if __name__ == "__main__":
          "excavation_spread": excavation_example_spread,
          "list_format": list_format_case, "output": output_case,
          "output_excavation": output_excavation_case,
-         "excavation_overburden": excavation_overburden_case}[sys.argv[1]]()
+         "excavation_overburden": excavation_overburden_case,
+         "c0": c0_case, "c1_like": c1_like_case, "form_a": form_a_case}[sys.argv[1]]()
     else:
         main()
