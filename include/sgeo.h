@@ -102,6 +102,33 @@ int sgeo_pcg_solver(sgeo_ctx *ctx, int32_t neq, int32_t nelmt,
  * call uploads k, dprecon and gdof_elmt again. */
 int sgeo_pcg_solver_invalidate(sgeo_ctx *ctx);
 
+/* Scratch of sgeo_pcg_solver_noalloc: a fixed device array of this many
+ * bytes, one per GPU the library runs on, held until the process ends. */
+#define SGEO_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* sgeo_pcg_solver without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same solve, same iteration order and results as
+ * sgeo_pcg_solver (bit for bit, in either reduction mode), but every array
+ * is a DEVICE pointer the caller owns: k (81*81*nelmt, reference layout),
+ * u (neq+1, in/out), f, dprecon (neq+1), gdof_elmt (81*nelmt).  The
+ * equation -> element-slot map, the work vectors and the reduction partials
+ * live in the scratch array, rebuilt at every call; nothing is cached.
+ * Returns SGEO_EARG, before touching the GPU, when the call needs more than
+ * SGEO_NOALLOC_SCRATCH_BYTES (sgeo_pcg_solver_noalloc_bytes gives the
+ * need: about 1 KB per element plus 36 bytes per equation, so up to about
+ * 35,000 elements of a hexahedral mesh), and SGEO_EARG when gdof_elmt holds
+ * an entry outside 0..neq.  All calls on one GPU share its scratch array:
+ * calls that overlap in time on one GPU (two host threads, two contexts)
+ * race and must be serialised by the caller. */
+int sgeo_pcg_solver_noalloc(sgeo_ctx *ctx, int32_t neq, int32_t nelmt,
+                            const double *k, double *u, const double *f,
+                            const double *dprecon, const int32_t *gdof_elmt,
+                            double cg_tol, int32_t cg_maxiter, int32_t *cg_iter,
+                            int32_t *errcode, char *errtag);
+/* Scratch bytes sgeo_pcg_solver_noalloc needs for this size (0 if invalid). */
+size_t sgeo_pcg_solver_noalloc_bytes(int32_t neq, int32_t nelmt);
 
 /* ---- ingest: GLL numbering on the device ---------------------------------
  * Replaces hex2spec + get_global + get_global_indirect_addressing

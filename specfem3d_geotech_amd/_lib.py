@@ -34,6 +34,9 @@ SIGNATURES = {
     "sgeo_pcg_solver": (C.c_int, [vp, C.c_int32, C.c_int32, d_p, d_p, d_p, d_p, i_p,
                                   C.c_double, C.c_int32, i_p, i_p, C.c_char_p]),
     "sgeo_pcg_solver_invalidate": (C.c_int, [vp]),
+    "sgeo_pcg_solver_noalloc": (C.c_int, [vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp,
+                                          C.c_double, C.c_int32, i_p, i_p, C.c_char_p]),
+    "sgeo_pcg_solver_noalloc_bytes": (C.c_size_t, [C.c_int32, C.c_int32]),
     "sgeo_mesh_create": (C.c_int, [vp, C.c_int32, C.c_int32, d_p, i_p, i_p,
                                    C.c_int32, C.POINTER(vp)]),
     "sgeo_mesh_destroy": (C.c_int, [vp]),

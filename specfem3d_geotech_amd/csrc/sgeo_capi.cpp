@@ -66,6 +66,10 @@ struct sgeo_ctx {
   double* d_f = nullptr;
   size_t uf_cap = 0;
   int reduction = SGEO_REDUCE_TREE;
+  // this device's copy of the allocation-free entry point's scratch array
+  // (hipGetSymbolAddress at sgeo_init), and a pinned word for its checks
+  unsigned char* d_scratch = nullptr;
+  int32_t* h_flag = nullptr;
   // HIP-event timing of the element-product kernel launches in PCG loops
   bool prof = false;
   std::vector<hipEvent_t> ev;  // pairs, reused per chunk
@@ -200,6 +204,14 @@ static int ensure_part(sgeo_ctx* c, size_t n) {
 }
 
 // --------------------------------------------------------------------------
+// scratch of the allocation-free drop-in (sgeo_pcg_solver_noalloc): device
+// memory of a fixed size that the code object brings with it, one copy per
+// GPU, held until the process ends.  Its name is a host-side stand-in; host
+// code uses only the per-device address hipGetSymbolAddress returns.
+// --------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[SGEO_NOALLOC_SCRATCH_BYTES];
+
+// --------------------------------------------------------------------------
 // context
 // --------------------------------------------------------------------------
 extern "C" int sgeo_init(int device, sgeo_ctx** out) {
@@ -219,6 +231,10 @@ extern "C" int sgeo_init(int device, sgeo_ctx** out) {
   TRY(dalloc(&c->d_ps, 1));
   HIPCHK(hipHostMalloc((void**)&c->h_sc, sizeof(PcgScalars)));
   HIPCHK(hipHostMalloc((void**)&c->h_ps, sizeof(PlasticScalars)));
+  HIPCHK(hipHostMalloc((void**)&c->h_flag, sizeof(int32_t)));
+  // the device-current copy of the scratch array (every GPU has its own)
+  HIPCHK(hipGetSymbolAddress((void**)&c->d_scratch, HIP_SYMBOL(scratch)));
+  if (!c->d_scratch) return fail(SGEO_EHIP, "sgeo_init: no address for the scratch array");
   HIPCHK(hipMemsetAsync(c->d_sc, 0, sizeof(PcgScalars), c->stream));
   HIPCHK(hipMemsetAsync(c->d_ps, 0, sizeof(PlasticScalars), c->stream));
   HIPCHK(hipGetLastError());
@@ -238,6 +254,7 @@ extern "C" int sgeo_finalize(sgeo_ctx* c) {
   if (c->d_ps) hipFree(c->d_ps);
   if (c->h_sc) hipHostFree(c->h_sc);
   if (c->h_ps) hipHostFree(c->h_ps);
+  if (c->h_flag) hipHostFree(c->h_flag);
   for (hipEvent_t e : c->ev) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -894,6 +911,90 @@ extern "C" int sgeo_pcg_solver(sgeo_ctx* c, int32_t neq, int32_t nelmt,
   TRY(pcg_device(c->cache, c->d_u, c->d_f, cg_tol, cg_maxiter, cg_iter, errcode));
   HIPCHK(hipMemcpyAsync(u, c->d_u, n1 * 8, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  if (*errcode != 0) set_errtag(errtag, "ERROR: PCG solver doesn't converge!");
+  else set_errtag(errtag, "");
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// Seam 1 without allocation: device arrays in, the inverse map and every
+// work vector in the context's scratch array
+// --------------------------------------------------------------------------
+static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of scratch one call of sgeo_pcg_solver_noalloc takes (rowptr, slots,
+// element products, r, z, p, kp, the reduction partials and a flag word).
+static size_t noalloc_bytes(int64_t neq, int64_t nelmt) {
+  const int64_t n1 = neq + 1, ns = nelmt * 81;
+  const size_t npart = 4 * (size_t)vec_reduce_blocks(n1) + 64;
+  return align256((size_t)(n1 + 1) * 4) + align256((size_t)ns * 4) +
+         align256((size_t)ns * 8) + 4 * align256((size_t)n1 * 8) + align256(npart * 8) +
+         align256(4);
+}
+
+extern "C" size_t sgeo_pcg_solver_noalloc_bytes(int32_t neq, int32_t nelmt) {
+  if (neq <= 0 || nelmt <= 0) return 0;
+  return noalloc_bytes(neq, nelmt);
+}
+
+extern "C" int sgeo_pcg_solver_noalloc(sgeo_ctx* c, int32_t neq, int32_t nelmt,
+                                       const double* d_k, double* d_u, const double* d_f,
+                                       const double* d_dprecon, const int32_t* d_gdof,
+                                       double cg_tol, int32_t cg_maxiter, int32_t* cg_iter,
+                                       int32_t* errcode, char* errtag) {
+  if (!c || !d_k || !d_u || !d_f || !d_dprecon || !d_gdof || !cg_iter || !errcode ||
+      neq <= 0 || nelmt <= 0)
+    return fail(SGEO_EARG, "sgeo_pcg_solver_noalloc: bad arguments");
+  if ((int64_t)nelmt * 81 > INT32_MAX)
+    return fail(SGEO_EARG, "sgeo_pcg_solver_noalloc: nelmt*81 exceeds int32");
+  const size_t need = noalloc_bytes(neq, nelmt);
+  if (need > (size_t)SGEO_NOALLOC_SCRATCH_BYTES)
+    return fail(SGEO_EARG,
+                "sgeo_pcg_solver_noalloc: neq=%d, nelmt=%d need %zu bytes of scratch, "
+                "more than its %zu",
+                neq, nelmt, need, (size_t)SGEO_NOALLOC_SCRATCH_BYTES);
+  if (!c->d_scratch) return fail(SGEO_ESTATE, "sgeo_pcg_solver_noalloc: no scratch");
+  set_errtag(errtag, "ERROR: unknown!");
+  *errcode = -1;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const int64_t n1 = (int64_t)neq + 1, ns = (int64_t)nelmt * 81;
+  // carve the scratch: each piece 256-byte aligned
+  unsigned char* q = c->d_scratch;
+  auto take = [&q](size_t bytes) {
+    unsigned char* r = q;
+    q += align256(bytes);
+    return r;
+  };
+  sgeo_mesh m;  // a view that owns nothing: the caller's K, dprecon and gdof,
+                // work arrays in the scratch
+  m.ctx = c;
+  m.nelmt = nelmt;
+  m.neq = neq;
+  m.kprec = SGEO_K_F64;
+  m.kfmt = SGEO_KFMT_FULL;
+  m.d_K64 = const_cast<double*>(d_k);
+  m.d_dinv = const_cast<double*>(d_dprecon);
+  m.d_gdof = const_cast<int32_t*>(d_gdof);
+  m.d_rowptr = reinterpret_cast<int32_t*>(take((size_t)(n1 + 1) * 4));
+  m.d_slots = reinterpret_cast<int32_t*>(take((size_t)ns * 4));
+  m.d_Y = reinterpret_cast<double*>(take((size_t)ns * 8));
+  m.d_r = reinterpret_cast<double*>(take((size_t)n1 * 8));
+  m.d_z = reinterpret_cast<double*>(take((size_t)n1 * 8));
+  m.d_p = reinterpret_cast<double*>(take((size_t)n1 * 8));
+  m.d_kp = reinterpret_cast<double*>(take((size_t)n1 * 8));
+  double* part = reinterpret_cast<double*>(take((4 * (size_t)vec_reduce_blocks(n1) + 64) * 8));
+  int32_t* bad = reinterpret_cast<int32_t*>(take(4));
+  // the equation -> slot map of gdof_elmt (z is the fill's cursor), then a
+  // check of the equation range before any product reads through gdof
+  launch_csr_build(s, ns, neq, d_gdof, m.d_rowptr, m.d_slots,
+                   reinterpret_cast<int32_t*>(m.d_z), bad);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->h_flag, bad, 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (*c->h_flag)
+    return fail(SGEO_EARG, "sgeo_pcg_solver_noalloc: gdof_elmt entry outside 0..%d", neq);
+  TRY(pcg_device(&m, d_u, d_f, cg_tol, cg_maxiter, cg_iter, errcode, part));
   if (*errcode != 0) set_errtag(errtag, "ERROR: PCG solver doesn't converge!");
   else set_errtag(errtag, "");
   return 0;

@@ -74,6 +74,8 @@ void launch_assemble_acc(hipStream_t s, int64_t n1, const int32_t* rowptr,
                          int subtract, const PlasticScalars* ps);
 void launch_invert_diag(hipStream_t s, int64_t n1, double* d);
 void launch_set_i32(hipStream_t s, int32_t* a, int32_t v);
+void launch_csr_build(hipStream_t s, int64_t ns, int32_t neq, const int32_t* gdof,
+                      int32_t* rowptr, int32_t* slots, int32_t* cursor, int32_t* bad);
 void launch_scale_by_mat(hipStream_t s, int32_t nelmt, const double* eld,
                          const int32_t* mat_id, const double* gam, double* Y);
 void launch_pcg_begin(hipStream_t s, int64_t n1, const double* u, double* part,

@@ -550,7 +550,10 @@ __global__ __launch_bounds__(256) void k_uerr_finalize(
 }
 
 // --------------------------------------------------------------------------
-// small fills
+// inverse map (equation -> element slots) built on the device: the same
+// CSR the host counting sort of sgeo_mesh_create builds (slots of every
+// equation g >= 1 in ascending e*81+i order; the index-0 sink stays empty,
+// no assembly reads it).  Every kernel checks the equation range itself.
 // --------------------------------------------------------------------------
 __global__ void k_fill_i32(int64_t n, int32_t* __restrict__ a, int32_t v) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
@@ -558,9 +561,121 @@ __global__ void k_fill_i32(int64_t n, int32_t* __restrict__ a, int32_t v) {
     a[q] = v;
 }
 
+// cnt[g + 1] = slots of equation g (1 <= g <= neq); *bad = 1 if an entry
+// lies outside 0..neq (such entries are skipped)
+__global__ void k_csr_count(int64_t ns, int32_t neq, const int32_t* __restrict__ gdof,
+                            int32_t* __restrict__ cnt, int32_t* __restrict__ bad) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ns;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t g = gdof[q];
+    if (g < 0 || g > neq) {
+      *bad = 1;
+      continue;
+    }
+    if (g > 0) atomicAdd(&cnt[g + 1], 1);
+  }
+}
+
+// in-place inclusive scan of a[0..n) by one block of 1024 threads, 4096
+// entries per round (4 per thread), the running total carried between rounds
+__global__ __launch_bounds__(1024) void k_scan_i32(int64_t n, int32_t* __restrict__ a) {
+  __shared__ int32_t wsum[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int32_t carry = 0;
+  for (int64_t base = 0; base < n; base += 4096) {
+    const int64_t i0 = base + (int64_t)t * 4;
+    int32_t v[4], s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      s += (i0 + k < n) ? a[i0 + k] : 0;
+      v[k] = s;
+    }
+    int32_t x = s;  // inclusive scan of the thread totals across the wave
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int32_t y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (w == 0) {
+      int32_t ws = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) {
+        const int32_t y = __shfl_up(ws, off, 64);
+        if (lane >= off) ws += y;
+      }
+      if (lane < 16) wsum[lane] = ws;
+    }
+    __syncthreads();
+    const int32_t excl = carry + (x - s) + (w > 0 ? wsum[w - 1] : 0);
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (i0 + k < n) a[i0 + k] = v[k] + excl;
+    carry += wsum[15];
+    __syncthreads();  // wsum is rewritten by the next round
+  }
+}
+
+// cursor[g] = rowptr[g]; slots[cursor[g]++] = q for every entry of g >= 1
+__global__ void k_csr_fill(int64_t ns, int32_t neq, const int32_t* __restrict__ gdof,
+                           int32_t* __restrict__ cursor, int32_t* __restrict__ slots) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ns;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t g = gdof[q];
+    if (g < 1 || g > neq) continue;
+    slots[atomicAdd(&cursor[g], 1)] = (int32_t)q;
+  }
+}
+
+// each equation's slots in ascending order (the fill's atomics leave them
+// in arrival order); rows hold at most a node's element count
+__global__ void k_csr_sort(int64_t n1, const int32_t* __restrict__ rowptr,
+                           int32_t* __restrict__ slots) {
+  for (int64_t g = 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n1;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t b = rowptr[g], e = rowptr[g + 1];
+    for (int32_t i = b + 1; i < e; i++) {
+      const int32_t v = slots[i];
+      int32_t j = i - 1;
+      while (j >= b && slots[j] > v) {
+        slots[j + 1] = slots[j];
+        j--;
+      }
+      slots[j + 1] = v;
+    }
+  }
+}
+
+__global__ void k_copy_i32(int64_t n, const int32_t* __restrict__ a, int32_t* __restrict__ b) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x)
+    b[q] = a[q];
+}
+
 // *a = v, queued on the stream (a kernel, not a memset node)
 void launch_set_i32(hipStream_t s, int32_t* a, int32_t v) {
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int64_t)1, a, v);
+}
+
+// rowptr (n1 + 1 entries) and slots of gdof (ns entries) into caller memory;
+// cursor (n1 entries) is a temporary; *bad (zeroed here) reports entries
+// outside 0..neq
+void launch_csr_build(hipStream_t s, int64_t ns, int32_t neq, const int32_t* gdof,
+                      int32_t* rowptr, int32_t* slots, int32_t* cursor, int32_t* bad) {
+  const int64_t n1 = (int64_t)neq + 1;
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_stride_blocks(n1 + 1)), dim3(SGEO_VEC_THREADS),
+                     0, s, n1 + 1, rowptr, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int64_t)1, bad, 0);
+  hipLaunchKernelGGL(k_csr_count, dim3(grid_stride_blocks(ns)), dim3(SGEO_VEC_THREADS), 0,
+                     s, ns, neq, gdof, rowptr, bad);
+  hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(1024), 0, s, n1 + 1, rowptr);
+  hipLaunchKernelGGL(k_copy_i32, dim3(grid_stride_blocks(n1)), dim3(SGEO_VEC_THREADS), 0,
+                     s, n1, rowptr, cursor);
+  hipLaunchKernelGGL(k_csr_fill, dim3(grid_stride_blocks(ns)), dim3(SGEO_VEC_THREADS), 0,
+                     s, ns, neq, gdof, cursor, slots);
+  hipLaunchKernelGGL(k_csr_sort, dim3(grid_stride_blocks(n1)), dim3(SGEO_VEC_THREADS), 0,
+                     s, n1, rowptr, slots);
 }
 
 // --------------------------------------------------------------------------
