@@ -103,6 +103,8 @@ def parse():
                     help="sweep mode: MFMA product launch geometry 'blocks_per_cu,xcd_split' "
                          "instead of the autotuner's pick (e.g. 3,1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--residual-period", type=int, default=2,
+                    help="batched engine: FP64 residuals on every n-th CG step")
     ap.add_argument("--roofline-steps", type=int, default=64,
                     help="sweep mode: steps of the HIP-event window after the timed one")
     ap.add_argument("--cpu-sample-iters", type=int, default=8,
@@ -186,6 +188,7 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
     eng = BatchEngine(dm, ext, B)
     if a.no_graphs:
         eng.set_graphs(False)
+    eng.set_residual_period(a.residual_period)
     real = workloads.form_a(B, len(gam))
     srf = list(ctl["srf"])
     nxt = [0] * B  # next SRF index per column (each column runs one analysis)
@@ -285,6 +288,7 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
                     nelmt=E, neq=N, trials=B,
                     form="batched mixed precision", k_storage="packed f32 (S=3328) + "
                     "packed f64 for the residual", cg_tol=ctl["cg_tol"], inner_tol=1e-5,
+                    residual_period=a.residual_period,
                     nl_tol=ctl["nl_tol"], nl_maxiter=ctl["nl_maxiter"],
                     batch_hbm_gb=round(nbytes / 1e9, 2), copy_kernel_gbs=round(copy_gbs, 1),
                     ingest_s=round(prep_s, 3), k_build_s=round(kbuild_s, 3)),

@@ -387,6 +387,13 @@ int sgeo_batch_autotune(sgeo_batch *batch, int32_t reps, int32_t *blocks_per_cu,
 /* Sets that geometry directly (blocks per CU 1..3, XCD split 0/1); the
  * default is 3 blocks per CU with the XCD split. */
 int sgeo_batch_set_geometry(sgeo_batch *batch, int32_t blocks_per_cu, int32_t xcd_split);
+/* FP64 residuals (the product on the packed FP64 operator that starts a
+ * column's PCG call or replaces its residual) are formed on every period-th
+ * CG step only (1..8, default 1), so one pass over the FP64 operator serves
+ * the columns of several steps; a column waiting for its residual idles.
+ * No result bit depends on the period, only the columns' timing.  No
+ * counterpart in the reference. */
+int sgeo_batch_set_residual_period(sgeo_batch *batch, int32_t period);
 /* Totals since creation, as of the last host synchronisation: CG
  * iterations of all trials (seeded elastic solutions not counted), FP64
  * residuals of all trials, and batched steps (one step = one CG iteration

@@ -28,7 +28,7 @@ from . import _lib as L
 from ._lib import check, f64, i32, ptr
 from .mesh import HexMesh
 from .plastic import dt_viscoplas, strength_reduction
-from .slope import SlopeResult, uxmax_umax
+from .slope import SlopeResult, slope3d, uxmax_umax
 from .solver import K_F64, Context, DeviceMesh, nodal_fields_call
 from .spec import SpecMesh, hex2spec_device
 
@@ -128,6 +128,12 @@ class BatchEngine:
     def set_geometry(self, blocks_per_cu: int, xcd_split: bool):
         check(L.lib().sgeo_batch_set_geometry(self.h, int(blocks_per_cu), int(xcd_split)),
               "sgeo_batch_set_geometry")
+
+    def set_residual_period(self, period: int):
+        """FP64 residuals on every `period`-th CG step only (timing, not
+        results, depends on it)."""
+        check(L.lib().sgeo_batch_set_residual_period(self.h, int(period)),
+              "sgeo_batch_set_residual_period")
 
     def nl_iters(self) -> np.ndarray:
         out = np.zeros(self.ncol, np.int32)
@@ -407,14 +413,21 @@ def slope3d_mixed(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int
     (row, x, plastic flags) as each SRF trial ends (in completion order),
     `progress` with the engine statistics now and then; `output`
     (output.SlopeOutput) receives the rows, displacements and nodal fields
-    for the reference's result files."""
-    if phinu:
-        # src/plastic_library.f90:72-86 changes nu with the SRF, and with it
-        # the stiffness (src/slope.F90:259-265): trials no longer share one
-        # operator, which the multi-trial engine relies on
-        raise NotImplementedError("phinu sweeps change K per SRF: use slope.slope3d")
+    for the reference's result files.  With `phinu`, SRFs whose Poisson's
+    ratio correction changes the stiffness run the whole sweep on the FP64
+    drop-in instead (the engine's trials share one operator)."""
     spec = spec or hex2spec_device(ctx, mesh)
     _, ym, nu, phi, coh, psi = mesh.material_arrays()
+    if phinu and any(strength_reduction(s, True, coh, nu, phi, psi)[4] == 1 for s in srf):
+        # src/plastic_library.f90:72-86 changes nu with the SRF, and with it
+        # the stiffness (src/slope.F90:259-265): such trials no longer share
+        # one operator, which the multi-trial engine relies on, so the sweep
+        # runs on the FP64 drop-in, which rebuilds K per SRF on the device
+        res = slope3d(ctx, mesh, cg_tol=cg_tol, cg_maxiter=cg_maxiter, nl_tol=nl_tol,
+                      nl_maxiter=nl_maxiter, srf=srf, phinu=True, spec=spec,
+                      on_srf=on_srf, output=output)
+        res.extra["engine"] = "fp64 drop-in (phinu changes the stiffness per SRF)"
+        return res
     dm, ext = device_mesh(ctx, mesh, spec)
     eng = BatchEngine(dm, ext, ncol)
     a = Analysis(coh=coh, phi=phi, psi=psi, srf=list(srf))

@@ -1418,6 +1418,9 @@ struct sgeo_batch {
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
   BmvGeom geom;  // launch geometry of the MFMA product
+  // FP64 residuals (starting and replacing columns) are formed on every
+  // resid_period-th step of a chunk; a column waiting for one idles
+  int resid_period = 1;
   int32_t* h_stage = nullptr;  // pinned staging of the small uploads
   unsigned stage_busy = 0;
   double step_us = 0.0;  // running mean of the wall time of one step
@@ -1561,6 +1564,14 @@ extern "C" int sgeo_batch_destroy(sgeo_batch* b) {
 extern "C" int sgeo_batch_set_graphs(sgeo_batch* b, int enable) {
   if (!b) return fail(SGEO_EARG, "null batch");
   b->use_graphs = enable != 0;
+  return 0;
+}
+
+extern "C" int sgeo_batch_set_residual_period(sgeo_batch* b, int32_t period) {
+  if (!b || period < 1 || period > 8)
+    return fail(SGEO_EARG, "sgeo_batch_set_residual_period: period 1..8");
+  if (period != b->resid_period) batch_drop_graphs(b);
+  b->resid_period = period;
   return 0;
 }
 
@@ -1865,23 +1876,23 @@ static bool pcg_running(int st) {
   return st == BCOL_START || st == BCOL_INNER || st == BCOL_REPLACE;
 }
 
-// `chunk` lockstep CG steps of every running column, then reads the states
-// back.  Each step first forms the FP64 residual of the columns that start
-// a call or replace their residual (a no-op launch when there are none), so
-// a column's arithmetic depends only on its own iterations.  h_bc holds the
-// device scalars on return.
 // One CG step of every column (launch sequence; no memset, no host work).
-static void batch_step_launches(sgeo_batch* b, bool prof) {
+// With `resid`, the step first forms the FP64 residual of the columns that
+// start a call or replace their residual (a no-op launch when there are
+// none); a column's arithmetic depends only on its own iterations.
+static void batch_step_launches(sgeo_batch* b, bool prof, bool resid) {
   sgeo_mesh* m = b->mesh;
   sgeo_ctx* c = m->ctx;
   hipStream_t s = c->stream;
   const int B = b->B;
   const int64_t n1 = b->n1;
-  launch_bmv_f64(s, m->nelmt, B, c->n_cu, m->d_KP64, m->d_gdof, b->d_x, b->d_Y64,
-                 b->d_cst, BCOL_BIT(BCOL_START) | BCOL_BIT(BCOL_REPLACE));
-  launch_bresid(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y64, b->d_load, b->d_dinv32,
-                b->d_r, b->d_p, b->d_part, b->d_cst);
-  launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_RESID);
+  if (resid) {
+    launch_bmv_f64(s, m->nelmt, B, c->n_cu, m->d_KP64, m->d_gdof, b->d_x, b->d_Y64,
+                   b->d_cst, BCOL_BIT(BCOL_START) | BCOL_BIT(BCOL_REPLACE));
+    launch_bresid(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y64, b->d_load, b->d_dinv32,
+                  b->d_r, b->d_p, b->d_part, b->d_cst);
+    launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_RESID);
+  }
   launch_bpupdate(s, n1, B, b->d_r, b->d_dinv32, b->d_p, b->d_cst, b->d_bc);
   if (prof) prof_mark(c);
   launch_bmv_mfma_f32(s, m->nelmt, B, c->n_cu, b->geom, m->d_KP32, m->d_gdof, b->d_p,
@@ -1895,6 +1906,9 @@ static void batch_step_launches(sgeo_batch* b, bool prof) {
   launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_INNER);
 }
 
+// `chunk` lockstep CG steps of every running column (FP64 residuals on
+// every resid_period-th of them, the first included), then reads the states
+// back; h_bc holds the device scalars on return.
 static int batch_steps(sgeo_batch* b, int chunk, bool* any_running) {
   sgeo_mesh* m = b->mesh;
   sgeo_ctx* c = m->ctx;
@@ -1905,7 +1919,8 @@ static int batch_steps(sgeo_batch* b, int chunk, bool* any_running) {
   if (b->use_graphs && !c->prof && gi >= 0) {
     if (!b->gexec[gi]) {
       HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      for (int it = 0; it < chunk; it++) batch_step_launches(b, false);
+      for (int it = 0; it < chunk; it++)
+        batch_step_launches(b, false, it % b->resid_period == 0);
       hipGraph_t g = nullptr;
       const hipError_t ec = hipStreamEndCapture(s, &g);
       if (ec != hipSuccess || !g) return fail(SGEO_EHIP, "graph capture failed");
@@ -1914,7 +1929,8 @@ static int batch_steps(sgeo_batch* b, int chunk, bool* any_running) {
     }
     HIPCHK(hipGraphLaunch(b->gexec[gi], s));
   } else {
-    for (int it = 0; it < chunk; it++) batch_step_launches(b, c->prof);
+    for (int it = 0; it < chunk; it++)
+      batch_step_launches(b, c->prof, it % b->resid_period == 0);
   }
   b->stat_steps += chunk;
   HIPCHK(hipGetLastError());

@@ -480,6 +480,63 @@ def c0_case():
     sweep_case("c0_sweep", w.make(), dict(w.control))
 
 
+def c0_spread_case():
+    """How far the reference's own c0 displacements move when its libm moves
+    by one ulp on about 1/64 of the arguments (tests/golden/ulp_shim.c
+    preloaded: asin only, then asin, sin, cos, tan and sincos): the converged
+    SRFs of the c0 sweep rerun under each variant, and the relative L2
+    distance of each SRF's final x from the unperturbed one stored beside the
+    c0 golden as x_spread[variant, srf].  A device math library differs from
+    the host libm in the last bit on some arguments, and the viscoplastic
+    iteration carries such differences into x: this is the reference's own
+    sensitivity to them."""
+    from specfem3d_geotech_amd import workloads
+    path = os.path.join(HERE, "c0_sweep.npz")
+    with np.load(path) as z:
+        out = {k: z[k] for k in z.files}
+    ctl = dict(workloads.get("c0").control)
+    summ = out["summary"]
+    conv = [float(s) for s, nl in zip(summ[:, 0], summ[:, 2]) if nl < ctl["nl_maxiter"]]
+    ctl["srf"] = conv
+    mesh = workloads.get("c0").make()
+    base = tempfile.mkdtemp(prefix="golden_c0spread_")
+    shim = os.path.join(base, "ulp_shim.so")
+    subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-o", shim,
+                    os.path.join(HERE, "ulp_shim.c"), "-ldl", "-lm"], check=True)
+    funcs = ["asin", "all"]
+    procs = []
+    for v in funcs:
+        work = os.path.join(base, v)
+        os.makedirs(work)
+        sem = write_reference_input(mesh, work, "c0s", **ctl)
+        env = dict(os.environ, SGEO_TAP_DIR=work, SGEO_TAP_MODE="full", LD_PRELOAD=shim,
+                   SGEO_ULP_FUNCS=v)
+        procs.append(subprocess.Popen(
+            ["bash", "-c", f"ulimit -s unlimited; exec {REF_TAP} {sem}"], cwd=work, env=env,
+            stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL))
+    for p in procs:
+        p.wait()
+    n1 = int(out["neq"]) + 1
+    rec = 16 + 24 * n1
+    spread = np.zeros((len(funcs), len(conv)))
+    rows = []
+    for i, v in enumerate(funcs):
+        work = os.path.join(base, v)
+        s = parse_summary(find_summary(work))
+        rows.append(s)
+        calls = np.memmap(os.path.join(work, "pcg_calls.bin"), dtype=np.uint8, mode="r")
+        for k, e in enumerate(np.cumsum(s[:, 2].astype(int))):
+            q = e - 1
+            x = calls[q * rec + 16 + 16 * n1:(q + 1) * rec].view(np.float64)
+            xr = out["x_final"][k]
+            spread[i, k] = np.linalg.norm(x - xr) / np.linalg.norm(xr)
+        del calls
+    shutil.rmtree(base)
+    out.update(x_spread=spread, spread_rows=np.stack(rows), spread_funcs=np.array(funcs))
+    np.savez_compressed(path, **out)
+    print("c0 spread", spread.tolist())
+
+
 def c1_like_case():
     """A reduced mesh with the 144,000-element configuration's features: the
     2H:1V slope at 30x5x15 elements, two layers split at z = 10 m (upper
@@ -555,6 +612,39 @@ def validation1_case():
     np.savez_compressed(os.path.join(HERE, "validation1_srf1.npz"), **out)
     shutil.rmtree(work)
     print("validation1", summ.tolist())
+
+
+def validation1_phinu_case():
+    """The reference's own example input/validation1_phinu (phinu=1: the
+    Poisson's ratio is corrected whenever sin(phi_f) < 1 - 2 nu, and the
+    stiffness rebuilt, src/plastic_library.f90:72-86, src/slope.F90:259-270)
+    at SRF 1.0 and 1.5, both of which change nu; timing tap only."""
+    work = tempfile.mkdtemp(prefix="golden_v1p_")
+    os.makedirs(os.path.join(work, "input"))
+    os.makedirs(os.path.join(work, "output"))
+    for f in os.listdir(REF_INPUT):
+        if f.startswith("validation1") and not f.endswith(".sem") and \
+                not f.endswith(".psem"):
+            shutil.copy(os.path.join(REF_INPUT, f), os.path.join(work, "input"))
+    src = open(os.path.join(REF_INPUT, "validation1_phinu.sem")).read()
+    old = "nsrf=9, srf=1.0 1.5 2.0 2.15 2.16 2.17 2.18 2.19 2.20, phinu=1"
+    assert old in src
+    src = src.replace(old, "nsrf=2, srf=1.0 1.5, phinu=1")
+    sem = os.path.join(work, "input", "validation1_phinu.sem")
+    open(sem, "w").write(src)
+    run_tap(sem, work, mode="time")
+    summ = parse_summary(find_summary(work))
+    times = np.loadtxt(os.path.join(work, "pcg_times.txt"), ndmin=2)
+    mesh, ctl = read_reference_input(sem)
+    assert ctl.get("phinu"), ctl
+    out = dict(mesh_arrays(mesh))
+    out.update(cg_log=times[:, 1].astype(np.int32), summary=summ,
+               control=np.array([ctl["cg_tol"], ctl["cg_maxiter"], ctl["nl_tol"],
+                                 ctl["nl_maxiter"]]), srf=np.array(ctl["srf"]),
+               phinu=np.int32(1))
+    np.savez_compressed(os.path.join(HERE, "validation1_phinu.npz"), **out)
+    shutil.rmtree(work)
+    print("validation1_phinu", summ.tolist())
 
 
 def gll_case():
@@ -634,6 +724,8 @@ if __name__ == "__main__":
          "list_format": list_format_case, "output": output_case,
          "output_excavation": output_excavation_case,
          "excavation_overburden": excavation_overburden_case,
-         "c0": c0_case, "c1_like": c1_like_case, "form_a": form_a_case}[sys.argv[1]]()
+         "c0": c0_case, "c0_spread": c0_spread_case, "c1_like": c1_like_case,
+         "form_a": form_a_case,
+         "validation1_phinu": validation1_phinu_case}[sys.argv[1]]()
     else:
         main()

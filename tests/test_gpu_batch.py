@@ -4,8 +4,10 @@
   (one column) and among other analyses that start at other times gives
   identical rows, displacements and plastic-point flags.
 * Against the FP64 reference (golden summaries from the compiled reference):
-  the same FOS bracket, the same plastic iteration counts on converged SRFs,
-  and displacements within the refinement tolerance.
+  a PCG solve within 1e-8 of the reference's; over a sweep the same FOS
+  bracket, the same plastic iteration counts on converged SRFs, and
+  displacements that differ from the reference's by less than the
+  reference's own CG truncation error.
 * One solve of K x = f against the FP64 oracle's PCG run to a tight
   tolerance: the FP32 inner CG with FP64 residual replacement, stopped by
   the reference's own test at cg_tol, is as accurate as the FP64 PCG
@@ -80,31 +82,71 @@ def test_batch_trial_identical_alone_and_in_mix(ctx):
         dm.close()
 
 
+@pytest.mark.parametrize("name", ["slope_4x2x3", "slope_8x4x6_layered"])
+def test_mixed_solve_within_1e8_of_reference_solve(ctx, name):
+    """One PCG solve, the reference's own first call (K x = extload from
+    x = 0, recorded by the pcg_solver tap): the FP32 inner CG with FP64
+    residual replacement, stopped by the reference's test at cg_tol, gives
+    the reference's FP64 solution to 1e-8 relative L2."""
+    from specfem3d_geotech_amd.plastic import dt_viscoplas
+    z = golden(name)
+    mesh = mesh_from_golden(z)
+    spec = hex2spec(mesh)
+    ctl = control_from_golden(z)
+    _, ym, nu, phi, coh, psi = mesh.material_arrays()
+    assert int(z["call_idx"][0]) == 0 and not z["call_uin"][0].any()
+    dm, ext = device_mesh(ctx, mesh, spec)
+    try:
+        eng = BatchEngine(dm, ext, 1)
+        try:
+            eng.begin(0, ym, nu, phi, coh, psi, dt_viscoplas(nu, phi, ym))
+            r = eng.plastic_iter(np.array([0], np.int32), ctl["cg_tol"], ctl["cg_maxiter"],
+                                 1e-5, ctl["nl_tol"])
+            x, _ = eng.download(0)
+        finally:
+            eng.close()
+    finally:
+        dm.close()
+    e = rel_l2(x, z["call_uout"][0])
+    print(f"{name}: mixed solve vs the reference's {e:.3e}; cg_iter "
+          f"{int(r['cg_iter'][0])} vs {int(z['cg_log'][0])}")
+    assert e <= 1e-8
+    assert abs(int(r["cg_iter"][0]) - int(z["cg_log"][0])) <= 2
+
+
 @pytest.mark.parametrize("name,ncol", [("slope_4x2x3", 1), ("slope_8x4x6_layered", 1),
                                        ("slope_8x4x6_layered", 4)])
 def test_mixed_sweep_matches_reference(ctx, name, ncol):
-    """Same FOS bracket as the FP64 reference; converged SRFs have the same
-    plastic iteration count and displacement maxima within 1e-7 (the FP64
-    reference's own PCG stops at a relative update of cg_tol = 1e-8)."""
+    """Same FOS bracket as the FP64 reference, the same plastic iteration
+    count at every converged SRF, and at every converged SRF a displacement
+    that differs from the reference's by less than the reference's own
+    distance from the exactly solved problem (its PCG stops at a relative
+    update of cg_tol = 1e-8; the viscoplastic loop carries each solve's
+    truncation into x): ||x_mixed - x_ref|| <= ||x_ref - x_tight||, x_ref and
+    x_tight the oracle's sweep of that SRF at cg_tol and at 1e-13."""
     z = golden(name)
     mesh = mesh_from_golden(z)
+    spec = hex2spec(mesh)
     ctl = control_from_golden(z)
-    res = slope3d_mixed(ctx, mesh, ncol=ncol, **ctl)
+    xs = {}
+    res = slope3d_mixed(ctx, mesh, ncol=ncol, on_srf=lambda row, x, fl: xs.__setitem__(
+        row["srf"], x.copy()), **ctl)
     ref = z["summary"]
     print(name, "stats", res.extra["stats"])
     assert len(res.rows) == ref.shape[0]
-    worst = 0.0
     for r, rr in zip(res.rows, ref):
         print(name, r, rr.tolist())
         assert r["srf"] == rr[0]
         if rr[2] < ctl["nl_maxiter"]:
             assert r["nl_tot"] == rr[2]
-            e = abs(r["uxmax"] - rr[3]) / rr[3]
-            worst = max(worst, e)
-            assert e <= 1e-7
+            one = dict(ctl, srf=[float(rr[0])])
+            x_ref = O.slope3d(spec, mesh, **one)["x"]
+            x_tight = O.slope3d(spec, mesh, **dict(one, cg_tol=1e-13))["x"]
+            e, e_ref = rel_l2(xs[r["srf"]], x_ref), rel_l2(x_ref, x_tight)
+            print(f"  x vs reference {e:.3e}; reference vs exact solves {e_ref:.3e}")
+            assert e <= max(1e-8, e_ref)
         else:
             assert r["nl_tot"] == ctl["nl_maxiter"]
-    print(f"{name}: worst converged-SRF uxmax relative difference {worst:.3e}")
     bracket = (float(ref[-2, 0]) if ref.shape[0] > 1 else None, float(ref[-1, 0])) \
         if ref[-1, 2] == ctl["nl_maxiter"] else None
     assert res.fos_bracket == bracket
@@ -242,3 +284,57 @@ def test_launch_geometry_does_not_change_bits(ctx):
         dm.close()
     for o in out[1:]:
         assert o == out[0]
+
+
+def test_column_released_mid_solve_restarts_clean(ctx):
+    """A column released in the middle of a PCG call (an abandoned
+    speculative SRF) stops on the device at once; a trial begun on it
+    afterwards gives the same bits as on a fresh engine, and the engine's
+    iteration count grows only by the surviving column's iterations."""
+    from specfem3d_geotech_amd.plastic import dt_viscoplas, strength_reduction
+    z = golden("slope_8x4x6_layered")
+    mesh = mesh_from_golden(z)
+    spec = hex2spec(mesh)
+    ctl = control_from_golden(z)
+    _, ym, nu, phi, coh, psi = mesh.material_arrays()
+    kw = dict(cg_tol=ctl["cg_tol"], cg_maxiter=ctl["cg_maxiter"], inner_tol=1e-5,
+              nl_tol=ctl["nl_tol"], nl_maxiter=20)
+
+    def begin(eng, col, s):
+        cohf, nuf, phif, psif, _ = strength_reduction(s, False, coh, nu, phi, psi)
+        eng.begin(col, ym, nuf, phif, cohf, psif, dt_viscoplas(nuf, phif, ym))
+
+    def finish(eng, col):
+        while True:
+            r = eng.advance(**kw)
+            if r["ended"][col]:
+                return eng.download(col), int(r["nl_tot"][col]), int(r["cg_tot"][col])
+
+    dm, ext = device_mesh(ctx, mesh, spec)
+    try:
+        fresh = BatchEngine(dm, ext, 2)
+        try:
+            begin(fresh, 0, 1.3)
+            want = finish(fresh, 0)
+        finally:
+            fresh.close()
+        eng = BatchEngine(dm, ext, 2)
+        try:
+            begin(eng, 0, 1.0)
+            begin(eng, 1, 1.6)
+            eng.advance(max_steps=24, **kw)  # both columns inside their PCG calls
+            eng.release(1)
+            s0 = eng.stats()
+            eng.advance(max_steps=16, **kw)
+            s1 = eng.stats()
+            # only column 0 iterates now: at most one CG iteration per step
+            assert s1["cg_iterations"] - s0["cg_iterations"] <= \
+                s1["batched_steps"] - s0["batched_steps"]
+            begin(eng, 1, 1.3)
+            got = finish(eng, 1)
+        finally:
+            eng.close()
+    finally:
+        dm.close()
+    assert got[1:] == want[1:]
+    assert np.array_equal(got[0][0], want[0][0]) and np.array_equal(got[0][1], want[0][1])

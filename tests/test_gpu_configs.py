@@ -80,7 +80,12 @@ def test_c0_fp64_sweep_matches_reference(ctx, ctx_seq):
         assert abs(r["uxmax"] - rr[3]) <= 1e-10 * rr[3]
         assert abs(r["umax"] - rr[4]) <= 1e-10 * rr[4]
         x, fl = got[r["srf"]]
-        assert rel_l2(x, z["x_final"][s]) <= 1e-10
+        e = rel_l2(x, z["x_final"][s])
+        # the reference's own spread under one-ulp libm changes (the device
+        # sin/cos/tan/asin differ from the host's in the last bit)
+        bar = max(1e-10, 2.0 * float(z["x_spread"][:, s].max()))
+        print(f"  x vs reference {e:.3e} (bar {bar:.3e})")
+        assert e <= bar
         ref_fl = np.unpackbits(z["flags_final"][s])[:fl.size].reshape(fl.shape)
         assert np.array_equal(fl.astype(bool), ref_fl.astype(bool))
     # the failing SRF, fast reductions
@@ -144,8 +149,11 @@ def test_validation1_srf1_on_gpu(ctx_seq):
 def test_c0_mixed_sweep_64_columns(ctx):
     """The mixed-precision sweep with 64 columns, every SRF of the list in
     flight at once (the B = 64 MFMA product and assembly): the reference's
-    bracket, its plastic iteration count at every converged SRF, and the
-    displacement of every converged SRF near the reference's."""
+    bracket, its plastic iteration count at every converged SRF, and at
+    every converged SRF a displacement that differs from the reference's by
+    less than the reference's own distance from the exactly solved problem
+    (each of its PCG solves stops at a relative update of 1e-8, and the
+    viscoplastic loop carries that truncation into x)."""
     z = golden("c0_sweep")
     mesh = mesh_from_golden(z)
     ctl = control_from_golden(z)
@@ -156,17 +164,19 @@ def test_c0_mixed_sweep_64_columns(ctx):
     print("stats", res.extra["stats"])
     assert res.fos_bracket == bracket_of(ref, ctl["nl_maxiter"])
     assert len(res.rows) == ref.shape[0]
+    conv = [float(s) for s, nl in zip(ref[:, 0], ref[:, 2]) if nl < ctl["nl_maxiter"]]
+    # the exactly solved problem: the FP64 drop-in with cg_tol = 1e-13
+    tight = {}
+    slope3d(ctx, mesh, on_srf=lambda row, x, fl: tight.__setitem__(row["srf"], x.copy()),
+            **dict(ctl, srf=conv, cg_tol=1e-13))
     for s, (r, rr) in enumerate(zip(res.rows, ref)):
         print("c0 mixed", r, rr.tolist(), flush=True)
         assert r["srf"] == rr[0] and r["nl_tot"] == rr[2]
         if rr[2] < ctl["nl_maxiter"]:
             e = rel_l2(got[r["srf"]], z["x_final"][s])
-            print(f"  x relative L2 vs reference {e:.3e}")
-            assert e <= MIXED_X_BAR
-
-
-# relative L2 of a converged SRF's displacement, mixed form vs the reference
-MIXED_X_BAR = 1e-7
+            e_ref = rel_l2(z["x_final"][s], tight[r["srf"]])
+            print(f"  x vs reference {e:.3e}; reference vs exact solves {e_ref:.3e}")
+            assert e <= max(1e-8, e_ref)
 
 
 def test_form_a_64_realisations_match_reference(ctx):
@@ -281,3 +291,32 @@ def test_c1_full_size_stiffness_and_capped_pcg(ctx_seq):
           f"bitwise {np.array_equal(u_g, u_o)}, rel {rel_l2(u_g, u_o):.3e}")
     assert (it_g, ec_g) == (it_o, ec_o) == (41, -1)
     assert np.array_equal(u_g, u_o)
+
+
+def test_validation1_phinu_rebuilds_stiffness_per_srf(ctx_seq):
+    """The reference's own input/validation1_phinu (phinu=1) at SRF 1.0 and
+    1.5: both correct the Poisson's ratio (sin(phi_f) < 1 - 2 nu,
+    src/plastic_library.f90:72-86), so the stiffness and its diagonal are
+    rebuilt on the device before each SRF (src/slope.F90:259-270).  Every
+    PCG call's cg_iter and both summary rows match the reference; the mixed
+    form's entry point routes such a sweep to the FP64 drop-in and gives the
+    same rows."""
+    from specfem3d_geotech_amd.plastic import strength_reduction
+    z = golden("validation1_phinu")
+    mesh = mesh_from_golden(z)
+    ctl = control_from_golden(z)
+    _, ym, nu, phi, coh, psi = mesh.material_arrays()
+    for s in ctl["srf"]:
+        assert strength_reduction(s, True, coh, nu, phi, psi)[4] == 1
+    res = slope3d(ctx_seq, mesh, phinu=True, **ctl)
+    ncall = 0
+    for r, rr in zip(res.rows, z["summary"]):
+        print("validation1_phinu", r, rr.tolist())
+        n = int(rr[2])
+        assert r["nl_tot"] == n and r["cg_tot"] == rr[1]
+        assert res.cg_log[ncall:ncall + n] == list(z["cg_log"][ncall:ncall + n])
+        ncall += n
+        assert abs(r["uxmax"] - rr[3]) <= 1e-10 * rr[3]
+    mix = slope3d_mixed(ctx_seq, mesh, phinu=True, ncol=2, **ctl)
+    assert "fp64" in mix.extra.get("engine", "")
+    assert mix.rows == res.rows
