@@ -186,6 +186,16 @@ int sgeo_calibrate_copy(sgeo_ctx *ctx, size_t bytes, int reps, double *gbs);
 int sgeo_mesh_download_stiffness(sgeo_mesh *mesh, double *storekm);
 /* kp = K p on the resident operator (host vectors, neq+1). */
 int sgeo_mesh_matvec(sgeo_mesh *mesh, const double *p, double *kp);
+/* The device's point functions of the plastic body on n records
+ * rec(9, n) = (stress(6), phi, c, psi), phi and psi in degrees:
+ * out(7, n) = sigm, dsbar, theta (stress_invariant, src/math_library.f90:
+ * 144-186), f (mohcouf, src/plastic_library.f90:95-108), dq1, dq2, dq3
+ * (mohcouq, :116-143).  Host arrays; for parity checks of the point math. */
+int sgeo_constitutive(sgeo_ctx *ctx, int32_t n, const double *rec, double *out);
+/* The device math library's asin, sin, cos or tan (fn 0..3) of n host
+ * values (no reference counterpart: compares the device functions the
+ * point math calls with the host libm the reference calls). */
+int sgeo_math(sgeo_ctx *ctx, int32_t fn, int32_t n, const double *x, double *y);
 /* HBM bytes held by the mesh. */
 int sgeo_mesh_bytes(sgeo_mesh *mesh, size_t *bytes);
 /* Node map for the nodal result fields (sgeo_slope_nodal_fields,

@@ -46,6 +46,8 @@ SIGNATURES = {
     "sgeo_mesh_set_operator": (C.c_int, [vp, C.c_int, C.c_int]),
     "sgeo_calibrate_copy": (C.c_int, [vp, C.c_size_t, C.c_int, d_p]),
     "sgeo_mesh_matvec": (C.c_int, [vp, d_p, d_p]),
+    "sgeo_constitutive": (C.c_int, [vp, C.c_int32, d_p, d_p]),
+    "sgeo_math": (C.c_int, [vp, C.c_int32, C.c_int32, d_p, d_p]),
     "sgeo_mesh_bytes": (C.c_int, [vp, C.POINTER(C.c_size_t)]),
     "sgeo_slope_create": (C.c_int, [vp, d_p, C.POINTER(vp)]),
     "sgeo_slope_destroy": (C.c_int, [vp]),

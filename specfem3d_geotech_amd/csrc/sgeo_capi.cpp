@@ -742,6 +742,41 @@ extern "C" int sgeo_calibrate_copy(sgeo_ctx* c, size_t bytes, int reps,
   return 0;
 }
 
+// Host arrays in, host arrays out: the device point functions of the
+// plastic body (parity checks of the point math).
+extern "C" int sgeo_constitutive(sgeo_ctx* c, int32_t n, const double* rec, double* out) {
+  if (!c || !rec || !out || n <= 0) return fail(SGEO_EARG, "sgeo_constitutive: bad arguments");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  double *d_rec = nullptr, *d_out = nullptr;
+  DevScratch tmp;
+  TRY(tmp.alloc(&d_rec, (size_t)n * 9));
+  TRY(tmp.alloc(&d_out, (size_t)n * 7));
+  HIPCHK(hipMemcpyAsync(d_rec, rec, (size_t)n * 72, hipMemcpyHostToDevice, s));
+  launch_constitutive(s, n, d_rec, d_out);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, d_out, (size_t)n * 56, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int sgeo_math(sgeo_ctx* c, int32_t fn, int32_t n, const double* x, double* y) {
+  if (!c || !x || !y || n <= 0 || fn < 0 || fn > 3)
+    return fail(SGEO_EARG, "sgeo_math: bad arguments");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  double *d_x = nullptr, *d_y = nullptr;
+  DevScratch tmp;
+  TRY(tmp.alloc(&d_x, (size_t)n));
+  TRY(tmp.alloc(&d_y, (size_t)n));
+  HIPCHK(hipMemcpyAsync(d_x, x, (size_t)n * 8, hipMemcpyHostToDevice, s));
+  launch_math(s, n, fn, d_x, d_y);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(y, d_y, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
 extern "C" int sgeo_mesh_matvec(sgeo_mesh* m, const double* p, double* kp) {
   if (!m || !p || !kp || m->kprec < 0) return fail(SGEO_EARG, "no stiffness");
   sgeo_ctx* c = m->ctx;

@@ -149,6 +149,42 @@ __device__ __forceinline__ void stress_invariant(const double* s, double& sigm,
   }
 }
 
+// mohcouf (src/plastic_library.f90:101-106): the Mohr-Coulomb yield value;
+// phi in degrees, theta in radians
+__device__ __forceinline__ double mohcouf(double phi, double c, double sigm, double dsbar,
+                                          double theta) {
+  const double deg2rad = 3.141592653589793 / 180.0;
+  const double phir = phi * deg2rad;
+  double snph, csph, snth, csth;
+  sincos(phir, &snph, &csph);
+  sincos(theta, &snth, &csth);
+  return snph * sigm + dsbar * (csth / sqrt(3.0) - snth * snph / 3.0) - c * csph;
+}
+
+// mohcouq (src/plastic_library.f90:123-141): derivatives of the potential
+// with respect to the three invariants; psi in degrees
+__device__ __forceinline__ void mohcouq(double psi, double dsbar, double theta,
+                                        double& dq1, double& dq2, double& dq3) {
+  const double deg2rad = 3.141592653589793 / 180.0;
+  const double psir = psi * deg2rad;
+  const double snth = sin(theta);
+  const double snps = sin(psir);
+  const double sq3 = sqrt(3.0);
+  dq1 = snps;
+  if (fabs(snth) > 0.49) {
+    const double c1 = snth < 0.0 ? -1.0 : 1.0;
+    dq2 = (sq3 * 0.5 - c1 * snps * 0.5 / sq3) * sq3 * 0.5 / dsbar;
+    dq3 = 0.0;
+  } else {
+    const double csth = cos(theta);
+    const double cs3th = cos(3.0 * theta);
+    const double tn3th = tan(3.0 * theta);
+    const double tnth = snth / csth;
+    dq2 = sq3 * csth / dsbar * ((1.0 + tnth * tn3th) + snps * (tn3th - tnth) / sq3) * 0.5;
+    dq3 = 0.5 * 3.0 * (sq3 * snth + snps * csth) / (cs3th * dsbar * dsbar);
+  }
+}
+
 // Factored material block of one strength-reduction trial
 // (src/plastic_library.f90:42-88) plus the trial's pseudo-time step.
 struct MatParams {
@@ -193,36 +229,11 @@ __device__ __forceinline__ double point_update(int i, const double jinv[3][3],
     inv[1] = dsbar;
     inv[2] = theta;
   }
-  // mohcouf (src/plastic_library.f90:101-106)
-  const double deg2rad = 3.141592653589793 / 180.0;
-  const double phir = mm.phif * deg2rad;
-  double snph, csph, snth, csth;
-  sincos(phir, &snph, &csph);
-  sincos(theta, &snth, &csth);
-  const double f = snph * sigm + dsbar * (csth / sqrt(3.0) - snth * snph / 3.0) -
-                   mm.cohf * csph;
+  const double f = mohcouf(mm.phif, mm.cohf, sigm, dsbar, theta);
   for (int s = 0; s < 6; s++) devp[s] = 0.0;
   if (!(f >= 0.0)) return f;
-  // mohcouq (src/plastic_library.f90:123-141)
-  const double psir = mm.psif * deg2rad;
-  const double snth2 = sin(theta);
-  const double snps = sin(psir);
-  const double sq3 = sqrt(3.0);
-  const double dq1 = snps;
-  double dq2, dq3;
-  if (fabs(snth2) > 0.49) {
-    const double c1 = snth2 < 0.0 ? -1.0 : 1.0;
-    dq2 = (sq3 * 0.5 - c1 * snps * 0.5 / sq3) * sq3 * 0.5 / dsbar;
-    dq3 = 0.0;
-  } else {
-    const double csth2 = cos(theta);
-    const double cs3th = cos(3.0 * theta);
-    const double tn3th = tan(3.0 * theta);
-    const double tnth = snth2 / csth2;
-    dq2 = sq3 * csth2 / dsbar *
-          ((1.0 + tnth * tn3th) + snps * (tn3th - tnth) / sq3) * 0.5;
-    dq3 = 0.5 * 3.0 * (sq3 * snth2 + snps * csth2) / (cs3th * dsbar * dsbar);
-  }
+  double dq1, dq2, dq3;
+  mohcouq(mm.psif, dsbar, theta, dq1, dq2, dq3);
   // formm (src/plastic_library.f90:164-206), flow = f*(m1 dq1+m2 dq2+m3 dq3),
   // erate = matmul(flow, effsigma), evp = erate*dt (src/slope.F90:351-356)
   const double sx = eff[0], sy = eff[1], sz = eff[2];

@@ -98,6 +98,8 @@ void launch_load(hipStream_t s, int64_t n1, const double* ext,
 void launch_uerr(hipStream_t s, int64_t n1, double* x, double* oldx,
                  double* part, PlasticScalars* ps, double nl_tol);
 void launch_plastic_body(hipStream_t s, const BodyArgs& b);
+void launch_constitutive(hipStream_t s, int32_t n, const double* rec, double* out);
+void launch_math(hipStream_t s, int32_t n, int fn, const double* x, double* y);
 void launch_elastic_stress(hipStream_t s, int32_t nelmt, const double* corners,
                            const int32_t* gdof, const int32_t* mat_id, const double* ym,
                            const double* nu, const double* x, double* stress);

@@ -527,4 +527,43 @@ void launch_plastic_body(hipStream_t s, const BodyArgs& b) {
                      b.vm_pt, b.sf_pt);
 }
 
+// The device's constitutive point functions on given records (stress(6),
+// phi, c, psi): sigm, dsbar, theta (stress_invariant), f (mohcouf), dq1..3
+// (mohcouq) -- the values the plastic body computes at a GLL point.
+__global__ void k_constitutive(int32_t n, const double* __restrict__ rec,
+                               double* __restrict__ out) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* r = rec + (int64_t)i * 9;
+  double sigm, dsbar, theta, dq1, dq2, dq3;
+  stress_invariant(r, sigm, dsbar, theta);
+  const double f = mohcouf(r[6], r[7], sigm, dsbar, theta);
+  mohcouq(r[8], dsbar, theta, dq1, dq2, dq3);
+  double* o = out + (int64_t)i * 7;
+  o[0] = sigm;
+  o[1] = dsbar;
+  o[2] = theta;
+  o[3] = f;
+  o[4] = dq1;
+  o[5] = dq2;
+  o[6] = dq3;
+}
+
+// The device math library's asin, sin, cos or tan (fn 0..3) of x.
+__global__ void k_math(int32_t n, int fn, const double* __restrict__ x,
+                       double* __restrict__ y) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double v = x[i];
+  y[i] = fn == 0 ? asin(v) : fn == 1 ? sin(v) : fn == 2 ? cos(v) : tan(v);
+}
+
+void launch_constitutive(hipStream_t s, int32_t n, const double* rec, double* out) {
+  hipLaunchKernelGGL(k_constitutive, dim3((n + 255) / 256), dim3(256), 0, s, n, rec, out);
+}
+
+void launch_math(hipStream_t s, int32_t n, int fn, const double* x, double* y) {
+  hipLaunchKernelGGL(k_math, dim3((n + 255) / 256), dim3(256), 0, s, n, fn, x, y);
+}
+
 }  // namespace sgeo
