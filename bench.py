@@ -531,7 +531,12 @@ def mode_serve(a):
     from specfem3d_geotech_amd.serving import Server, make_requests
     reqs = make_requests(a.requests, rate=a.rate or 4.0,
                          nelmt_range=(a.min_elements, a.max_elements), seed=4)
-    srv = Server(max_workers=a.workers, slope_columns=a.columns)
+    t_start = time.perf_counter()
+
+    def event(r, text):
+        log(f"    {time.perf_counter() - t_start:7.1f} s  request {r.tag} ({r.kind}, "
+            f"{r.nelmt} el): {text}")
+    srv = Server(max_workers=a.workers, slope_columns=a.columns, on_event=event)
 
     def done(r):
         log(f"  request {r.tag} {r.kind} {r.nelmt} el: latency {r.latency:.2f} s")

@@ -130,13 +130,15 @@ def excavation3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
                  kfmt: int = KFMT_FULL, spec: Optional[SpecMesh] = None,
                  progress=None, output=None, z0: float = 0.0, s0: float = 0.0,
                  k0: float = 0.0, form: str = "fp64",
-                 inner_tol: float = 1e-5) -> ExcavationResult:
+                 inner_tol: float = 1e-5, iter_progress=None) -> ExcavationResult:
     """Staged excavation on the GPU for one SRF.  The reference runs
     excavation with nsrf = 1 only (it reallocates and closes per-SRF state
     inside its SRF loop); the first SRF is used.  `output`
     (output.ExcavationOutput) receives the in-situ step and every stage's
     row, displacement and nodal fields (computed on the device) for the
-    reference's result files.  form "fp64" runs each increment's plastic
+    reference's result files; `progress` is called with each stage's row and
+    `iter_progress` with (stage, increment, nl_iter, cg_iter) after each
+    plastic iteration of the FP64 form.  form "fp64" runs each increment's plastic
     loop on the FP64 drop-in (operator kprec/kfmt), "mixed" on a one-column
     mixed-precision engine (FP32 inner CG, FP64 residuals to cg_tol, the
     elastic memo serving every increment after a stage's first); the
@@ -230,6 +232,8 @@ def excavation3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
                             raise PCGNotConverged("ERROR: PCG solver doesn't converge!")
                         cg_tot += r["cg_iter"]
                         fmax = r["fmax"]
+                        if iter_progress:
+                            iter_progress(i_ex + 1, inc + 1, nl_iter, r["cg_iter"])
                         if r["nl_isconv"] or nl_iter == nl_maxiter:
                             break
                     x = trial.x()

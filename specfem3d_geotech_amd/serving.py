@@ -82,8 +82,13 @@ class Server:
 
     def __init__(self, device: int = 0, hbm_budget: Optional[int] = None,
                  max_workers: int = 8, slope_columns: int = 8,
-                 context_factory=None, defer_frac: Optional[float] = None):
+                 context_factory=None, defer_frac: Optional[float] = None,
+                 on_event: Optional[Callable[[Request, str], None]] = None):
+        """on_event(request, text), if given, is called from the workers as a
+        request's work proceeds (SRF trials and plastic iterations ending,
+        batched-step counts)."""
         self.device = device
+        self.on_event = on_event
         self.max_workers = max_workers
         self.slope_columns = slope_columns
         self.defer_frac = defer_frac
@@ -103,19 +108,31 @@ class Server:
 
     def run_request(self, req: Request, ctx: Context):
         mesh = req.make_mesh()
+        ev = self.on_event
         if req.kind == "slope":
             from .batch import slope3d_mixed
             # SRFs run slope_columns at a time; with defer_frac, trials above
             # one that has run that fraction of nl_maxiter wait for it
             defer = (max(10, int(self.defer_frac * int(req.control["nl_maxiter"])))
                      if self.defer_frac else None)
+            hooks = {}
+            if ev:
+                hooks = dict(
+                    on_srf=lambda row, x, fl: ev(req, f"SRF {row['srf']} nl {row['nl_tot']}"),
+                    progress=lambda st: ev(req, f"{st['batched_steps']} steps"))
             return slope3d_mixed(ctx, mesh, ncol=self.slope_columns, defer_nl=defer,
-                                 **req.control)
+                                 **hooks, **req.control)
         from .excavation import excavation3d
         from .solver import KFMT_PACKED
+        hooks = {}
+        if ev:
+            hooks = dict(
+                progress=lambda row: ev(req, f"stage {row['stage']} nl {row['nl_tot']}"),
+                iter_progress=lambda st, inc, nl, cg: nl % 10 == 0 and ev(
+                    req, f"stage {st} increment {inc} plastic iteration {nl}"))
         # the packed FP64 operator: half the stiffness bytes of the full
         # (reference-layout) one, the same arithmetic to rounding
-        return excavation3d(ctx, mesh, kfmt=KFMT_PACKED, **req.control)
+        return excavation3d(ctx, mesh, kfmt=KFMT_PACKED, **hooks, **req.control)
 
     def _worker(self, req: Request, done: threading.Event):
         ctx = None

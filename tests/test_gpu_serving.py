@@ -37,3 +37,43 @@ def test_results_identical_alone_and_in_mix():
         m = [r for r in reqs if r.tag == a.tag][0]
         print(a.kind, a.tag, "alone", a.result.rows[:2])
         assert _summary(a) == _summary(m)
+
+
+def test_served_results_match_reference():
+    """A served slope sweep (the mixed multi-trial engine) and a served
+    staged excavation (the packed FP64 operator), in one mix with other
+    requests, against the compiled reference's own results on the same
+    meshes: the slope's FOS bracket and the plastic iteration count of every
+    converged SRF; every excavation stage's plastic iteration count, CG
+    iteration count within 1% and displacement maximum within 1e-8."""
+    from conftest import control_from_golden, golden, mesh_from_golden
+    from specfem3d_geotech_amd.serving import Request
+    zs = golden("slope_8x4x6_layered")
+    zp = golden("pit_8x6x6")
+    cp = zp["control"]
+    nexcav, ninc, *ids = [int(v) for v in zp["excav"]]
+    pit_ctl = dict(cg_tol=float(cp[0]), cg_maxiter=int(cp[1]), nl_tol=float(cp[2]),
+                   nl_maxiter=int(cp[3]), srf=[float(s) for s in zp["srf"]],
+                   nexcav=nexcav, nexcavid=[1] * nexcav, excavid=ids, ninc=ninc)
+    slope = Request("slope", lambda: mesh_from_golden(zs), control_from_golden(zs),
+                    int(zs["conn"].shape[0]), 0.0, tag="slope_8x4x6")
+    pit = Request("excavation", lambda: mesh_from_golden(zp), pit_ctl,
+                  int(zp["conn"].shape[0]), 0.05, tag="pit_8x6x6")
+    others = make_requests(3, rate=20.0, nelmt_range=(300, 900), seed=5, nl_maxiter=40)
+    for r in others:
+        r.arrival += 0.02
+    st = Server(max_workers=5, slope_columns=4).serve([slope, pit] + others)
+    print("mix:", st)
+    ref = zs["summary"]
+    nl_max = int(zs["control"][3])
+    assert slope.result.fos_bracket == (float(ref[-2, 0]), float(ref[-1, 0]))
+    for r, rr in zip(slope.result.rows, ref):
+        print("served slope", r, rr.tolist())
+        if rr[2] < nl_max:
+            assert r["nl_tot"] == rr[2]
+    for r, rr in zip(pit.result.rows, zp["summary"]):
+        print("served pit", r, rr.tolist())
+        assert r["nl_tot"] == rr[2]
+        assert abs(r["cg_tot"] - rr[1]) <= 0.01 * rr[1]
+        assert abs(r["uxmax"] - rr[3]) <= 1e-8 * rr[3]
+    assert len(pit.result.rows) == zp["summary"].shape[0]
