@@ -486,7 +486,8 @@ def c0_spread_case():
     preloaded: asin only, then asin, sin, cos, tan and sincos): the converged
     SRFs of the c0 sweep rerun under each variant, and the relative L2
     distance of each SRF's final x from the unperturbed one stored beside the
-    c0 golden as x_spread[variant, srf].  A device math library differs from
+    c0 golden as x_spread[variant, srf] (variants: asin, or all five, moved on
+    1/64 of the arguments; all five moved on half of them).  A device math library differs from
     the host libm in the last bit on some arguments, and the viscoplastic
     iteration carries such differences into x: this is the reference's own
     sensitivity to them."""
@@ -503,14 +504,18 @@ def c0_spread_case():
     shim = os.path.join(base, "ulp_shim.so")
     subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-o", shim,
                     os.path.join(HERE, "ulp_shim.c"), "-ldl", "-lm"], check=True)
-    funcs = ["asin", "all"]
+    # asin or all five functions moved on 1/64 of the arguments, and all
+    # five on half of them (a libm whose last bits are independent of the
+    # host's)
+    variants = [("asin", 64), ("all", 64), ("all", 2)]
+    funcs = [f"{v}/{n}" for v, n in variants]
     procs = []
-    for v in funcs:
-        work = os.path.join(base, v)
+    for (v, every), name in zip(variants, funcs):
+        work = os.path.join(base, name.replace("/", "_"))
         os.makedirs(work)
         sem = write_reference_input(mesh, work, "c0s", **ctl)
         env = dict(os.environ, SGEO_TAP_DIR=work, SGEO_TAP_MODE="full", LD_PRELOAD=shim,
-                   SGEO_ULP_FUNCS=v)
+                   SGEO_ULP_FUNCS=v, SGEO_ULP_EVERY=str(every))
         procs.append(subprocess.Popen(
             ["bash", "-c", f"ulimit -s unlimited; exec {REF_TAP} {sem}"], cwd=work, env=env,
             stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL))
@@ -521,7 +526,7 @@ def c0_spread_case():
     spread = np.zeros((len(funcs), len(conv)))
     rows = []
     for i, v in enumerate(funcs):
-        work = os.path.join(base, v)
+        work = os.path.join(base, v.replace("/", "_"))
         s = parse_summary(find_summary(work))
         rows.append(s)
         calls = np.memmap(os.path.join(work, "pcg_calls.bin"), dtype=np.uint8, mode="r")
@@ -618,7 +623,7 @@ def validation1_phinu_case():
     """The reference's own example input/validation1_phinu (phinu=1: the
     Poisson's ratio is corrected whenever sin(phi_f) < 1 - 2 nu, and the
     stiffness rebuilt, src/plastic_library.f90:72-86, src/slope.F90:259-270)
-    at SRF 1.0 and 1.5, both of which change nu; timing tap only."""
+    at SRF 1.0 and 1.05, each of which sets its own nu; timing tap only."""
     work = tempfile.mkdtemp(prefix="golden_v1p_")
     os.makedirs(os.path.join(work, "input"))
     os.makedirs(os.path.join(work, "output"))
@@ -629,7 +634,7 @@ def validation1_phinu_case():
     src = open(os.path.join(REF_INPUT, "validation1_phinu.sem")).read()
     old = "nsrf=9, srf=1.0 1.5 2.0 2.15 2.16 2.17 2.18 2.19 2.20, phinu=1"
     assert old in src
-    src = src.replace(old, "nsrf=2, srf=1.0 1.5, phinu=1")
+    src = src.replace(old, "nsrf=2, srf=1.0 1.05, phinu=1")
     sem = os.path.join(work, "input", "validation1_phinu.sem")
     open(sem, "w").write(src)
     run_tap(sem, work, mode="time")
