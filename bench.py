@@ -269,11 +269,11 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
     b_eff = ncg / max(nsteps, 1)  # trials iterating per step
     per_trial_ms = dt_s / max(ncg, 1) * 1e3
     # algorithmic bytes of one element-product launch at the roofline
-    # window's columns: packed FP32 K, gdof, the gathered P and the written
-    # Y of the iterating columns
+    # window's columns: packed FP32 K, gdof, the gathered P, the written Y
+    # and the written FP64 p.Kp share of the iterating columns
     b_prof = (sp1["cg_iterations"] - sp0["cg_iterations"]) / max(
         sp1["batched_steps"] - sp0["batched_steps"], 1)
-    kbytes = E * (3328 * 4 + 81 * 4 + 2 * 81 * 4 * b_prof)
+    kbytes = E * (3328 * 4 + 81 * 4 + (2 * 81 * 4 + 8) * b_prof)
     res = dict(
         metric=METRIC, value=N * ncg / dt_s, unit="DOF-iterations/s", n_gpus=1,
         steps=nsteps, warmup=warm, ms_per_step=dt_s / nsteps * 1e3,
@@ -310,7 +310,7 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
                              "events around every k_bmv_mfma launch on the library's stream",
                     roofline_window_ms_per_step=round(dtp_s / max(prof_steps, 1) * 1e3, 4)),
         roofline=roofline("k_bmv_mfma<4> (v_mfma_f32_16x16x4f32)", kbytes, nl, kms,
-                          dict(bytes_model="E*(3328*4 + 81*4 + 2*81*4*b), "
+                          dict(bytes_model="E*(3328*4 + 81*4 + (2*81*4 + 8)*b), "
                                "b = mean trials iterating per step of the roofline window",
                                b=round(b_prof, 2),
                                traffic_source="profiles/bmv_mfma_c1_b64_traffic.md "

@@ -41,9 +41,42 @@ __device__ __forceinline__ bool wanted(const int32_t* cst, int j, int mask) {
 // 16-lane groups of an A-operand read (c, c+1, c+2, c+3) on disjoint banks.
 // A column's result depends only on its own P column, so trials are
 // independent of the batch; columns not in `want_mask` are not written.
+//
+// The epilogue also forms the element's share of p.Kp,
+//   edot[e][j] = sum_r P(g(e,r), j) Y[e][r][j]   (FP64 sums of exact products),
+// since p.Kp = sum_g p_g sum_{e,r: g(e,r)=g} Y[e][r] = sum_e p_e . Y_e.  The
+// accumulator of row tile mt holds Y(16 mt + 4 kq + v, j) in lane (kq, j),
+// register v, while the B operand of k-step 4 mt + i holds P(16 mt + 4 i +
+// kq, j) in lane (kq, j).  A 4x4 transpose of (lane group, register) -- two
+// v_permlane32_swap and two v_permlane16_swap -- puts Y(16 mt + 4 i + kq, j)
+// in register i of lane (kq, j), beside its P.  The share is summed in one
+// order for every column tiling: per lane, row tiles 0..4 (each the four
+// products i = 0..3 in order), then row 80 in lane group 0, then the four
+// lane groups (xor 16, xor 32; IEEE addition commutes, so all four lanes
+// hold the same sum).  When a column tile's row tiles are spread over
+// several waves (B <= 32), the per-tile sums meet in LDS and one wave adds
+// them in that order.  Per-element shares keep the reduction independent of
+// which block computed an element.
 // --------------------------------------------------------------------------
 #define BMV_LDK 112
 #define BMV_KC 84
+
+// (lane group, register) transpose of a 4x4 block of 16-lane groups:
+// on return a[i] in lane group g holds what a[g] held in lane group i
+__device__ __forceinline__ void lanegroup_transpose4(float (&a)[4]) {
+  uint32_t u0 = __float_as_uint(a[0]), u1 = __float_as_uint(a[1]);
+  uint32_t u2 = __float_as_uint(a[2]), u3 = __float_as_uint(a[3]);
+  // lanes 32..63 of the first operand <-> lanes 0..31 of the second
+  auto s02 = __builtin_amdgcn_permlane32_swap(u0, u2, false, false);
+  auto s13 = __builtin_amdgcn_permlane32_swap(u1, u3, false, false);
+  // odd 16-lane rows of the first operand <-> even rows of the second
+  auto t01 = __builtin_amdgcn_permlane16_swap(s02[0], s13[0], false, false);
+  auto t23 = __builtin_amdgcn_permlane16_swap(s02[1], s13[1], false, false);
+  a[0] = __uint_as_float(t01[0]);
+  a[1] = __uint_as_float(t01[1]);
+  a[2] = __uint_as_float(t23[0]);
+  a[3] = __uint_as_float(t23[1]);
+}
 
 __device__ __forceinline__ int tri_col(int q) {  // c with c(c+1)/2 <= q < (c+1)(c+2)/2
   int c = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
@@ -70,6 +103,7 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
                                                   const int32_t* __restrict__ gdof,
                                                   const float* __restrict__ P,
                                                   float* __restrict__ Y,
+                                                  double* __restrict__ edot,
                                                   const int32_t* __restrict__ cst,
                                                   int want_mask) {
   constexpr int NS = 4 / NT;             // waves per column tile
@@ -78,6 +112,9 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
   constexpr int KPT = (3321 + 255) / 256;  // packed entries per thread
   __shared__ float Kf[BMV_KC * BMV_LDK];
   __shared__ int32_t sg[2][BMV_KC];
+  // p.Kp shares of the row tiles (and row 80) of each column tile, when they
+  // come from several waves
+  __shared__ double xs[NS > 1 ? NT : 1][NS > 1 ? NMT + 1 : 1][64];
   const int t = threadIdx.x;
   // element range of this block's XCD, and the block's stride within it
   const int xcd = blockIdx.x % nxcd;
@@ -90,7 +127,7 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
   // barriers
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int l = t & 63;
-  const int n = w % NT, mpart = w / NT;
+  const int n = w % NT, mpart = NS == 1 ? 0 : w / NT;
   const int j = n * 16 + (l & 15);
   const bool jv = j < B && wanted(cst, j, want_mask);
   const bool wave_on = __any(jv);
@@ -183,10 +220,44 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
         }
         if (row80 && kq == 0) Y[(e * 81 + 80) * B + j] = r80;
       }
+      // the element's share of p.Kp, per lane: row tiles in order, then row 80
+      double dsum = 0.0;
+#pragma unroll
+      for (int i = 0; i < NM; i++) {
+#pragma unroll
+        for (int mc = 0; mc < NMT; mc++) {
+          if (mc != mpart + i * NS) continue;  // wave-uniform; keeps bv indices static
+          float a[4] = {acc[i][0], acc[i][1], acc[i][2], acc[i][3]};
+          lanegroup_transpose4(a);
+          double u = (double)a[0] * (double)bv[4 * mc];
+          u = u + (double)a[1] * (double)bv[4 * mc + 1];
+          u = u + (double)a[2] * (double)bv[4 * mc + 2];
+          u = u + (double)a[3] * (double)bv[4 * mc + 3];
+          if (NS == 1) dsum = dsum + u;
+          else xs[n][mc][l] = u;
+        }
+      }
+      const double d80 = (row80 && kq == 0) ? (double)r80 * (double)bv[20] : 0.0;
+      if (NS == 1) {
+        dsum = dsum + d80;
+        dsum = dsum + __shfl_xor(dsum, 16);
+        dsum = dsum + __shfl_xor(dsum, 32);
+        if (jv && kq == 0) edot[e * B + j] = dsum;
+      } else if (row80) {
+        xs[n][NMT][l] = d80;
+      }
     }
     if (t < 81) sg[cur ^ 1][t] = greg;
     __syncthreads();  // Kf free for the next element, its gdof visible
     cur ^= 1;
+    if (NS > 1 && wave_on && mpart == 0) {  // the shares of the column tile's waves
+      double dsum = 0.0;
+#pragma unroll
+      for (int mc = 0; mc <= NMT; mc++) dsum = dsum + xs[n][mc][l];
+      dsum = dsum + __shfl_xor(dsum, 16);
+      dsum = dsum + __shfl_xor(dsum, 32);
+      if (jv && kq == 0) edot[e * B + j] = dsum;
+    }
     if (wave_on && en < xe1) {
 #pragma unroll
       for (int s = 0; s < 21; s++) bv[s] = P[(uint32_t)(sg[cur][4 * s + kq] * B + jl)];
@@ -291,18 +362,22 @@ __global__ __launch_bounds__(256) void k_bpupdate(int64_t n1, int B, int ck,
   }
 }
 
-// kp[g][j] = sum over slots of Y (element order, FP32); partial p.kp (FP64)
-// and max|p| for INNER columns.  The (at most 8 on hexahedral meshes)
-// element contributions of an equation are loaded together, then summed in
-// slot order.  WAVE_CHUNK: B = 64, so a wave's 64 lanes are the 64 columns
-// of one chunk and the index loads (rowptr, slots) are wave-uniform scalar
-// loads.
+// The update of an iteration, with kp assembled on the fly:
+//   kp[g][j] = sum over the slots of Y (element order, FP32),
+//   x += alpha p (FP64), r -= alpha kp, z = D r,
+// and the partials max|x| (after the update, x(0) = 0), r.z (FP64) and
+// max|p| for INNER columns.  The (at most 8 on hexahedral meshes) element
+// contributions of an equation are loaded together, then summed in slot
+// order; two equations per trip keep twice the loads in flight.
+// WAVE_CHUNK: B = 64, so a wave's 64 lanes are the 64 columns of one chunk
+// and the index loads (rowptr, slots) are wave-uniform scalar loads.
 template <bool WAVE_CHUNK>
-__global__ __launch_bounds__(256) void k_bassemble_dot(
+__global__ __launch_bounds__(256) void k_bupdate(
     int64_t n1, int B, int ck, const int32_t* __restrict__ rowptr,
     const int32_t* __restrict__ slots, const float* __restrict__ Y,
-    const float* __restrict__ p, float* __restrict__ kp,
-    double* __restrict__ part, const int32_t* __restrict__ cst) {
+    const float* __restrict__ p, const float* __restrict__ dinv,
+    double* __restrict__ x, float* __restrict__ r, double* __restrict__ part,
+    const int32_t* __restrict__ cst, const BCol* __restrict__ bc) {
   ChunkIdx c = chunk_of(n1, B, ck);
   if (WAVE_CHUNK) {
     const int ch = __builtin_amdgcn_readfirstlane((int)c.chunk);
@@ -312,9 +387,9 @@ __global__ __launch_bounds__(256) void k_bassemble_dot(
     c.valid = c.g0 < n1;
   }
   if (!c.valid || cst[c.j] != BCOL_INNER) return;
-  double s = 0.0, m = 0.0;
-  // kp[g] = the slots' Y values summed in slot order (the reference's
-  // element order); two equations per trip keep twice the loads in flight
+  const double alpha = bc[c.j].alpha;
+  const float alpha32 = (float)alpha;
+  double mx = 0.0, s = 0.0, mp = 0.0;
   auto gather = [&](int64_t g, float* v, int32_t& q0, int32_t& q1) {
     q0 = g > 0 ? rowptr[g] : 0;
     q1 = g > 0 ? rowptr[g + 1] : 0;
@@ -323,15 +398,21 @@ __global__ __launch_bounds__(256) void k_bassemble_dot(
       v[k] = (q0 + k < q1) ? Y[(int64_t)slots[q0 + k] * B + c.j] : 0.0f;
   };
   auto finish = [&](int64_t g, const float* v, int32_t q0, int32_t q1) {
-    float acc = 0.0f;
+    float kp = 0.0f;
 #pragma unroll
     for (int k = 0; k < 8; k++)
-      if (q0 + k < q1) acc = acc + v[k];
-    for (int32_t q = q0 + 8; q < q1; q++) acc = acc + Y[(int64_t)slots[q] * B + c.j];
-    kp[g * B + c.j] = acc;
-    const double pg = (double)p[g * B + c.j];
-    s = s + pg * (double)acc;
-    m = fmax(m, fabs(pg));
+      if (q0 + k < q1) kp = kp + v[k];
+    for (int32_t q = q0 + 8; q < q1; q++) kp = kp + Y[(int64_t)slots[q] * B + c.j];
+    const int64_t q = g * B + c.j;
+    const float pg = p[q];
+    const double xg = (g == 0) ? 0.0 : x[q] + alpha * (double)pg;
+    x[q] = xg;
+    mx = fmax(mx, fabs(xg));
+    mp = fmax(mp, fabs((double)pg));
+    const float rg = r[q] - alpha32 * kp;
+    r[q] = rg;
+    const float zg = dinv[g] * rg;
+    s = s + (double)rg * (double)zg;
   };
   int64_t g = c.g0;
   for (; g + 1 < c.g1; g += 2) {
@@ -349,35 +430,9 @@ __global__ __launch_bounds__(256) void k_bassemble_dot(
     finish(g, va, a0, a1);
   }
   const int64_t nch = nchunks(n1, ck);
-  part[(0 * nch + c.chunk) * B + c.j] = s;
-  part[(1 * nch + c.chunk) * B + c.j] = m;
-}
-
-// x += alpha p (FP64); r -= alpha kp; partial max|x| (after the update, with
-// x(0) = 0) and r.(D r) (FP64)
-__global__ __launch_bounds__(256) void k_bupdate(
-    int64_t n1, int B, int ck, const float* __restrict__ p, const float* __restrict__ kp,
-    const float* __restrict__ dinv, double* __restrict__ x, float* __restrict__ r,
-    double* __restrict__ part, const int32_t* __restrict__ cst,
-    const BCol* __restrict__ bc) {
-  const ChunkIdx c = chunk_of(n1, B, ck);
-  if (!c.valid || cst[c.j] != BCOL_INNER) return;
-  const double alpha = bc[c.j].alpha;
-  const float alpha32 = (float)alpha;
-  double mx = 0.0, s = 0.0;
-  for (int64_t g = c.g0; g < c.g1; g++) {
-    const int64_t q = g * B + c.j;
-    const double xg = (g == 0) ? 0.0 : x[q] + alpha * (double)p[q];
-    x[q] = xg;
-    mx = fmax(mx, fabs(xg));
-    const float rg = r[q] - alpha32 * kp[q];
-    r[q] = rg;
-    const float zg = dinv[g] * rg;
-    s = s + (double)rg * (double)zg;
-  }
-  const int64_t nch = nchunks(n1, ck);
   part[(0 * nch + c.chunk) * B + c.j] = mx;
   part[(1 * nch + c.chunk) * B + c.j] = s;
+  part[(2 * nch + c.chunk) * B + c.j] = mp;
 }
 
 // FP64 residual for START / REPLACE columns: r64 = load - sum Y64 (element
@@ -406,9 +461,7 @@ __global__ __launch_bounds__(256) void k_bresid(
     if (st == BCOL_START) p[q] = zg;
     s = s + (double)rg * (double)zg;
   }
-  const int64_t nch = nchunks(n1, ck);
-  part[(0 * nch + c.chunk) * B + c.j] = s;
-  part[(1 * nch + c.chunk) * B + c.j] = 0.0;
+  part[c.chunk * B + c.j] = s;
 }
 
 // Reductions of the chunk partials, in a fixed order independent of B:
@@ -418,17 +471,24 @@ __global__ __launch_bounds__(256) void k_bresid(
 // depends only on the chunk count (batch_group): 64 on large systems, 16
 // when there are few chunks, so a small system's stage 1 is one round of
 // loads and its stage 2 stays short.
+//   RESID: chunks of equations; r.z of the FP64 residual.
+//   ALPHA: the elements' p.Kp shares from the MFMA product (a "chunk" is an
+//          element).
+//   INNER: chunks of equations; max|x|, r.z, max|p|.
 #define BGROUP_MIN 16
+#define BFIN_KINDS 3
 
 __device__ __forceinline__ int fin_states(int what) {
   return what == BFIN_RESID ? (BCOL_BIT(BCOL_START) | BCOL_BIT(BCOL_REPLACE))
                             : BCOL_BIT(BCOL_INNER);
 }
 
-// partial kinds: RESID (sum r.z, -), ALPHA (sum p.kp, max |p|),
-// INNER (max |x|, sum r.z)
-__device__ __forceinline__ bool fin_max0(int what) { return what == BFIN_INNER; }
-__device__ __forceinline__ bool fin_max1(int what) { return what == BFIN_ALPHA; }
+__device__ __forceinline__ int fin_kinds(int what) { return what == BFIN_INNER ? 3 : 1; }
+
+// kind k of `what` combines by max (else by sum)
+__device__ __forceinline__ bool fin_max(int what, int k) {
+  return what == BFIN_INNER && k != 1;
+}
 
 __global__ __launch_bounds__(256) void k_bgroup(int64_t nch, int B, int G,
                                                 const double* __restrict__ part,
@@ -441,32 +501,27 @@ __global__ __launch_bounds__(256) void k_bgroup(int64_t nch, int B, int G,
   const int j = (int)(q % B);
   const int64_t g = q / B;
   if (!wanted(cst, j, fin_states(what))) return;
-  const bool max0 = fin_max0(what), max1 = fin_max1(what);
+  const int nk = fin_kinds(what);
   const int64_t b0 = g * G, b1 = b0 + G < nch ? b0 + G : nch;
-  double v0 = 0.0, v1 = 0.0;
-  int64_t b = b0;
-  // sixteen partials loaded together, then combined in chunk order
-  for (; b + 16 <= b1; b += 16) {
-    double a0[16], a1[16];
+  for (int k = 0; k < nk; k++) {
+    const bool mx = fin_max(what, k);
+    const double* src = part + k * nch * B;
+    double v = 0.0;
+    int64_t b = b0;
+    // sixteen partials loaded together, then combined in chunk order
+    for (; b + 16 <= b1; b += 16) {
+      double a[16];
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-      a0[k] = part[(0 * nch + b + k) * B + j];
-      a1[k] = part[(1 * nch + b + k) * B + j];
-    }
+      for (int i = 0; i < 16; i++) a[i] = src[(b + i) * B + j];
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-      v0 = max0 ? fmax(v0, a0[k]) : v0 + a0[k];
-      v1 = max1 ? fmax(v1, a1[k]) : v1 + a1[k];
+      for (int i = 0; i < 16; i++) v = mx ? fmax(v, a[i]) : v + a[i];
     }
+    for (; b < b1; b++) {
+      const double a = src[b * B + j];
+      v = mx ? fmax(v, a) : v + a;
+    }
+    part2[(k * ng + g) * B + j] = v;
   }
-  for (; b < b1; b++) {
-    const double a0 = part[(0 * nch + b) * B + j];
-    const double a1 = part[(1 * nch + b) * B + j];
-    v0 = max0 ? fmax(v0, a0) : v0 + a0;
-    v1 = max1 ? fmax(v1, a1) : v1 + a1;
-  }
-  part2[(0 * ng + g) * B + j] = v0;
-  part2[(1 * ng + g) * B + j] = v1;
 }
 
 __global__ __launch_bounds__(256) void k_bfinalize(int64_t nch, int B, int G,
@@ -479,55 +534,56 @@ __global__ __launch_bounds__(256) void k_bfinalize(int64_t nch, int B, int G,
   const int st = cst[j];
   if (!wanted(cst, j, fin_states(what))) return;  // uniform per block
   const int64_t ng = (nch + G - 1) / G;
-  const bool max0 = fin_max0(what), max1 = fin_max1(what);
-  double v0 = 0.0, v1 = 0.0;
+  const int nk = fin_kinds(what);
+  double v[BFIN_KINDS] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < nk; k++) {
+    const bool mx = fin_max(what, k);
+    double a = 0.0;
 #pragma unroll 4
-  for (int64_t b = threadIdx.x; b < ng; b += 256) {
-    const double a0 = part2[(0 * ng + b) * B + j];
-    const double a1 = part2[(1 * ng + b) * B + j];
-    v0 = max0 ? fmax(v0, a0) : v0 + a0;
-    v1 = max1 ? fmax(v1, a1) : v1 + a1;
+    for (int64_t b = threadIdx.x; b < ng; b += 256) {
+      const double y = part2[(k * ng + b) * B + j];
+      a = mx ? fmax(a, y) : a + y;
+    }
+    v[k] = mx ? block_max<4>(a, red) : block_sum<4>(a, red);
   }
-  v0 = max0 ? block_max<4>(v0, red) : block_sum<4>(v0, red);
-  v1 = max1 ? block_max<4>(v1, red) : block_sum<4>(v1, red);
   if (threadIdx.x != 0) return;
   BCol& c = bc[j];
   switch (what) {
     case BFIN_RESID:  // v0 = r.z of the FP64 residual
       c.nrep += 1;
       if (st == BCOL_START) {
-        c.rz = v0;
-        c.rz0 = v0;
+        c.rz = v[0];
+        c.rz0 = v[0];
         c.it = 0;
         c.pending = 0;
-        cst[j] = (v0 == 0.0) ? BCOL_DONE : BCOL_INNER;  // r = 0: x is exact
+        cst[j] = (v[0] == 0.0) ? BCOL_DONE : BCOL_INNER;  // r = 0: x is exact
       } else {  // replaces r_{k+1}: beta = r.z (true) / r_k.z_k
-        c.beta = v0 / c.rz_prev;
-        c.rz = v0;
-        c.rz0 = v0;
+        c.beta = v[0] / c.rz_prev;
+        c.rz = v[0];
+        c.rz0 = v[0];
         c.pending = 1;
         cst[j] = BCOL_INNER;
       }
       break;
-    case BFIN_ALPHA:  // v0 = p.kp, v1 = max|p|
-      c.maxp = v1;
-      c.alpha = c.rz / v0;
+    case BFIN_ALPHA:  // v0 = p.Kp
+      c.alpha = c.rz / v[0];
       c.pending = 0;
       break;
-    case BFIN_INNER: {  // v0 = max|x|, v1 = r.z
+    case BFIN_INNER: {  // v0 = max|x|, v1 = r.z, v2 = max|p|
       c.it += 1;
+      c.maxp = v[2];
       // relative update test of src/solver.f90:65 (after the x update)
-      if (fabs(c.alpha) * c.maxp / v0 <= c.tol) {
+      if (fabs(c.alpha) * v[2] / v[0] <= c.tol) {
         cst[j] = BCOL_DONE;
       } else if (c.it >= c.maxiter) {
         cst[j] = BCOL_FAIL;
       } else {
         c.rz_prev = c.rz;
-        if (v1 <= c.inner_tol * c.rz0) {
+        if (v[1] <= c.inner_tol * c.rz0) {
           cst[j] = BCOL_REPLACE;  // FP64 residual at the column's next step
         } else {
-          c.beta = v1 / c.rz;
-          c.rz = v1;
+          c.beta = v[1] / c.rz;
+          c.rz = v[1];
           c.pending = 1;
         }
       }
@@ -669,8 +725,15 @@ static inline int batch_group(int64_t nch) { return nch < 16384 ? 16 : 64; }
 
 int64_t batch_partials(int64_t n1) {
   const int64_t nch = nchunks_host(n1);
-  return 2 * nch + 2 * ((nch + BGROUP_MIN - 1) / BGROUP_MIN);  // chunk + group partials
+  // chunk + group partials of every kind
+  return BFIN_KINDS * (nch + (nch + BGROUP_MIN - 1) / BGROUP_MIN);
 }
+
+int64_t batch_element_partials(int32_t nelmt) {
+  return (int64_t)nelmt + ((int64_t)nelmt + BGROUP_MIN - 1) / BGROUP_MIN;
+}
+
+int64_t batch_equation_chunks(int64_t n1) { return nchunks_host(n1); }
 
 static inline unsigned chunk_blocks(int64_t n1, int B) {
   const int64_t per = 256 / B;  // chunks per block
@@ -689,7 +752,7 @@ static inline int bmv_threads(int B) {
 // XCD-aware element split (nxcd 8) or a plain grid-stride walk (nxcd 1).
 void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu, BmvGeom geom,
                          const float* KP, const int32_t* gdof, const float* P,
-                         float* Y, const int32_t* cst, int want_mask) {
+                         float* Y, double* edot, const int32_t* cst, int want_mask) {
   const int nx = geom.xcd_split ? BMV_XCD : 1;
   const int bpc = geom.blocks_per_cu < 1 ? 1 : geom.blocks_per_cu > 3 ? 3 : geom.blocks_per_cu;
   int64_t g = (int64_t)n_cu * bpc;
@@ -699,13 +762,13 @@ void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu, BmvGeom 
   const dim3 grid((unsigned)g);
   if (B >= 64)
     hipLaunchKernelGGL(k_bmv_mfma<4>, grid, dim3(256), 0, s, nelmt, B, nx, KP, gdof, P,
-                       Y, cst, want_mask);
+                       Y, edot, cst, want_mask);
   else if (B == 32)
     hipLaunchKernelGGL(k_bmv_mfma<2>, grid, dim3(256), 0, s, nelmt, B, nx, KP, gdof, P,
-                       Y, cst, want_mask);
+                       Y, edot, cst, want_mask);
   else
     hipLaunchKernelGGL(k_bmv_mfma<1>, grid, dim3(256), 0, s, nelmt, B, nx, KP, gdof, P,
-                       Y, cst, want_mask);
+                       Y, edot, cst, want_mask);
 }
 
 // FP64: columns in chunks of at most 32 (LDS <= 48 KB per block), 3 blocks
@@ -722,23 +785,17 @@ void launch_bmv_f64(hipStream_t s, int32_t nelmt, int B, int n_cu, const double*
                      lds, s, nelmt, Bc, B, KP, gdof, X, Y, cst, want_mask);
 }
 
-void launch_bassemble_dot(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
-                          const int32_t* slots, const float* Y, const float* p,
-                          float* kp, double* part, const int32_t* cst) {
+void launch_bupdate(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
+                    const int32_t* slots, const float* Y, const float* p,
+                    const float* dinv, double* x, float* r, double* part,
+                    const int32_t* cst, const BCol* bc) {
   const int ck = batch_chunk(n1);
   if (B == 64)
-    hipLaunchKernelGGL(k_bassemble_dot<true>, dim3(chunk_blocks(n1, B)), dim3(256), 0,
-                       s, n1, B, ck, rowptr, slots, Y, p, kp, part, cst);
+    hipLaunchKernelGGL(k_bupdate<true>, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1,
+                       B, ck, rowptr, slots, Y, p, dinv, x, r, part, cst, bc);
   else
-    hipLaunchKernelGGL(k_bassemble_dot<false>, dim3(chunk_blocks(n1, B)), dim3(256), 0,
-                       s, n1, B, ck, rowptr, slots, Y, p, kp, part, cst);
-}
-
-void launch_bupdate(hipStream_t s, int64_t n1, int B, const float* p,
-                    const float* kp, const float* dinv, double* x, float* r,
-                    double* part, const int32_t* cst, const BCol* bc) {
-  hipLaunchKernelGGL(k_bupdate, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
-                     batch_chunk(n1), p, kp, dinv, x, r, part, cst, bc);
+    hipLaunchKernelGGL(k_bupdate<false>, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1,
+                       B, ck, rowptr, slots, Y, p, dinv, x, r, part, cst, bc);
 }
 
 void launch_bpupdate(hipStream_t s, int64_t n1, int B, const float* r,
@@ -756,12 +813,13 @@ void launch_bresid(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
                      batch_chunk(n1), rowptr, slots, Y64, load, dinv, r, p, part, cst);
 }
 
-void launch_bfinalize(hipStream_t s, int64_t n1, int B, double* part,
+// `nch` partials per column and kind in `part` (equation chunks, or elements
+// for ALPHA); the group partials follow them
+void launch_bfinalize(hipStream_t s, int64_t nch, int B, double* part,
                       int32_t* cst, BCol* bc, int what) {
-  const int64_t nch = nchunks_host(n1);
   const int G = batch_group(nch);
   const int64_t ng = (nch + G - 1) / G;
-  double* part2 = part + 2 * nch * B;
+  double* part2 = part + (what == BFIN_INNER ? 3 : 1) * nch * B;
   hipLaunchKernelGGL(k_bgroup, dim3((unsigned)((ng * B + 255) / 256)), dim3(256), 0,
                      s, nch, B, G, part, part2, cst, what);
   hipLaunchKernelGGL(k_bfinalize, dim3(B), dim3(256), 0, s, nch, B, G, part2, cst, bc,

@@ -1424,7 +1424,7 @@ struct sgeo_batch {
   int64_t n1 = 0;
   double *d_x = nullptr, *d_oldx = nullptr, *d_load = nullptr, *d_body = nullptr;
   double *d_Y64 = nullptr, *d_ext = nullptr, *d_xel = nullptr;
-  float *d_r = nullptr, *d_p = nullptr, *d_kp = nullptr;
+  float *d_r = nullptr, *d_p = nullptr;
   float *d_Y32 = nullptr, *d_dinv32 = nullptr;
   double *d_evpt = nullptr, *d_stress = nullptr, *d_BL = nullptr;
   uint8_t* d_flags = nullptr;
@@ -1433,7 +1433,8 @@ struct sgeo_batch {
   BCol *d_bc = nullptr, *h_bc = nullptr;
   int32_t *d_cst = nullptr, *h_cst = nullptr;
   int32_t *d_mask = nullptr, *d_islast = nullptr, *d_cols = nullptr;
-  double* d_part = nullptr;
+  double* d_part = nullptr;  // equation-chunk partials and their groups
+  double* d_edot = nullptr;  // the elements' p.Kp shares and their groups
   // per column
   std::vector<int32_t> phase, nl_iter, cg_tot, need_start, seeded, seed_cg, err;
   std::vector<int32_t> last_it, last_rep;  // counters at the last synchronisation
@@ -1520,7 +1521,6 @@ extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extlo
   TRY(dalloc(&b->d_xel, (size_t)b->n1, acc));
   TRY(dalloc(&b->d_r, nv, acc));
   TRY(dalloc(&b->d_p, nv, acc));
-  TRY(dalloc(&b->d_kp, nv, acc));
   TRY(dalloc(&b->d_Y32, ny, acc));
   TRY(dalloc(&b->d_dinv32, (size_t)b->n1, acc));
   TRY(dalloc(&b->d_evpt, ns, acc));
@@ -1535,6 +1535,7 @@ extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extlo
   TRY(dalloc(&b->d_islast, (size_t)ncol, acc));
   TRY(dalloc(&b->d_cols, (size_t)ncol, acc));
   TRY(dalloc(&b->d_part, (size_t)batch_partials(b->n1) * ncol, acc));
+  TRY(dalloc(&b->d_edot, (size_t)batch_element_partials(m->nelmt) * ncol, acc));
   HIPCHK(hipHostMalloc((void**)&b->h_ps, sizeof(PlasticScalars) * ncol));
   HIPCHK(hipHostMalloc((void**)&b->h_bc, sizeof(BCol) * ncol));
   HIPCHK(hipHostMalloc((void**)&b->h_cst, sizeof(int32_t) * ncol));
@@ -1580,7 +1581,7 @@ extern "C" int sgeo_batch_destroy(sgeo_batch* b) {
   hipSetDevice(b->mesh->ctx->device);
   hipStreamSynchronize(b->mesh->ctx->stream);
   void* ptrs[] = {b->d_x,     b->d_oldx,   b->d_load,   b->d_body,  b->d_Y64,
-                  b->d_ext,   b->d_xel,    b->d_r,      b->d_p,     b->d_kp,
+                  b->d_ext,   b->d_xel,    b->d_r,      b->d_p,     b->d_edot,
                   b->d_Y32,   b->d_dinv32, b->d_evpt,   b->d_stress, b->d_BL,
                   b->d_flags, b->d_mp,     b->d_ps,     b->d_bc,    b->d_cst,
                   b->d_mask,  b->d_islast, b->d_cols,   b->d_part,  b->d_vm,
@@ -1651,7 +1652,7 @@ extern "C" int sgeo_batch_autotune(sgeo_batch* b, int32_t reps, int32_t* blocks_
       g.xcd_split = xs;
       auto launch = [&]() {
         launch_bmv_mfma_f32(s, m->nelmt, b->B, c->n_cu, g, m->d_KP32, m->d_gdof, b->d_p,
-                            b->d_Y32, b->d_cst, BCOL_BIT(BCOL_INNER));
+                            b->d_Y32, b->d_edot, b->d_cst, BCOL_BIT(BCOL_INNER));
       };
       launch();  // warm-up
       HIPCHK(hipEventRecord(e0, s));
@@ -1921,24 +1922,25 @@ static void batch_step_launches(sgeo_batch* b, bool prof, bool resid) {
   hipStream_t s = c->stream;
   const int B = b->B;
   const int64_t n1 = b->n1;
+  const int64_t nch = batch_equation_chunks(n1);
   if (resid) {
     launch_bmv_f64(s, m->nelmt, B, c->n_cu, m->d_KP64, m->d_gdof, b->d_x, b->d_Y64,
                    b->d_cst, BCOL_BIT(BCOL_START) | BCOL_BIT(BCOL_REPLACE));
     launch_bresid(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y64, b->d_load, b->d_dinv32,
                   b->d_r, b->d_p, b->d_part, b->d_cst);
-    launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_RESID);
+    launch_bfinalize(s, nch, B, b->d_part, b->d_cst, b->d_bc, BFIN_RESID);
   }
   launch_bpupdate(s, n1, B, b->d_r, b->d_dinv32, b->d_p, b->d_cst, b->d_bc);
   if (prof) prof_mark(c);
+  // Y = K P and the elements' p.Kp shares; alpha = r.z / p.Kp
   launch_bmv_mfma_f32(s, m->nelmt, B, c->n_cu, b->geom, m->d_KP32, m->d_gdof, b->d_p,
-                      b->d_Y32, b->d_cst, BCOL_BIT(BCOL_INNER));
+                      b->d_Y32, b->d_edot, b->d_cst, BCOL_BIT(BCOL_INNER));
   if (prof) prof_mark(c);
-  launch_bassemble_dot(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y32, b->d_p, b->d_kp,
-                       b->d_part, b->d_cst);
-  launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_ALPHA);
-  launch_bupdate(s, n1, B, b->d_p, b->d_kp, b->d_dinv32, b->d_x, b->d_r, b->d_part,
-                 b->d_cst, b->d_bc);
-  launch_bfinalize(s, n1, B, b->d_part, b->d_cst, b->d_bc, BFIN_INNER);
+  launch_bfinalize(s, m->nelmt, B, b->d_edot, b->d_cst, b->d_bc, BFIN_ALPHA);
+  // kp assembled from Y; x, r, r.z, max|x|, max|p|; the stop test
+  launch_bupdate(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y32, b->d_p, b->d_dinv32,
+                 b->d_x, b->d_r, b->d_part, b->d_cst, b->d_bc);
+  launch_bfinalize(s, nch, B, b->d_part, b->d_cst, b->d_bc, BFIN_INNER);
 }
 
 // `chunk` lockstep CG steps of every running column (FP64 residuals on

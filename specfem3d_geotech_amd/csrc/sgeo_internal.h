@@ -166,19 +166,19 @@ struct BmvGeom {
   int xcd_split = 1;
 };
 
-int64_t batch_partials(int64_t n1);
+int64_t batch_partials(int64_t n1);          // per column, equation-chunk partials
+int64_t batch_element_partials(int32_t nelmt);  // per column, element p.Kp shares
+int64_t batch_equation_chunks(int64_t n1);
 void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu, BmvGeom geom,
                          const float* KP, const int32_t* gdof, const float* P,
-                         float* Y, const int32_t* cst, int want_mask);
+                         float* Y, double* edot, const int32_t* cst, int want_mask);
 void launch_bmv_f64(hipStream_t s, int32_t nelmt, int B, int n_cu, const double* KP,
                     const int32_t* gdof, const double* X, double* Y,
                     const int32_t* cst, int want_mask);
-void launch_bassemble_dot(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
-                          const int32_t* slots, const float* Y, const float* p,
-                          float* kp, double* part, const int32_t* cst);
-void launch_bupdate(hipStream_t s, int64_t n1, int B, const float* p,
-                    const float* kp, const float* dinv, double* x, float* r,
-                    double* part, const int32_t* cst, const BCol* bc);
+void launch_bupdate(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
+                    const int32_t* slots, const float* Y, const float* p,
+                    const float* dinv, double* x, float* r, double* part,
+                    const int32_t* cst, const BCol* bc);
 void launch_bpupdate(hipStream_t s, int64_t n1, int B, const float* r,
                      const float* dinv, float* p, const int32_t* cst,
                      const BCol* bc);
@@ -186,7 +186,7 @@ void launch_bresid(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
                    const int32_t* slots, const double* Y64, const double* load,
                    const float* dinv, float* r, float* p, double* part,
                    const int32_t* cst);
-void launch_bfinalize(hipStream_t s, int64_t n1, int B, double* part,
+void launch_bfinalize(hipStream_t s, int64_t nch, int B, double* part,
                       int32_t* cst, BCol* bc, int what);
 void launch_bload(hipStream_t s, int64_t n1, int B, const double* ext,
                   const double* body, double* load, const int32_t* active);
