@@ -103,8 +103,8 @@ def parse():
                     help="sweep mode: MFMA product launch geometry 'blocks_per_cu,xcd_split' "
                          "instead of the autotuner's pick (e.g. 3,1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--residual-period", type=int, default=2,
-                    help="batched engine: FP64 residuals on every n-th CG step")
+    ap.add_argument("--residual-period", type=int, default=8,
+                    help="batched engine: FP64 residuals on every n-th CG step (1, 2, 4, 8)")
     ap.add_argument("--roofline-steps", type=int, default=64,
                     help="sweep mode: steps of the HIP-event window after the timed one")
     ap.add_argument("--cpu-sample-iters", type=int, default=8,
@@ -168,7 +168,7 @@ def roofline(kernel, bytes_per_launch, n, ms, extra=None, traffic=None):
 def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
     import numpy as np
     from specfem3d_geotech_amd import workloads
-    from specfem3d_geotech_amd.batch import BatchEngine, device_mesh
+    from specfem3d_geotech_amd.batch import DEFAULT_INNER_TOL, BatchEngine, device_mesh
     from specfem3d_geotech_amd.plastic import dt_viscoplas, strength_reduction
     from specfem3d_geotech_amd.solver import K_F64, KFMT_PACKED
     steps = a.steps or 1000
@@ -192,7 +192,7 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
     real = workloads.form_a(B, len(gam))
     srf = list(ctl["srf"])
     nxt = [0] * B  # next SRF index per column (each column runs one analysis)
-    kw = dict(cg_tol=ctl["cg_tol"], cg_maxiter=ctl["cg_maxiter"], inner_tol=1e-5,
+    kw = dict(cg_tol=ctl["cg_tol"], cg_maxiter=ctl["cg_maxiter"], inner_tol=DEFAULT_INNER_TOL,
               nl_tol=ctl["nl_tol"], nl_maxiter=ctl["nl_maxiter"])
     done_rows = []
 
@@ -287,7 +287,7 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
                     f"{ended_in_window} SRF trials ended in the window",
                     nelmt=E, neq=N, trials=B,
                     form="batched mixed precision", k_storage="packed f32 (S=3328) + "
-                    "packed f64 for the residual", cg_tol=ctl["cg_tol"], inner_tol=1e-5,
+                    "packed f64 for the residual", cg_tol=ctl["cg_tol"], inner_tol=DEFAULT_INNER_TOL,
                     residual_period=a.residual_period,
                     nl_tol=ctl["nl_tol"], nl_maxiter=ctl["nl_maxiter"],
                     batch_hbm_gb=round(nbytes / 1e9, 2), copy_kernel_gbs=round(copy_gbs, 1),
@@ -470,7 +470,7 @@ def mode_mixed(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
     relative L2 difference of the displacement, and the number of GLL
     points whose plastic flag differs."""
     import numpy as np
-    from specfem3d_geotech_amd.batch import slope3d_mixed
+    from specfem3d_geotech_amd.batch import DEFAULT_INNER_TOL, slope3d_mixed
     from specfem3d_geotech_amd.slope import slope3d
     from specfem3d_geotech_amd.solver import K_F64, KFMT_PACKED
     srf = [float(v) for v in a.srf.split(",")] if a.srf else [1.0, 1.05, 1.1, 1.15]
@@ -516,7 +516,7 @@ def mode_mixed(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
         scaling="weak", vs_baseline=None,
         dtype="f32 stiffness + inner CG, f64 refinement/dots/x/yield", data=DATA,
         config=dict(workload=wl.description, nelmt=E, neq=N, mode="mixed", srf=srf,
-                    srf_in_flight=ncol, cg_tol=ctl["cg_tol"], inner_tol=1e-5,
+                    srf_in_flight=ncol, cg_tol=ctl["cg_tol"], inner_tol=DEFAULT_INNER_TOL,
                     nl_tol=ctl["nl_tol"], nl_maxiter=ctl["nl_maxiter"],
                     ingest_s=round(prep_s, 3)),
         fp64=dict(seconds=round(t64, 2), cg_iterations=cg64,

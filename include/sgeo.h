@@ -399,10 +399,12 @@ int sgeo_batch_autotune(sgeo_batch *batch, int32_t reps, int32_t *blocks_per_cu,
 int sgeo_batch_set_geometry(sgeo_batch *batch, int32_t blocks_per_cu, int32_t xcd_split);
 /* FP64 residuals (the product on the packed FP64 operator that starts a
  * column's PCG call or replaces its residual) are formed on every period-th
- * CG step only (1..8, default 1), so one pass over the FP64 operator serves
- * the columns of several steps; a column waiting for its residual idles.
- * No result bit depends on the period, only the columns' timing.  No
- * counterpart in the reference. */
+ * CG step of the engine only (1, 2, 4 or 8; default 1), so one pass over the
+ * FP64 operator serves the columns of several steps.  A starting column
+ * waits for the next such step; a column due for a replacement keeps
+ * iterating until then.  Results depend on the period (the iterate at which
+ * a residual is replaced), never on the batch or on the other columns.
+ * Set it before starting trials.  No counterpart in the reference. */
 int sgeo_batch_set_residual_period(sgeo_batch *batch, int32_t period);
 /* Totals since creation, as of the last host synchronisation: CG
  * iterations of all trials (seeded elastic solutions not counted), FP64

@@ -33,10 +33,23 @@ from .solver import K_F64, Context, DeviceMesh, nodal_fields_call
 from .spec import SpecMesh, hex2spec_device
 
 
+# FP64 residuals on every 8th engine step: one pass over the packed FP64
+# operator then serves the starts and replacements of 8 steps (c1 with 64
+# trials: 4.93 ms per step against 5.47 ms with a residual on every step)
+DEFAULT_RESID_PERIOD = 8
+# A column is due for an FP64 residual once r.z has fallen by this factor
+# since its last one.  Replacing early keeps the FP32 recursion's drift
+# small: with residuals every 8th step, 1e-3 puts the reference's first
+# solve of the layered 8x4x6 slope 2.7e-9 from the reference's solution
+# (1.1e-8 with 1e-5), at the same CG iteration count.
+DEFAULT_INNER_TOL = 1e-3
+
+
 class BatchEngine:
     """sgeo_batch: `ncol` trial columns on one DeviceMesh."""
 
-    def __init__(self, dmesh: DeviceMesh, extload: np.ndarray, ncol: int):
+    def __init__(self, dmesh: DeviceMesh, extload: np.ndarray, ncol: int,
+                 resid_period: int = DEFAULT_RESID_PERIOD):
         self.dmesh = dmesh
         self.ncol = ncol
         self._ext = f64(extload)
@@ -44,6 +57,7 @@ class BatchEngine:
         check(L.lib().sgeo_batch_create(dmesh.h, ncol, ptr(self._ext), C.byref(h)),
               "sgeo_batch_create")
         self.h = h
+        self.set_residual_period(resid_period)
 
     def begin(self, col, ym, nuf, phif, cohf, psif, dt, x0=None, x0_cg=0):
         x0a = f64(x0) if x0 is not None else None
@@ -130,8 +144,9 @@ class BatchEngine:
               "sgeo_batch_set_geometry")
 
     def set_residual_period(self, period: int):
-        """FP64 residuals on every `period`-th CG step only (timing, not
-        results, depends on it)."""
+        """FP64 residuals on every `period`-th CG step of the engine only
+        (1, 2, 4 or 8).  A column due for a replacement iterates on until
+        then, so results depend on the period, but never on the batch."""
         check(L.lib().sgeo_batch_set_residual_period(self.h, int(period)),
               "sgeo_batch_set_residual_period")
 
@@ -203,7 +218,7 @@ class PCGNotConverged(RuntimeError):
 
 def run_analyses(engine, analyses: List[Analysis], *, ym, nu, cg_tol: float,
                  cg_maxiter: int, nl_tol: float, nl_maxiter: int,
-                 inner_tol: float = 1e-5, srf_parallel: int = 1,
+                 inner_tol: float = DEFAULT_INNER_TOL, srf_parallel: int = 1,
                  keep_state: bool = False, on_trial=None,
                  arrivals: Optional[List[float]] = None,
                  clock=time.perf_counter, on_done=None, poll_steps: int = 64,
@@ -403,9 +418,11 @@ def _trial_sink(on_srf, output):
 
 def slope3d_mixed(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
                   nl_tol: float, nl_maxiter: int, srf: Sequence[float],
-                  inner_tol: float = 1e-5, ncol: int = 1, spec=None, on_srf=None,
+                  inner_tol: float = DEFAULT_INNER_TOL, ncol: int = 1, spec=None,
+                  on_srf=None,
                   progress=None, output=None, phinu: bool = False,
-                  defer_nl: Optional[int] = None) -> SlopeResult:
+                  defer_nl: Optional[int] = None,
+                  resid_period: int = DEFAULT_RESID_PERIOD) -> SlopeResult:
     """The reference's SRF sweep (src/slope.F90:244-443) in the mixed-precision
     form: FP32 stiffness and inner CG, FP64 refinement to cg_tol.  With
     ncol > 1 up to ncol SRFs of the sweep run at once; the rows and the
@@ -415,7 +432,9 @@ def slope3d_mixed(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int
     (output.SlopeOutput) receives the rows, displacements and nodal fields
     for the reference's result files.  With `phinu`, SRFs whose Poisson's
     ratio correction changes the stiffness run the whole sweep on the FP64
-    drop-in instead (the engine's trials share one operator)."""
+    drop-in instead (the engine's trials share one operator).
+    `resid_period`: FP64 residuals on every resid_period-th engine step
+    (BatchEngine.set_residual_period)."""
     spec = spec or hex2spec_device(ctx, mesh)
     _, ym, nu, phi, coh, psi = mesh.material_arrays()
     if phinu and any(strength_reduction(s, True, coh, nu, phi, psi)[4] == 1 for s in srf):
@@ -432,6 +451,7 @@ def slope3d_mixed(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int
     eng = BatchEngine(dm, ext, ncol)
     a = Analysis(coh=coh, phi=phi, psi=psi, srf=list(srf))
     try:
+        eng.set_residual_period(resid_period)
         run_analyses(eng, [a], ym=ym, nu=nu, cg_tol=cg_tol, cg_maxiter=cg_maxiter,
                      nl_tol=nl_tol, nl_maxiter=nl_maxiter, inner_tol=inner_tol,
                      srf_parallel=ncol, keep_state=True, progress=progress,

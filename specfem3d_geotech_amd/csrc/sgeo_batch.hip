@@ -435,6 +435,18 @@ __global__ __launch_bounds__(256) void k_bupdate(
   part[(2 * nch + c.chunk) * B + c.j] = mp;
 }
 
+// At a residual step: INNER columns due for a replacement become REPLACE
+// (their last iteration left r_{k+1}, rz_prev = r_k.z_k and a pending beta,
+// which the FP64 residual's finalize recomputes).
+__global__ void k_bpromote(int B, int32_t* __restrict__ cst, BCol* __restrict__ bc) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= B) return;
+  if (cst[j] == BCOL_INNER && bc[j].want_rep) {
+    cst[j] = BCOL_REPLACE;
+    bc[j].want_rep = 0;
+  }
+}
+
 // FP64 residual for START / REPLACE columns: r64 = load - sum Y64 (element
 // order, FP64); r = (float) r64; START: p = D r; partial r.(D r)
 __global__ __launch_bounds__(256) void k_bresid(
@@ -579,13 +591,12 @@ __global__ __launch_bounds__(256) void k_bfinalize(int64_t nch, int B, int G,
         cst[j] = BCOL_FAIL;
       } else {
         c.rz_prev = c.rz;
-        if (v[1] <= c.inner_tol * c.rz0) {
-          cst[j] = BCOL_REPLACE;  // FP64 residual at the column's next step
-        } else {
-          c.beta = v[1] / c.rz;
-          c.rz = v[1];
-          c.pending = 1;
-        }
+        // due for an FP64 residual: replaced at the next residual step, by
+        // when the column may have iterated on
+        if (v[1] <= c.inner_tol * c.rz0) c.want_rep = 1;
+        c.beta = v[1] / c.rz;
+        c.rz = v[1];
+        c.pending = 1;
       }
       break;
     }
@@ -803,6 +814,10 @@ void launch_bpupdate(hipStream_t s, int64_t n1, int B, const float* r,
                      const BCol* bc) {
   hipLaunchKernelGGL(k_bpupdate, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
                      batch_chunk(n1), r, dinv, p, cst, bc);
+}
+
+void launch_bpromote(hipStream_t s, int B, int32_t* cst, BCol* bc) {
+  hipLaunchKernelGGL(k_bpromote, dim3((B + 63) / 64), dim3(64), 0, s, B, cst, bc);
 }
 
 void launch_bresid(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,

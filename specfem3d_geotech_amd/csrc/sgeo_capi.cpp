@@ -1454,9 +1454,12 @@ struct sgeo_batch {
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
   BmvGeom geom;  // launch geometry of the MFMA product
-  // FP64 residuals (starting and replacing columns) are formed on every
-  // resid_period-th step of a chunk; a column waiting for one idles
+  // FP64 residuals (starting and replacing columns) are formed on the steps
+  // gstep with gstep % resid_period == 0; gstep counts every step the
+  // engine has run, so a column's residual steps fall at fixed counts of its
+  // own iterations
   int resid_period = 1;
+  int64_t gstep = 0;
   int32_t* h_stage = nullptr;  // pinned staging of the small uploads
   unsigned stage_busy = 0;
   double step_us = 0.0;  // running mean of the wall time of one step
@@ -1604,8 +1607,8 @@ extern "C" int sgeo_batch_set_graphs(sgeo_batch* b, int enable) {
 }
 
 extern "C" int sgeo_batch_set_residual_period(sgeo_batch* b, int32_t period) {
-  if (!b || period < 1 || period > 8)
-    return fail(SGEO_EARG, "sgeo_batch_set_residual_period: period 1..8");
+  if (!b || (period != 1 && period != 2 && period != 4 && period != 8))
+    return fail(SGEO_EARG, "sgeo_batch_set_residual_period: period 1, 2, 4 or 8");
   if (period != b->resid_period) batch_drop_graphs(b);
   b->resid_period = period;
   return 0;
@@ -1913,9 +1916,9 @@ static bool pcg_running(int st) {
 }
 
 // One CG step of every column (launch sequence; no memset, no host work).
-// With `resid`, the step first forms the FP64 residual of the columns that
-// start a call or replace their residual (a no-op launch when there are
-// none); a column's arithmetic depends only on its own iterations.
+// With `resid` (a residual step), the step first forms the FP64 residual of
+// the columns that start a call or are due for a replacement (a no-op
+// launch when there are none).
 static void batch_step_launches(sgeo_batch* b, bool prof, bool resid) {
   sgeo_mesh* m = b->mesh;
   sgeo_ctx* c = m->ctx;
@@ -1924,6 +1927,7 @@ static void batch_step_launches(sgeo_batch* b, bool prof, bool resid) {
   const int64_t n1 = b->n1;
   const int64_t nch = batch_equation_chunks(n1);
   if (resid) {
+    launch_bpromote(s, B, b->d_cst, b->d_bc);
     launch_bmv_f64(s, m->nelmt, B, c->n_cu, m->d_KP64, m->d_gdof, b->d_x, b->d_Y64,
                    b->d_cst, BCOL_BIT(BCOL_START) | BCOL_BIT(BCOL_REPLACE));
     launch_bresid(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y64, b->d_load, b->d_dinv32,
@@ -1943,9 +1947,11 @@ static void batch_step_launches(sgeo_batch* b, bool prof, bool resid) {
   launch_bfinalize(s, nch, B, b->d_part, b->d_cst, b->d_bc, BFIN_INNER);
 }
 
-// `chunk` lockstep CG steps of every running column (FP64 residuals on
-// every resid_period-th of them, the first included), then reads the states
-// back; h_bc holds the device scalars on return.
+// `chunk` lockstep CG steps of every running column (FP64 residuals on the
+// engine's residual steps), then reads the states back; h_bc holds the
+// device scalars on return.  Graphs (8, 16 or 32 steps, multiples of the
+// period) are replayed only from a step aligned to the period, so their
+// residual positions are the engine's.
 static int batch_steps(sgeo_batch* b, int chunk, bool* any_running) {
   sgeo_mesh* m = b->mesh;
   sgeo_ctx* c = m->ctx;
@@ -1953,11 +1959,11 @@ static int batch_steps(sgeo_batch* b, int chunk, bool* any_running) {
   const int B = b->B;
   const auto t0 = std::chrono::steady_clock::now();
   const int gi = chunk == 8 ? 0 : chunk == 16 ? 1 : chunk == 32 ? 2 : -1;
-  if (b->use_graphs && !c->prof && gi >= 0) {
+  const int per = b->resid_period;
+  if (b->use_graphs && !c->prof && gi >= 0 && b->gstep % per == 0) {
     if (!b->gexec[gi]) {
       HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      for (int it = 0; it < chunk; it++)
-        batch_step_launches(b, false, it % b->resid_period == 0);
+      for (int it = 0; it < chunk; it++) batch_step_launches(b, false, it % per == 0);
       hipGraph_t g = nullptr;
       const hipError_t ec = hipStreamEndCapture(s, &g);
       if (ec != hipSuccess || !g) return fail(SGEO_EHIP, "graph capture failed");
@@ -1967,8 +1973,9 @@ static int batch_steps(sgeo_batch* b, int chunk, bool* any_running) {
     HIPCHK(hipGraphLaunch(b->gexec[gi], s));
   } else {
     for (int it = 0; it < chunk; it++)
-      batch_step_launches(b, c->prof, it % b->resid_period == 0);
+      batch_step_launches(b, c->prof, (b->gstep + it) % per == 0);
   }
+  b->gstep += chunk;
   b->stat_steps += chunk;
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(b->h_cst, b->d_cst, B * 4, hipMemcpyDeviceToHost, s));

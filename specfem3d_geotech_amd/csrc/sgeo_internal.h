@@ -118,16 +118,22 @@ void launch_nodal_acc_out(hipStream_t s, int32_t nnode, const int32_t* rowptr,
 // ---- multi-trial mixed-precision engine (sgeo_batch.hip) -----------------
 // Per-column states of the mixed-precision PCG.  The inner iteration runs in
 // FP32 (operator, r, p, kp) with FP64 dot products and an FP64 solution x;
-// the FP64 true residual f - K x (FP64 operator) starts each solve and, at
-// the column's next step, replaces the recursive FP32 residual once r.z has
-// dropped by inner_tol since the last replacement, keeping the search
-// direction.  Everything a column computes depends only on its own
-// iteration count, never on when the host synchronises.
+// the FP64 true residual f - K x (FP64 operator) starts each solve and
+// replaces the recursive FP32 residual, keeping the search direction, once
+// r.z has dropped by inner_tol since the last replacement.  FP64 residuals
+// are formed on the residual steps only: every period-th step of the
+// engine's own step count.  A starting column waits for the next residual
+// step; a column due for a replacement keeps iterating and is replaced at
+// the next one.  Since a column starts on a residual step and then advances
+// one iteration per step, its residual steps fall at its own iteration
+// counts 0, period, 2 period, ...: everything a column computes depends
+// only on its own iterations and the period, never on the other columns or
+// on when the host synchronises.
 enum BColState {
   BCOL_OFF = 0,      // column not in this solve
   BCOL_START = 1,    // next: FP64 residual, p = D r
   BCOL_INNER = 2,    // FP32 CG iteration
-  BCOL_REPLACE = 3,  // next: FP64 residual replaces r, p = D r + beta p
+  BCOL_REPLACE = 3,  // now: FP64 residual replaces r, p = D r + beta p
   BCOL_DONE = 4,     // stop test passed
   BCOL_FAIL = 5,     // maxiter reached
 };
@@ -143,6 +149,7 @@ struct BCol {  // per-column scalars, device resident
   int32_t it, maxiter;  // CG iterations of this solve, limit
   int32_t pending;      // 1: p = D r + beta p before the next product
   int32_t nrep;         // FP64 residuals of this solve
+  int32_t want_rep;     // 1: replace r at the next residual step
 };
 
 // GLL numbering of a hex8 mesh, built on the device (sgeo_ingest.hip)
@@ -182,6 +189,7 @@ void launch_bupdate(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
 void launch_bpupdate(hipStream_t s, int64_t n1, int B, const float* r,
                      const float* dinv, float* p, const int32_t* cst,
                      const BCol* bc);
+void launch_bpromote(hipStream_t s, int B, int32_t* cst, BCol* bc);
 void launch_bresid(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
                    const int32_t* slots, const double* Y64, const double* load,
                    const float* dinv, float* r, float* p, double* part,

@@ -20,6 +20,7 @@ from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
+from .batch import DEFAULT_INNER_TOL
 from .mesh import HexMesh
 from .plastic import dt_viscoplas, strength_reduction
 from .slope import DeviceSlope, PCGNotConverged
@@ -130,7 +131,8 @@ def excavation3d(ctx: Context, mesh: HexMesh, *, cg_tol: float, cg_maxiter: int,
                  kfmt: int = KFMT_FULL, spec: Optional[SpecMesh] = None,
                  progress=None, output=None, z0: float = 0.0, s0: float = 0.0,
                  k0: float = 0.0, form: str = "fp64",
-                 inner_tol: float = 1e-5, iter_progress=None) -> ExcavationResult:
+                 inner_tol: float = DEFAULT_INNER_TOL,
+                 iter_progress=None) -> ExcavationResult:
     """Staged excavation on the GPU for one SRF.  The reference runs
     excavation with nsrf = 1 only (it reallocates and closes per-SRF state
     inside its SRF loop); the first SRF is used.  `output`
