@@ -105,6 +105,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--residual-period", type=int, default=8,
                     help="batched engine: FP64 residuals on every n-th CG step (1, 2, 4, 8)")
+    ap.add_argument("--inner-tol", type=float, default=None,
+                    help="batched engine: FP64 residual once r.z fell by this factor "
+                    "(default: the engine's)")
     ap.add_argument("--roofline-steps", type=int, default=64,
                     help="sweep mode: steps of the HIP-event window after the timed one")
     ap.add_argument("--cpu-sample-iters", type=int, default=8,
@@ -192,7 +195,8 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
     real = workloads.form_a(B, len(gam))
     srf = list(ctl["srf"])
     nxt = [0] * B  # next SRF index per column (each column runs one analysis)
-    kw = dict(cg_tol=ctl["cg_tol"], cg_maxiter=ctl["cg_maxiter"], inner_tol=DEFAULT_INNER_TOL,
+    itol = a.inner_tol if a.inner_tol is not None else DEFAULT_INNER_TOL
+    kw = dict(cg_tol=ctl["cg_tol"], cg_maxiter=ctl["cg_maxiter"], inner_tol=itol,
               nl_tol=ctl["nl_tol"], nl_maxiter=ctl["nl_maxiter"])
     done_rows = []
 
@@ -287,7 +291,7 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
                     f"{ended_in_window} SRF trials ended in the window",
                     nelmt=E, neq=N, trials=B,
                     form="batched mixed precision", k_storage="packed f32 (S=3328) + "
-                    "packed f64 for the residual", cg_tol=ctl["cg_tol"], inner_tol=DEFAULT_INNER_TOL,
+                    "packed f64 for the residual", cg_tol=ctl["cg_tol"], inner_tol=itol,
                     residual_period=a.residual_period,
                     nl_tol=ctl["nl_tol"], nl_maxiter=ctl["nl_maxiter"],
                     batch_hbm_gb=round(nbytes / 1e9, 2), copy_kernel_gbs=round(copy_gbs, 1),

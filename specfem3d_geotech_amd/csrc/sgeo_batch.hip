@@ -266,53 +266,124 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
 }
 
 // --------------------------------------------------------------------------
-// FP64 batched element product (the true residual's K x): one element per
-// block; the packed upper triangle of K_e and the gathered X_e (81 x B) in
-// LDS; thread t owns column j = t % B and rows t / B + k * (nthreads / B),
-// each summed over c = 0..80 in order.  blockIdx.y picks the column chunk
-// col0 = blockIdx.y * B .. col0+B-1 of arrays with column stride Bs; columns
-// not in want_mask read zero, write nothing.
+// FP64 batched element product for the true residual f - K x, on the FP64
+// MFMA (v_mfma_f64_16x16x4f64), for the columns of the residual list only:
+//   Y64[e][r][q] = sum_c K_e(r,c) X[g(e,c)][rlist[q]],  q < nres = rlist[64],
+// with the list's columns packed into tiles of 16 (q = 16 n + lane % 16), so
+// the work scales with the columns that need a residual, not with B.  The
+// packed upper triangle of K_e (FP64) is unpacked into LDS as
+// Kf64[c * B64_LDK + r]; rows 0..79 are 5 row tiles of 16, k runs in 21
+// steps of 4 (81 -> 84, zero padded), row 80 is a dot product on the vector
+// ALUs as in the FP32 product.  The list length is read on the device, so
+// one captured launch serves any list.  Operand layouts: A (16x4) lane l
+// holds A(l % 16, l / 16); B (4x16) lane l holds B(l / 16, l % 16); D (16x16)
+// lane l, register v holds D(4 v + l / 16, l % 16) -- rows interleaved,
+// unlike the FP32 instruction's 4 (l / 16) + v.  A column's sums do not
+// depend on its place in the list.
+// Persistent and pipelined like the FP32 product: K and gdof of the block's
+// next element are loaded during the MFMAs.
 // --------------------------------------------------------------------------
-// Persistent: a few blocks per CU walk the elements, so a launch in which no
-// column is wanted (the common case: the kernel is launched every step and
-// only starting or replacing columns need it) costs a handful of blocks.
-__global__ __launch_bounds__(256) void k_bmv64(int32_t nelmt, int B, int Bs,
+#define B64_LDK 88
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// Wave w multiplies list tile w (list places 16 w .. 16 w + 15) over all
+// row tiles; waves past the list idle.
+__device__ __forceinline__ void bmv64_element(
+    int64_t e, int nres, int B, const double* __restrict__ X, double* __restrict__ Y,
+    const int32_t* __restrict__ rlist, const double* Kf, const int32_t* sgc, int w, int l) {
+  constexpr int NMT = 5;
+  if (w * 16 >= nres) return;  // wave-uniform: no column in this tile
+  const int q = w * 16 + (l & 15);  // list place of this lane's column
+  const bool qv = q < nres;
+  const int jc = rlist[qv ? q : 0];
+  const int kq = l >> 4;
+  double bv[21];
+#pragma unroll
+  for (int s = 0; s < 21; s++) bv[s] = X[(uint32_t)(sgc[4 * s + kq] * B + jc)];
+  f64x4 acc[NMT];
+#pragma unroll
+  for (int i = 0; i < NMT; i++) acc[i] = f64x4{0.0, 0.0, 0.0, 0.0};
+  double r80 = 0.0;
+#pragma unroll
+  for (int s = 0; s < 21; s++) {
+    const double* kc = Kf + (4 * s + kq) * B64_LDK + (l & 15);
+#pragma unroll
+    for (int i = 0; i < NMT; i++)
+      acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(kc[i * 16], bv[s], acc[i], 0, 0, 0);
+    r80 = r80 + Kf[(4 * s + kq) * B64_LDK + 80] * bv[s];
+  }
+  // the four k-groups of the column: lanes p, p+16, p+32, p+48
+  r80 = r80 + __shfl_xor(r80, 16);
+  r80 = r80 + __shfl_xor(r80, 32);
+  if (!qv) return;
+#pragma unroll
+  for (int i = 0; i < NMT; i++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) Y[(e * 81 + i * 16 + 4 * v + kq) * B + q] = acc[i][v];
+  if (kq == 0) Y[(e * 81 + 80) * B + q] = r80;
+}
+
+__global__ __launch_bounds__(256) void k_bmv64(int32_t nelmt, int B,
                                                const double* __restrict__ KP,
                                                const int32_t* __restrict__ gdof,
                                                const double* __restrict__ X,
                                                double* __restrict__ Y,
-                                               const int32_t* __restrict__ cst,
-                                               int want_mask) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  double* u = reinterpret_cast<double*>(smem);
-  double* pe = u + SGEO_KPACK;
-  const int t = threadIdx.x, nt = blockDim.x;
-  const int col0 = blockIdx.y * B;  // this block's column chunk
-  bool anyw = false;
-  for (int q = 0; q < B; q++) anyw = anyw || wanted(cst, col0 + q, want_mask);
-  if (!anyw) return;  // uniform per block
-  const int j = t % B;
-  const int nrg = nt / B, rg = t / B;
-  const bool mine = wanted(cst, col0 + j, want_mask);
-  for (int64_t e = blockIdx.x; e < nelmt; e += gridDim.x) {
+                                               const int32_t* __restrict__ rlist) {
+  constexpr int KPT = (3321 + 255) / 256;
+  __shared__ double Kf[BMV_KC * B64_LDK];
+  __shared__ int32_t sg[2][BMV_KC];
+  const int nres = __builtin_amdgcn_readfirstlane(rlist[64]);
+  if (nres == 0) return;  // the common case: no column needs a residual
+  const int t = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int l = t & 63;
+  for (int q = t; q < 3 * 81; q += 256) Kf[(81 + q / 81) * B64_LDK + q % 81] = 0.0;
+  if (t >= 81 && t < BMV_KC) { sg[0][t] = 0; sg[1][t] = 0; }
+  uint32_t kaddr[KPT];
+#pragma unroll
+  for (int i = 0; i < KPT; i++) {
+    const int q = t + 256 * i;
+    const int c = q < 3321 ? tri_col(q) : 0;
+    const int r = q < 3321 ? q - c * (c + 1) / 2 : 0;
+    kaddr[i] = (uint32_t)(c * B64_LDK + r) | ((uint32_t)(r * B64_LDK + c) << 16);
+  }
+  int64_t e = blockIdx.x;
+  double kreg[KPT];
+  int32_t greg = 0;
+  if (e < nelmt) {
     const double* src = KP + e * SGEO_KPACK;
-    for (int q = t; q < SGEO_KPACK; q += nt) u[q] = src[q];
-    for (int q = t; q < 81 * B; q += nt) {
-      const int c = q / B, jj = q - c * B;
-      pe[q] = wanted(cst, col0 + jj, want_mask)
-                  ? X[(int64_t)gdof[e * 81 + c] * Bs + col0 + jj] : 0.0;
+#pragma unroll
+    for (int i = 0; i < KPT; i++) {
+      const int q = t + 256 * i;
+      kreg[i] = q < 3321 ? src[q] : 0.0;
     }
-    __syncthreads();
-    if (mine) {
-      for (int r = rg; r < 81; r += nrg) {
-        const int rb = r * (r + 1) / 2;
-        double a = 0.0;
-        for (int c = 0; c < r; c++) a = a + u[rb + c] * pe[c * B + j];
-        for (int c = r; c < 81; c++) a = a + u[c * (c + 1) / 2 + r] * pe[c * B + j];
-        Y[(e * 81 + r) * Bs + col0 + j] = a;
+    if (t < 81) greg = gdof[e * 81 + t];
+  }
+  if (t < 81) sg[0][t] = greg;
+  int cur = 0;
+  for (; e < nelmt; e += gridDim.x) {
+#pragma unroll
+    for (int i = 0; i < KPT; i++) {
+      if (t + 256 * i < 3321) {
+        Kf[kaddr[i] & 0xffffu] = kreg[i];
+        Kf[kaddr[i] >> 16] = kreg[i];
       }
     }
-    __syncthreads();  // LDS reused by the next element
+    __syncthreads();  // K_e and its gdof visible
+    const int64_t en = e + gridDim.x;
+    if (en < nelmt) {  // next element's K and gdof, in flight during the MFMAs
+      const double* src = KP + en * SGEO_KPACK;
+#pragma unroll
+      for (int i = 0; i < KPT; i++) {
+        const int q = t + 256 * i;
+        kreg[i] = q < 3321 ? src[q] : 0.0;
+      }
+      if (t < 81) greg = gdof[en * 81 + t];
+    }
+    bmv64_element(e, nres, B, X, Y, rlist, Kf, sg[cur], w, l);
+    if (t < 81) sg[cur ^ 1][t] = greg;
+    __syncthreads();  // Kf free for the next element, its gdof visible
+    cur ^= 1;
   }
 }
 
@@ -437,35 +508,55 @@ __global__ __launch_bounds__(256) void k_bupdate(
 
 // At a residual step: INNER columns due for a replacement become REPLACE
 // (their last iteration left r_{k+1}, rz_prev = r_k.z_k and a pending beta,
-// which the FP64 residual's finalize recomputes).
-__global__ void k_bpromote(int B, int32_t* __restrict__ cst, BCol* __restrict__ bc) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= B) return;
-  if (cst[j] == BCOL_INNER && bc[j].want_rep) {
-    cst[j] = BCOL_REPLACE;
-    bc[j].want_rep = 0;
+// which the FP64 residual's finalize recomputes), and the columns that need
+// an FP64 residual now (START, REPLACE) form the residual list, in column
+// order: rlist[0..nres-1], rlist[64] = nres, rslot[j] = column j's place.
+// One wave (B <= 64).
+__global__ __launch_bounds__(64) void k_bpromote(int B, int32_t* __restrict__ cst,
+                                                 BCol* __restrict__ bc,
+                                                 int32_t* __restrict__ rlist,
+                                                 int32_t* __restrict__ rslot) {
+  const int j = threadIdx.x;
+  bool want = false;
+  if (j < B) {
+    int st = cst[j];
+    if (st == BCOL_INNER && bc[j].want_rep) {
+      st = BCOL_REPLACE;
+      cst[j] = st;
+      bc[j].want_rep = 0;
+    }
+    want = st == BCOL_START || st == BCOL_REPLACE;
   }
+  const unsigned long long m = __ballot(want);
+  const int slot = __popcll(m & ((1ull << j) - 1ull));
+  if (want) {
+    rlist[slot] = j;
+    rslot[j] = slot;
+  }
+  if (j == 0) rlist[64] = __popcll(m);
 }
 
 // FP64 residual for START / REPLACE columns: r64 = load - sum Y64 (element
-// order, FP64); r = (float) r64; START: p = D r; partial r.(D r)
+// order, FP64; Y64 holds the residual list's columns at their list places);
+// r = (float) r64; START: p = D r; partial r.(D r)
 __global__ __launch_bounds__(256) void k_bresid(
     int64_t n1, int B, int ck, const int32_t* __restrict__ rowptr,
     const int32_t* __restrict__ slots, const double* __restrict__ Y64,
     const double* __restrict__ load, const float* __restrict__ dinv,
     float* __restrict__ r, float* __restrict__ p, double* __restrict__ part,
-    const int32_t* __restrict__ cst) {
+    const int32_t* __restrict__ cst, const int32_t* __restrict__ rslot) {
   const ChunkIdx c = chunk_of(n1, B, ck);
   if (!c.valid) return;
   const int st = cst[c.j];
   if (st != BCOL_START && st != BCOL_REPLACE) return;
+  const int qs = rslot[c.j];
   double s = 0.0;
   for (int64_t g = c.g0; g < c.g1; g++) {
     const int64_t q = g * B + c.j;
     double kx = 0.0;
     if (g > 0)
       for (int32_t w = rowptr[g]; w < rowptr[g + 1]; w++)
-        kx = kx + Y64[(int64_t)slots[w] * B + c.j];
+        kx = kx + Y64[(int64_t)slots[w] * B + qs];
     const double r64 = (g == 0) ? 0.0 : load[q] - kx;
     const float rg = (float)r64;
     const float zg = dinv[g] * rg;
@@ -782,18 +873,16 @@ void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu, BmvGeom 
                        Y, edot, cst, want_mask);
 }
 
-// FP64: columns in chunks of at most 32 (LDS <= 48 KB per block), 3 blocks
-// per CU
+// FP64 residual product: persistent, 2 blocks per CU (60 KB of LDS each);
+// every block returns at once when the residual list is empty
 void launch_bmv_f64(hipStream_t s, int32_t nelmt, int B, int n_cu, const double* KP,
                     const int32_t* gdof, const double* X, double* Y,
-                    const int32_t* cst, int want_mask) {
-  const int Bc = B < 32 ? B : 32;
-  const size_t lds = SGEO_KPACK * sizeof(double) + (size_t)81 * Bc * sizeof(double);
-  int64_t g = (int64_t)n_cu * 3;
+                    const int32_t* rlist) {
+  int64_t g = (int64_t)n_cu * 2;
   if (g > nelmt) g = nelmt;
   if (g < 1) g = 1;
-  hipLaunchKernelGGL(k_bmv64, dim3((unsigned)g, (unsigned)(B / Bc)), dim3(bmv_threads(Bc)),
-                     lds, s, nelmt, Bc, B, KP, gdof, X, Y, cst, want_mask);
+  hipLaunchKernelGGL(k_bmv64, dim3((unsigned)g), dim3(256), 0, s, nelmt, B, KP, gdof, X,
+                     Y, rlist);
 }
 
 void launch_bupdate(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
@@ -816,16 +905,18 @@ void launch_bpupdate(hipStream_t s, int64_t n1, int B, const float* r,
                      batch_chunk(n1), r, dinv, p, cst, bc);
 }
 
-void launch_bpromote(hipStream_t s, int B, int32_t* cst, BCol* bc) {
-  hipLaunchKernelGGL(k_bpromote, dim3((B + 63) / 64), dim3(64), 0, s, B, cst, bc);
+void launch_bpromote(hipStream_t s, int B, int32_t* cst, BCol* bc, int32_t* rlist,
+                     int32_t* rslot) {
+  hipLaunchKernelGGL(k_bpromote, dim3(1), dim3(64), 0, s, B, cst, bc, rlist, rslot);
 }
 
 void launch_bresid(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
                    const int32_t* slots, const double* Y64, const double* load,
                    const float* dinv, float* r, float* p, double* part,
-                   const int32_t* cst) {
+                   const int32_t* cst, const int32_t* rslot) {
   hipLaunchKernelGGL(k_bresid, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
-                     batch_chunk(n1), rowptr, slots, Y64, load, dinv, r, p, part, cst);
+                     batch_chunk(n1), rowptr, slots, Y64, load, dinv, r, p, part, cst,
+                     rslot);
 }
 
 // `nch` partials per column and kind in `part` (equation chunks, or elements

@@ -1433,6 +1433,9 @@ struct sgeo_batch {
   BCol *d_bc = nullptr, *h_bc = nullptr;
   int32_t *d_cst = nullptr, *h_cst = nullptr;
   int32_t *d_mask = nullptr, *d_islast = nullptr, *d_cols = nullptr;
+  // residual list of a residual step: columns [0..64), count at [64]; and
+  // each listed column's place in it
+  int32_t *d_rlist = nullptr, *d_rslot = nullptr;
   double* d_part = nullptr;  // equation-chunk partials and their groups
   double* d_edot = nullptr;  // the elements' p.Kp shares and their groups
   // per column
@@ -1537,6 +1540,8 @@ extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extlo
   TRY(dalloc(&b->d_mask, (size_t)ncol, acc));
   TRY(dalloc(&b->d_islast, (size_t)ncol, acc));
   TRY(dalloc(&b->d_cols, (size_t)ncol, acc));
+  TRY(dalloc(&b->d_rlist, (size_t)65, acc));
+  TRY(dalloc(&b->d_rslot, (size_t)64, acc));
   TRY(dalloc(&b->d_part, (size_t)batch_partials(b->n1) * ncol, acc));
   TRY(dalloc(&b->d_edot, (size_t)batch_element_partials(m->nelmt) * ncol, acc));
   HIPCHK(hipHostMalloc((void**)&b->h_ps, sizeof(PlasticScalars) * ncol));
@@ -1561,6 +1566,8 @@ extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extlo
   TRY(batch_sync_operator(b));
   HIPCHK(hipMemsetAsync(b->d_mask, 0, ncol * 4, s));
   HIPCHK(hipMemsetAsync(b->d_cst, 0, ncol * 4, s));
+  HIPCHK(hipMemsetAsync(b->d_rlist, 0, 65 * 4, s));
+  HIPCHK(hipMemsetAsync(b->d_rslot, 0, 64 * 4, s));
   HIPCHK(hipMemsetAsync(b->d_bc, 0, ncol * sizeof(BCol), s));
   HIPCHK(hipMemsetAsync(b->d_ps, 0, ncol * sizeof(PlasticScalars), s));
   HIPCHK(hipMemsetAsync(b->d_x, 0, nv * 8, s));
@@ -1588,7 +1595,7 @@ extern "C" int sgeo_batch_destroy(sgeo_batch* b) {
                   b->d_Y32,   b->d_dinv32, b->d_evpt,   b->d_stress, b->d_BL,
                   b->d_flags, b->d_mp,     b->d_ps,     b->d_bc,    b->d_cst,
                   b->d_mask,  b->d_islast, b->d_cols,   b->d_part,  b->d_vm,
-                  b->d_sf,    b->d_nacc};
+                  b->d_sf,    b->d_nacc,   b->d_rlist,  b->d_rslot};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (b->h_ps) hipHostFree(b->h_ps);
@@ -1927,11 +1934,11 @@ static void batch_step_launches(sgeo_batch* b, bool prof, bool resid) {
   const int64_t n1 = b->n1;
   const int64_t nch = batch_equation_chunks(n1);
   if (resid) {
-    launch_bpromote(s, B, b->d_cst, b->d_bc);
+    launch_bpromote(s, B, b->d_cst, b->d_bc, b->d_rlist, b->d_rslot);
     launch_bmv_f64(s, m->nelmt, B, c->n_cu, m->d_KP64, m->d_gdof, b->d_x, b->d_Y64,
-                   b->d_cst, BCOL_BIT(BCOL_START) | BCOL_BIT(BCOL_REPLACE));
+                   b->d_rlist);
     launch_bresid(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y64, b->d_load, b->d_dinv32,
-                  b->d_r, b->d_p, b->d_part, b->d_cst);
+                  b->d_r, b->d_p, b->d_part, b->d_cst, b->d_rslot);
     launch_bfinalize(s, nch, B, b->d_part, b->d_cst, b->d_bc, BFIN_RESID);
   }
   launch_bpupdate(s, n1, B, b->d_r, b->d_dinv32, b->d_p, b->d_cst, b->d_bc);

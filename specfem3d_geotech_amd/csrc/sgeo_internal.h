@@ -181,7 +181,7 @@ void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu, BmvGeom 
                          float* Y, double* edot, const int32_t* cst, int want_mask);
 void launch_bmv_f64(hipStream_t s, int32_t nelmt, int B, int n_cu, const double* KP,
                     const int32_t* gdof, const double* X, double* Y,
-                    const int32_t* cst, int want_mask);
+                    const int32_t* rlist);
 void launch_bupdate(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
                     const int32_t* slots, const float* Y, const float* p,
                     const float* dinv, double* x, float* r, double* part,
@@ -189,11 +189,12 @@ void launch_bupdate(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
 void launch_bpupdate(hipStream_t s, int64_t n1, int B, const float* r,
                      const float* dinv, float* p, const int32_t* cst,
                      const BCol* bc);
-void launch_bpromote(hipStream_t s, int B, int32_t* cst, BCol* bc);
+void launch_bpromote(hipStream_t s, int B, int32_t* cst, BCol* bc, int32_t* rlist,
+                     int32_t* rslot);
 void launch_bresid(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
                    const int32_t* slots, const double* Y64, const double* load,
                    const float* dinv, float* r, float* p, double* part,
-                   const int32_t* cst);
+                   const int32_t* cst, const int32_t* rslot);
 void launch_bfinalize(hipStream_t s, int64_t nch, int B, double* part,
                       int32_t* cst, BCol* bc, int what);
 void launch_bload(hipStream_t s, int64_t n1, int B, const double* ext,
