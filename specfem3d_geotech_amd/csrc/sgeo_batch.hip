@@ -102,9 +102,11 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
                                                   const float* __restrict__ KP,
                                                   const int32_t* __restrict__ gdof,
                                                   const float* __restrict__ P,
+                                                  int64_t nv,
                                                   float* __restrict__ Y,
                                                   double* __restrict__ edot,
                                                   const int32_t* __restrict__ cst,
+                                                  const BCol* __restrict__ bc,
                                                   int want_mask) {
   constexpr int NS = 4 / NT;             // waves per column tile
   constexpr int NMT = 5;                 // MFMA row tiles: rows 0..79
@@ -132,6 +134,8 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
   const bool jv = j < B && wanted(cst, j, want_mask);
   const bool wave_on = __any(jv);
   const int jl = j < B ? j : B - 1;  // lanes past B read column B-1, store nothing
+  // the column's search direction of the iteration being computed
+  const uint32_t poff = ((bc[jl].it + 1) & 1) ? (uint32_t)nv : 0u;
   const int kq = l >> 4;
   const bool row80 = mpart == NS - 1;  // wave-uniform
   // zero padding, once: k columns 81..83 (rows 0..80)
@@ -165,7 +169,7 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
   float bv[21];
   if (wave_on && e < xe1) {
 #pragma unroll
-    for (int s = 0; s < 21; s++) bv[s] = P[(uint32_t)(sg[0][4 * s + kq] * B + jl)];
+    for (int s = 0; s < 21; s++) bv[s] = P[(uint32_t)(sg[0][4 * s + kq] * B + jl) + poff];
   }
   int cur = 0;
   for (; e < xe1; e += nbx) {
@@ -260,7 +264,8 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
     }
     if (wave_on && en < xe1) {
 #pragma unroll
-      for (int s = 0; s < 21; s++) bv[s] = P[(uint32_t)(sg[cur][4 * s + kq] * B + jl)];
+      for (int s = 0; s < 21; s++)
+        bv[s] = P[(uint32_t)(sg[cur][4 * s + kq] * B + jl) + poff];
     }
   }
 }
@@ -417,7 +422,8 @@ __device__ __forceinline__ int64_t nchunks(int64_t n1, int ck) {
   return (n1 + ck - 1) / ck;
 }
 
-// p = D r + beta p for INNER columns with a pending beta
+// p_i = D r + beta p_{i-1} for INNER columns with a pending beta (i the
+// iteration about to run; p_i in buffer i & 1 of the two n1 x B buffers)
 __global__ __launch_bounds__(256) void k_bpupdate(int64_t n1, int B, int ck,
                                                   const float* __restrict__ r,
                                                   const float* __restrict__ dinv,
@@ -427,21 +433,55 @@ __global__ __launch_bounds__(256) void k_bpupdate(int64_t n1, int B, int ck,
   const ChunkIdx c = chunk_of(n1, B, ck);
   if (!c.valid || cst[c.j] != BCOL_INNER || !bc[c.j].pending) return;
   const float beta = (float)bc[c.j].beta;
+  const int i = bc[c.j].it + 1;
+  const int64_t nv = n1 * B;
+  const float* __restrict__ pold = p + ((i - 1) & 1) * nv;
+  float* __restrict__ pnew = p + (i & 1) * nv;
   for (int64_t g = c.g0; g < c.g1; g++) {
     const int64_t q = g * B + c.j;
-    p[q] = dinv[g] * r[q] + beta * p[q];
+    pnew[q] = dinv[g] * r[q] + beta * pold[q];
   }
+}
+
+// The deferred x update of a column whose solve ended on an odd iteration
+// (DONE or FAIL with xlag): x += alpha p_it, then xlag is cleared by
+// k_bxlag_clear once every chunk is done.
+__global__ __launch_bounds__(256) void k_bmaterialize(int64_t n1, int B, int ck,
+                                                      double* __restrict__ x,
+                                                      const float* __restrict__ p,
+                                                      const int32_t* __restrict__ cst,
+                                                      const BCol* __restrict__ bc) {
+  const ChunkIdx c = chunk_of(n1, B, ck);
+  if (!c.valid) return;
+  const int st = cst[c.j];
+  if ((st != BCOL_DONE && st != BCOL_FAIL) || !bc[c.j].xlag) return;
+  const double alpha = bc[c.j].alpha;
+  const float* __restrict__ pi = p + (bc[c.j].it & 1) * (n1 * B);
+  for (int64_t g = c.g0; g < c.g1; g++) {
+    const int64_t q = g * B + c.j;
+    x[q] = (g == 0) ? 0.0 : x[q] + alpha * (double)pi[q];
+  }
+}
+
+__global__ void k_bxlag_clear(int B, const int32_t* __restrict__ cst,
+                              BCol* __restrict__ bc) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= B) return;
+  if (cst[j] == BCOL_DONE || cst[j] == BCOL_FAIL) bc[j].xlag = 0;
 }
 
 // The update of an iteration, with kp assembled on the fly:
 //   kp[g][j] = sum over the slots of Y (element order, FP32),
-//   x += alpha p (FP64), r -= alpha kp, z = D r,
-// and the partials max|x| (after the update, x(0) = 0), r.z (FP64) and
-// max|p| for INNER columns.  The (at most 8 on hexahedral meshes) element
+//   x += alpha p (FP64; see BCol.xdefer), r -= alpha kp, z = D r,
+// and the partials max|x| (after the update, x(0) = 0; when x is written),
+// r.z (FP64) and max|p| for INNER columns.  The (at most 8 on hexahedral meshes) element
 // contributions of an equation are loaded together, then summed in slot
-// order; two equations per trip keep twice the loads in flight.
+// order; UPD_EQ equations per trip keep that many gathers in flight.
 // WAVE_CHUNK: B = 64, so a wave's 64 lanes are the 64 columns of one chunk
 // and the index loads (rowptr, slots) are wave-uniform scalar loads.
+#ifndef UPD_EQ
+#define UPD_EQ 4
+#endif
 template <bool WAVE_CHUNK>
 __global__ __launch_bounds__(256) void k_bupdate(
     int64_t n1, int B, int ck, const int32_t* __restrict__ rowptr,
@@ -458,8 +498,17 @@ __global__ __launch_bounds__(256) void k_bupdate(
     c.valid = c.g0 < n1;
   }
   if (!c.valid || cst[c.j] != BCOL_INNER) return;
-  const double alpha = bc[c.j].alpha;
+  const BCol& bj = bc[c.j];
+  const int i = bj.it + 1;  // the iteration this step computes
+  // x is written on every iteration, or with xdefer on even ones only,
+  // then with the deferred update of iteration i - 1 first
+  const bool xw = !bj.xdefer || !(i & 1);
+  const bool two = bj.xdefer && !(i & 1);
+  const double alpha = bj.alpha, alpha_p = bj.alpha_prev;
   const float alpha32 = (float)alpha;
+  const int64_t nv = n1 * B;
+  const float* __restrict__ pi = p + (i & 1) * nv;
+  const float* __restrict__ pp = p + ((i - 1) & 1) * nv;
   double mx = 0.0, s = 0.0, mp = 0.0;
   auto gather = [&](int64_t g, float* v, int32_t& q0, int32_t& q1) {
     q0 = g > 0 ? rowptr[g] : 0;
@@ -475,10 +524,17 @@ __global__ __launch_bounds__(256) void k_bupdate(
       if (q0 + k < q1) kp = kp + v[k];
     for (int32_t q = q0 + 8; q < q1; q++) kp = kp + Y[(int64_t)slots[q] * B + c.j];
     const int64_t q = g * B + c.j;
-    const float pg = p[q];
-    const double xg = (g == 0) ? 0.0 : x[q] + alpha * (double)pg;
-    x[q] = xg;
-    mx = fmax(mx, fabs(xg));
+    const float pg = pi[q];
+    if (xw) {
+      double xg = 0.0;
+      if (g != 0) {
+        xg = x[q];
+        if (two) xg = xg + alpha_p * (double)pp[q];
+        xg = xg + alpha * (double)pg;
+      }
+      x[q] = xg;
+      mx = fmax(mx, fabs(xg));
+    }
     mp = fmax(mp, fabs((double)pg));
     const float rg = r[q] - alpha32 * kp;
     r[q] = rg;
@@ -486,15 +542,15 @@ __global__ __launch_bounds__(256) void k_bupdate(
     s = s + (double)rg * (double)zg;
   };
   int64_t g = c.g0;
-  for (; g + 1 < c.g1; g += 2) {
-    float va[8], vb[8];
-    int32_t a0, a1, b0, b1;
-    gather(g, va, a0, a1);
-    gather(g + 1, vb, b0, b1);
-    finish(g, va, a0, a1);
-    finish(g + 1, vb, b0, b1);
+  for (; g + UPD_EQ <= c.g1; g += UPD_EQ) {
+    float v[UPD_EQ][8];
+    int32_t q0[UPD_EQ], q1[UPD_EQ];
+#pragma unroll
+    for (int k = 0; k < UPD_EQ; k++) gather(g + k, v[k], q0[k], q1[k]);
+#pragma unroll
+    for (int k = 0; k < UPD_EQ; k++) finish(g + k, v[k], q0[k], q1[k]);
   }
-  if (g < c.g1) {
+  for (; g < c.g1; g++) {
     float va[8];
     int32_t a0, a1;
     gather(g, va, a0, a1);
@@ -538,19 +594,21 @@ __global__ __launch_bounds__(64) void k_bpromote(int B, int32_t* __restrict__ cs
 
 // FP64 residual for START / REPLACE columns: r64 = load - sum Y64 (element
 // order, FP64; Y64 holds the residual list's columns at their list places);
-// r = (float) r64; START: p = D r; partial r.(D r)
+// r = (float) r64; START: p_1 = D r; partials r.(D r) and (START) max|x|
 __global__ __launch_bounds__(256) void k_bresid(
     int64_t n1, int B, int ck, const int32_t* __restrict__ rowptr,
     const int32_t* __restrict__ slots, const double* __restrict__ Y64,
     const double* __restrict__ load, const float* __restrict__ dinv,
-    float* __restrict__ r, float* __restrict__ p, double* __restrict__ part,
-    const int32_t* __restrict__ cst, const int32_t* __restrict__ rslot) {
+    const double* __restrict__ x, float* __restrict__ r, float* __restrict__ p,
+    double* __restrict__ part, const int32_t* __restrict__ cst,
+    const int32_t* __restrict__ rslot) {
   const ChunkIdx c = chunk_of(n1, B, ck);
   if (!c.valid) return;
   const int st = cst[c.j];
   if (st != BCOL_START && st != BCOL_REPLACE) return;
   const int qs = rslot[c.j];
-  double s = 0.0;
+  const int64_t nv = n1 * B;
+  double s = 0.0, mx = 0.0;
   for (int64_t g = c.g0; g < c.g1; g++) {
     const int64_t q = g * B + c.j;
     double kx = 0.0;
@@ -561,10 +619,15 @@ __global__ __launch_bounds__(256) void k_bresid(
     const float rg = (float)r64;
     const float zg = dinv[g] * rg;
     r[q] = rg;
-    if (st == BCOL_START) p[q] = zg;
+    if (st == BCOL_START) {  // p_1 = D r (buffer 1); max|x_0| for the stop test
+      p[nv + q] = zg;
+      mx = fmax(mx, fabs(x[q]));
+    }
     s = s + (double)rg * (double)zg;
   }
-  part[c.chunk * B + c.j] = s;
+  const int64_t nch = nchunks(n1, ck);
+  part[(0 * nch + c.chunk) * B + c.j] = s;
+  part[(1 * nch + c.chunk) * B + c.j] = mx;
 }
 
 // Reductions of the chunk partials, in a fixed order independent of B:
@@ -574,7 +637,7 @@ __global__ __launch_bounds__(256) void k_bresid(
 // depends only on the chunk count (batch_group): 64 on large systems, 16
 // when there are few chunks, so a small system's stage 1 is one round of
 // loads and its stage 2 stays short.
-//   RESID: chunks of equations; r.z of the FP64 residual.
+//   RESID: chunks of equations; r.z of the FP64 residual, max|x| (START).
 //   ALPHA: the elements' p.Kp shares from the MFMA product (a "chunk" is an
 //          element).
 //   INNER: chunks of equations; max|x|, r.z, max|p|.
@@ -586,11 +649,13 @@ __device__ __forceinline__ int fin_states(int what) {
                             : BCOL_BIT(BCOL_INNER);
 }
 
-__device__ __forceinline__ int fin_kinds(int what) { return what == BFIN_INNER ? 3 : 1; }
+__device__ __forceinline__ int fin_kinds(int what) {
+  return what == BFIN_INNER ? 3 : what == BFIN_RESID ? 2 : 1;
+}
 
 // kind k of `what` combines by max (else by sum)
 __device__ __forceinline__ bool fin_max(int what, int k) {
-  return what == BFIN_INNER && k != 1;
+  return (what == BFIN_INNER && k != 1) || (what == BFIN_RESID && k == 1);
 }
 
 __global__ __launch_bounds__(256) void k_bgroup(int64_t nch, int B, int G,
@@ -652,13 +717,15 @@ __global__ __launch_bounds__(256) void k_bfinalize(int64_t nch, int B, int G,
   if (threadIdx.x != 0) return;
   BCol& c = bc[j];
   switch (what) {
-    case BFIN_RESID:  // v0 = r.z of the FP64 residual
+    case BFIN_RESID:  // v0 = r.z of the FP64 residual, v1 = max|x| (START)
       c.nrep += 1;
       if (st == BCOL_START) {
         c.rz = v[0];
         c.rz0 = v[0];
         c.it = 0;
         c.pending = 0;
+        c.maxx = v[1];
+        c.xlag = 0;
         cst[j] = (v[0] == 0.0) ? BCOL_DONE : BCOL_INNER;  // r = 0: x is exact
       } else {  // replaces r_{k+1}: beta = r.z (true) / r_k.z_k
         c.beta = v[0] / c.rz_prev;
@@ -669,14 +736,30 @@ __global__ __launch_bounds__(256) void k_bfinalize(int64_t nch, int B, int G,
       }
       break;
     case BFIN_ALPHA:  // v0 = p.Kp
+      c.alpha_prev = c.alpha;
       c.alpha = c.rz / v[0];
       c.pending = 0;
       break;
-    case BFIN_INNER: {  // v0 = max|x|, v1 = r.z, v2 = max|p|
+    case BFIN_INNER: {  // v0 = max|x| (x written), v1 = r.z, v2 = max|p|
       c.it += 1;
       c.maxp = v[2];
-      // relative update test of src/solver.f90:65 (after the x update)
-      if (fabs(c.alpha) * v[2] / v[0] <= c.tol) {
+      const bool lag = c.xdefer && (c.it & 1);
+      bool stop;
+      if (!lag) {
+        // relative update test of src/solver.f90:65 (after the x update)
+        stop = fabs(c.alpha) * v[2] / v[0] <= c.tol;
+        c.maxx = v[0];
+      } else {
+        // x_it = x_{it-1} + alpha p_it is not written yet, but
+        // max|x_it| >= max|x_{it-1}| - a: a <= tol (max|x_{it-1}| - a)
+        // passes the same test for certain.  Only inside the window of
+        // relative width ~tol^2 where the bound cannot decide does the solve
+        // iterate once more than the test would.
+        const double a = fabs(c.alpha) * v[2];
+        stop = a <= c.tol * (c.maxx - a);
+      }
+      c.xlag = lag;
+      if (stop) {
         cst[j] = BCOL_DONE;
       } else if (c.it >= c.maxiter) {
         cst[j] = BCOL_FAIL;
@@ -854,7 +937,8 @@ static inline int bmv_threads(int B) {
 // XCD-aware element split (nxcd 8) or a plain grid-stride walk (nxcd 1).
 void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu, BmvGeom geom,
                          const float* KP, const int32_t* gdof, const float* P,
-                         float* Y, double* edot, const int32_t* cst, int want_mask) {
+                         int64_t nv, float* Y, double* edot, const int32_t* cst,
+                         const BCol* bc, int want_mask) {
   const int nx = geom.xcd_split ? BMV_XCD : 1;
   const int bpc = geom.blocks_per_cu < 1 ? 1 : geom.blocks_per_cu > 3 ? 3 : geom.blocks_per_cu;
   int64_t g = (int64_t)n_cu * bpc;
@@ -864,13 +948,13 @@ void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu, BmvGeom 
   const dim3 grid((unsigned)g);
   if (B >= 64)
     hipLaunchKernelGGL(k_bmv_mfma<4>, grid, dim3(256), 0, s, nelmt, B, nx, KP, gdof, P,
-                       Y, edot, cst, want_mask);
+                       nv, Y, edot, cst, bc, want_mask);
   else if (B == 32)
     hipLaunchKernelGGL(k_bmv_mfma<2>, grid, dim3(256), 0, s, nelmt, B, nx, KP, gdof, P,
-                       Y, edot, cst, want_mask);
+                       nv, Y, edot, cst, bc, want_mask);
   else
     hipLaunchKernelGGL(k_bmv_mfma<1>, grid, dim3(256), 0, s, nelmt, B, nx, KP, gdof, P,
-                       Y, edot, cst, want_mask);
+                       nv, Y, edot, cst, bc, want_mask);
 }
 
 // FP64 residual product: persistent, 2 blocks per CU (60 KB of LDS each);
@@ -912,11 +996,18 @@ void launch_bpromote(hipStream_t s, int B, int32_t* cst, BCol* bc, int32_t* rlis
 
 void launch_bresid(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
                    const int32_t* slots, const double* Y64, const double* load,
-                   const float* dinv, float* r, float* p, double* part,
-                   const int32_t* cst, const int32_t* rslot) {
+                   const float* dinv, const double* x, float* r, float* p,
+                   double* part, const int32_t* cst, const int32_t* rslot) {
   hipLaunchKernelGGL(k_bresid, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
-                     batch_chunk(n1), rowptr, slots, Y64, load, dinv, r, p, part, cst,
-                     rslot);
+                     batch_chunk(n1), rowptr, slots, Y64, load, dinv, x, r, p, part,
+                     cst, rslot);
+}
+
+void launch_bmaterialize(hipStream_t s, int64_t n1, int B, double* x, const float* p,
+                         const int32_t* cst, BCol* bc) {
+  hipLaunchKernelGGL(k_bmaterialize, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
+                     batch_chunk(n1), x, p, cst, bc);
+  hipLaunchKernelGGL(k_bxlag_clear, dim3((B + 63) / 64), dim3(64), 0, s, B, cst, bc);
 }
 
 // `nch` partials per column and kind in `part` (equation chunks, or elements
@@ -925,7 +1016,8 @@ void launch_bfinalize(hipStream_t s, int64_t nch, int B, double* part,
                       int32_t* cst, BCol* bc, int what) {
   const int G = batch_group(nch);
   const int64_t ng = (nch + G - 1) / G;
-  double* part2 = part + (what == BFIN_INNER ? 3 : 1) * nch * B;
+  const int nk = what == BFIN_INNER ? 3 : what == BFIN_RESID ? 2 : 1;  // fin_kinds
+  double* part2 = part + nk * nch * B;
   hipLaunchKernelGGL(k_bgroup, dim3((unsigned)((ng * B + 255) / 256)), dim3(256), 0,
                      s, nch, B, G, part, part2, cst, what);
   hipLaunchKernelGGL(k_bfinalize, dim3(B), dim3(256), 0, s, nch, B, G, part2, cst, bc,

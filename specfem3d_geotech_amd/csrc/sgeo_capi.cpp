@@ -1526,7 +1526,7 @@ extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extlo
   TRY(dalloc(&b->d_ext, (size_t)b->n1, acc));
   TRY(dalloc(&b->d_xel, (size_t)b->n1, acc));
   TRY(dalloc(&b->d_r, nv, acc));
-  TRY(dalloc(&b->d_p, nv, acc));
+  TRY(dalloc(&b->d_p, 2 * nv, acc));  // search directions of even and odd iterations
   TRY(dalloc(&b->d_Y32, ny, acc));
   TRY(dalloc(&b->d_dinv32, (size_t)b->n1, acc));
   TRY(dalloc(&b->d_evpt, ns, acc));
@@ -1616,6 +1616,9 @@ extern "C" int sgeo_batch_set_graphs(sgeo_batch* b, int enable) {
 extern "C" int sgeo_batch_set_residual_period(sgeo_batch* b, int32_t period) {
   if (!b || (period != 1 && period != 2 && period != 4 && period != 8))
     return fail(SGEO_EARG, "sgeo_batch_set_residual_period: period 1, 2, 4 or 8");
+  for (int j = 0; j < b->B; j++)
+    if (b->phase[j] == BPH_RUN && period != b->resid_period)
+      return fail(SGEO_ESTATE, "sgeo_batch_set_residual_period: trials are running");
   if (period != b->resid_period) batch_drop_graphs(b);
   b->resid_period = period;
   return 0;
@@ -1662,7 +1665,8 @@ extern "C" int sgeo_batch_autotune(sgeo_batch* b, int32_t reps, int32_t* blocks_
       g.xcd_split = xs;
       auto launch = [&]() {
         launch_bmv_mfma_f32(s, m->nelmt, b->B, c->n_cu, g, m->d_KP32, m->d_gdof, b->d_p,
-                            b->d_Y32, b->d_edot, b->d_cst, BCOL_BIT(BCOL_INNER));
+                            b->n1 * b->B, b->d_Y32, b->d_edot, b->d_cst, b->d_bc,
+                            BCOL_BIT(BCOL_INNER));
       };
       launch();  // warm-up
       HIPCHK(hipEventRecord(e0, s));
@@ -1908,6 +1912,9 @@ static int batch_start(sgeo_batch* b, const std::vector<int32_t>& cols, double c
     bc.tol = cg_tol;
     bc.inner_tol = inner_tol;
     bc.maxiter = cg_maxiter;
+    // residual steps fall on even iterations only with an even period:
+    // then odd iterations may defer their x update to the next one
+    bc.xdefer = b->resid_period % 2 == 0 ? 1 : 0;
     bc.it = skip ? b->seed_cg[j] : 0;
     b->need_start[j] = 0;
     b->last_it[j] = bc.it;
@@ -1938,14 +1945,14 @@ static void batch_step_launches(sgeo_batch* b, bool prof, bool resid) {
     launch_bmv_f64(s, m->nelmt, B, c->n_cu, m->d_KP64, m->d_gdof, b->d_x, b->d_Y64,
                    b->d_rlist);
     launch_bresid(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y64, b->d_load, b->d_dinv32,
-                  b->d_r, b->d_p, b->d_part, b->d_cst, b->d_rslot);
+                  b->d_x, b->d_r, b->d_p, b->d_part, b->d_cst, b->d_rslot);
     launch_bfinalize(s, nch, B, b->d_part, b->d_cst, b->d_bc, BFIN_RESID);
   }
   launch_bpupdate(s, n1, B, b->d_r, b->d_dinv32, b->d_p, b->d_cst, b->d_bc);
   if (prof) prof_mark(c);
   // Y = K P and the elements' p.Kp shares; alpha = r.z / p.Kp
   launch_bmv_mfma_f32(s, m->nelmt, B, c->n_cu, b->geom, m->d_KP32, m->d_gdof, b->d_p,
-                      b->d_Y32, b->d_edot, b->d_cst, BCOL_BIT(BCOL_INNER));
+                      n1 * B, b->d_Y32, b->d_edot, b->d_cst, b->d_bc, BCOL_BIT(BCOL_INNER));
   if (prof) prof_mark(c);
   launch_bfinalize(s, m->nelmt, B, b->d_edot, b->d_cst, b->d_bc, BFIN_ALPHA);
   // kp assembled from Y; x, r, r.z, max|x|, max|p|; the stop test
@@ -2019,6 +2026,8 @@ static int batch_body(sgeo_batch* b, const std::vector<int32_t>& cols, double nl
   sgeo_mesh* m = b->mesh;
   hipStream_t s = m->ctx->stream;
   const int B = b->B;
+  // x of the solves that ended on a deferred (odd) iteration
+  launch_bmaterialize(s, b->n1, B, b->d_x, b->d_p, b->d_cst, b->d_bc);
   for (int j : cols) {  // first PCG call of a trial from x = 0: the elastic memo
     if (!b->memo_valid && b->nl_iter[j] == 0 && !b->seeded[j] && b->cst[j] == BCOL_DONE) {
       launch_col_io(s, b->n1, B, j, b->d_x, nullptr, b->d_xel);

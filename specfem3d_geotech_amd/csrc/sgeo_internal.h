@@ -140,16 +140,24 @@ enum BColState {
 #define BCOL_BIT(s) (1 << (s))
 enum BFinWhat { BFIN_RESID = 0, BFIN_ALPHA = 1, BFIN_INNER = 2 };
 
+// With xdefer, the x update of every odd iteration is deferred and applied
+// with the next one, x_i = (x_{i-2} + alpha_{i-1} p_{i-1}) + alpha_i p_i (the
+// same FP64 operations in the same order as one update per iteration); the
+// search direction of iteration i lives in p buffer i & 1.
 struct BCol {  // per-column scalars, device resident
   double rz;        // r.z of the current residual
   double rz0;       // r.z at the last FP64 residual
   double alpha, beta, maxp;
   double tol, inner_tol;
   double rz_prev;   // r.z of the previous iteration (beta after a replacement)
+  double alpha_prev;  // alpha of the previous iteration
+  double maxx;        // max|x| of the last x written (exact)
   int32_t it, maxiter;  // CG iterations of this solve, limit
   int32_t pending;      // 1: p = D r + beta p before the next product
   int32_t nrep;         // FP64 residuals of this solve
   int32_t want_rep;     // 1: replace r at the next residual step
+  int32_t xdefer;       // 1: odd iterations defer their x update
+  int32_t xlag;         // 1: x lacks alpha p of the last (odd) iteration
 };
 
 // GLL numbering of a hex8 mesh, built on the device (sgeo_ingest.hip)
@@ -178,7 +186,8 @@ int64_t batch_element_partials(int32_t nelmt);  // per column, element p.Kp shar
 int64_t batch_equation_chunks(int64_t n1);
 void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu, BmvGeom geom,
                          const float* KP, const int32_t* gdof, const float* P,
-                         float* Y, double* edot, const int32_t* cst, int want_mask);
+                         int64_t nv, float* Y, double* edot, const int32_t* cst,
+                         const BCol* bc, int want_mask);
 void launch_bmv_f64(hipStream_t s, int32_t nelmt, int B, int n_cu, const double* KP,
                     const int32_t* gdof, const double* X, double* Y,
                     const int32_t* rlist);
@@ -189,12 +198,14 @@ void launch_bupdate(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
 void launch_bpupdate(hipStream_t s, int64_t n1, int B, const float* r,
                      const float* dinv, float* p, const int32_t* cst,
                      const BCol* bc);
+void launch_bmaterialize(hipStream_t s, int64_t n1, int B, double* x, const float* p,
+                         const int32_t* cst, BCol* bc);
 void launch_bpromote(hipStream_t s, int B, int32_t* cst, BCol* bc, int32_t* rlist,
                      int32_t* rslot);
 void launch_bresid(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
                    const int32_t* slots, const double* Y64, const double* load,
-                   const float* dinv, float* r, float* p, double* part,
-                   const int32_t* cst, const int32_t* rslot);
+                   const float* dinv, const double* x, float* r, float* p,
+                   double* part, const int32_t* cst, const int32_t* rslot);
 void launch_bfinalize(hipStream_t s, int64_t nch, int B, double* part,
                       int32_t* cst, BCol* bc, int what);
 void launch_bload(hipStream_t s, int64_t n1, int B, const double* ext,
