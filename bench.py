@@ -297,6 +297,7 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
                     batch_hbm_gb=round(nbytes / 1e9, 2), copy_kernel_gbs=round(copy_gbs, 1),
                     ingest_s=round(prep_s, 3), k_build_s=round(kbuild_s, 3)),
         trial_iterations=ncg, mean_trials_iterating=round(b_eff, 2),
+        fp64_residuals=s1["fp64_residuals"] - s0["fp64_residuals"],
         per_trial_iteration_ms=round(per_trial_ms, 5),
         fp64_dropin=dict(ms_per_iteration=round(fp64_ms_iter, 4),
                          dof_iterations_per_s=N / (fp64_ms_iter * 1e-3),
@@ -317,7 +318,7 @@ def mode_sweep(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
                           dict(bytes_model="E*(3328*4 + 81*4 + (2*81*4 + 8)*b), "
                                "b = mean trials iterating per step of the roofline window",
                                b=round(b_prof, 2),
-                               traffic_source="profiles/bmv_mfma_c1_b64_traffic.md "
+                               traffic_source="profiles/batched_step_c1_b64_traffic.md "
                                "(c1, 64 trials iterating)"),
                           traffic=measured_traffic("k_bmv_mfma<4>")
                           if (a.workload == "c1" and B == 64) else None))
