@@ -93,6 +93,7 @@ SIGNATURES = {
     "sgeo_batch_nl_iters": (C.c_int, [vp, C.POINTER(C.c_int32)]),
     "sgeo_batch_set_geometry": (C.c_int, [vp, C.c_int32, C.c_int32]),
     "sgeo_batch_set_residual_period": (C.c_int, [vp, C.c_int32]),
+    "sgeo_batch_set_deferred_x": (C.c_int, [vp, C.c_int32]),
     "sgeo_batch_autotune": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_int32),
                                       C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]),

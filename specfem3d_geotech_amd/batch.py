@@ -150,6 +150,12 @@ class BatchEngine:
         check(L.lib().sgeo_batch_set_residual_period(self.h, int(period)),
               "sgeo_batch_set_residual_period")
 
+    def set_deferred_x(self, enable: bool):
+        """Odd iterations defer their x update to the next one (with an even
+        residual period); x has the same bits either way."""
+        check(L.lib().sgeo_batch_set_deferred_x(self.h, int(bool(enable))),
+              "sgeo_batch_set_deferred_x")
+
     def nl_iters(self) -> np.ndarray:
         out = np.zeros(self.ncol, np.int32)
         check(L.lib().sgeo_batch_nl_iters(self.h, ptr(out, C.c_int32)),

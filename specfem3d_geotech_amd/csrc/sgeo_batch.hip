@@ -423,11 +423,13 @@ __device__ __forceinline__ int64_t nchunks(int64_t n1, int ck) {
 }
 
 // p_i = D r + beta p_{i-1} for INNER columns with a pending beta (i the
-// iteration about to run; p_i in buffer i & 1 of the two n1 x B buffers)
+// iteration about to run; p_i in buffer i & 1 of the two n1 x B buffers),
+// and the chunk partial of max|p_i| for the iteration's stop test
 __global__ __launch_bounds__(256) void k_bpupdate(int64_t n1, int B, int ck,
                                                   const float* __restrict__ r,
                                                   const float* __restrict__ dinv,
                                                   float* __restrict__ p,
+                                                  double* __restrict__ part,
                                                   const int32_t* __restrict__ cst,
                                                   const BCol* __restrict__ bc) {
   const ChunkIdx c = chunk_of(n1, B, ck);
@@ -437,10 +439,14 @@ __global__ __launch_bounds__(256) void k_bpupdate(int64_t n1, int B, int ck,
   const int64_t nv = n1 * B;
   const float* __restrict__ pold = p + ((i - 1) & 1) * nv;
   float* __restrict__ pnew = p + (i & 1) * nv;
+  double mp = 0.0;
   for (int64_t g = c.g0; g < c.g1; g++) {
     const int64_t q = g * B + c.j;
-    pnew[q] = dinv[g] * r[q] + beta * pold[q];
+    const float pg = dinv[g] * r[q] + beta * pold[q];
+    pnew[q] = pg;
+    mp = fmax(mp, fabs((double)pg));
   }
+  part[(2 * nchunks(n1, ck) + c.chunk) * B + c.j] = mp;  // max|p_i| (INNER kind 2)
 }
 
 // The deferred x update of a column whose solve ended on an odd iteration
@@ -473,8 +479,10 @@ __global__ void k_bxlag_clear(int B, const int32_t* __restrict__ cst,
 // The update of an iteration, with kp assembled on the fly:
 //   kp[g][j] = sum over the slots of Y (element order, FP32),
 //   x += alpha p (FP64; see BCol.xdefer), r -= alpha kp, z = D r,
-// and the partials max|x| (after the update, x(0) = 0; when x is written),
-// r.z (FP64) and max|p| for INNER columns.  The (at most 8 on hexahedral meshes) element
+// and the partials max|x| (after the update, x(0) = 0; when x is written)
+// and r.z (FP64) for INNER columns.  The max|p| partial of the iteration
+// comes from where p was formed (k_bpupdate, or k_bresid for p_1), so a
+// deferred iteration does not read p at all.  The (at most 8 on hexahedral meshes) element
 // contributions of an equation are loaded together, then summed in slot
 // order; UPD_EQ equations per trip keep that many gathers in flight.
 // WAVE_CHUNK: B = 64, so a wave's 64 lanes are the 64 columns of one chunk
@@ -509,7 +517,7 @@ __global__ __launch_bounds__(256) void k_bupdate(
   const int64_t nv = n1 * B;
   const float* __restrict__ pi = p + (i & 1) * nv;
   const float* __restrict__ pp = p + ((i - 1) & 1) * nv;
-  double mx = 0.0, s = 0.0, mp = 0.0;
+  double mx = 0.0, s = 0.0;
   auto gather = [&](int64_t g, float* v, int32_t& q0, int32_t& q1) {
     q0 = g > 0 ? rowptr[g] : 0;
     q1 = g > 0 ? rowptr[g + 1] : 0;
@@ -524,18 +532,16 @@ __global__ __launch_bounds__(256) void k_bupdate(
       if (q0 + k < q1) kp = kp + v[k];
     for (int32_t q = q0 + 8; q < q1; q++) kp = kp + Y[(int64_t)slots[q] * B + c.j];
     const int64_t q = g * B + c.j;
-    const float pg = pi[q];
     if (xw) {
       double xg = 0.0;
       if (g != 0) {
         xg = x[q];
         if (two) xg = xg + alpha_p * (double)pp[q];
-        xg = xg + alpha * (double)pg;
+        xg = xg + alpha * (double)pi[q];
       }
       x[q] = xg;
       mx = fmax(mx, fabs(xg));
     }
-    mp = fmax(mp, fabs((double)pg));
     const float rg = r[q] - alpha32 * kp;
     r[q] = rg;
     const float zg = dinv[g] * rg;
@@ -558,8 +564,7 @@ __global__ __launch_bounds__(256) void k_bupdate(
   }
   const int64_t nch = nchunks(n1, ck);
   part[(0 * nch + c.chunk) * B + c.j] = mx;
-  part[(1 * nch + c.chunk) * B + c.j] = s;
-  part[(2 * nch + c.chunk) * B + c.j] = mp;
+  part[(1 * nch + c.chunk) * B + c.j] = s;  // kind 2, max|p_i|: k_bpupdate / k_bresid
 }
 
 // At a residual step: INNER columns due for a replacement become REPLACE
@@ -594,7 +599,8 @@ __global__ __launch_bounds__(64) void k_bpromote(int B, int32_t* __restrict__ cs
 
 // FP64 residual for START / REPLACE columns: r64 = load - sum Y64 (element
 // order, FP64; Y64 holds the residual list's columns at their list places);
-// r = (float) r64; START: p_1 = D r; partials r.(D r) and (START) max|x|
+// r = (float) r64; START: p_1 = D r; partials r.(D r) and (START) max|x_0|,
+// and for START the first iteration's max|p_1|
 __global__ __launch_bounds__(256) void k_bresid(
     int64_t n1, int B, int ck, const int32_t* __restrict__ rowptr,
     const int32_t* __restrict__ slots, const double* __restrict__ Y64,
@@ -608,7 +614,7 @@ __global__ __launch_bounds__(256) void k_bresid(
   if (st != BCOL_START && st != BCOL_REPLACE) return;
   const int qs = rslot[c.j];
   const int64_t nv = n1 * B;
-  double s = 0.0, mx = 0.0;
+  double s = 0.0, mx = 0.0, mp = 0.0;
   for (int64_t g = c.g0; g < c.g1; g++) {
     const int64_t q = g * B + c.j;
     double kx = 0.0;
@@ -619,15 +625,17 @@ __global__ __launch_bounds__(256) void k_bresid(
     const float rg = (float)r64;
     const float zg = dinv[g] * rg;
     r[q] = rg;
-    if (st == BCOL_START) {  // p_1 = D r (buffer 1); max|x_0| for the stop test
+    if (st == BCOL_START) {  // p_1 = D r (buffer 1); max|x_0|, max|p_1|
       p[nv + q] = zg;
       mx = fmax(mx, fabs(x[q]));
+      mp = fmax(mp, fabs((double)zg));
     }
     s = s + (double)rg * (double)zg;
   }
   const int64_t nch = nchunks(n1, ck);
   part[(0 * nch + c.chunk) * B + c.j] = s;
   part[(1 * nch + c.chunk) * B + c.j] = mx;
+  if (st == BCOL_START) part[(2 * nch + c.chunk) * B + c.j] = mp;  // INNER kind 2
 }
 
 // Reductions of the chunk partials, in a fixed order independent of B:
@@ -983,10 +991,10 @@ void launch_bupdate(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
 }
 
 void launch_bpupdate(hipStream_t s, int64_t n1, int B, const float* r,
-                     const float* dinv, float* p, const int32_t* cst,
+                     const float* dinv, float* p, double* part, const int32_t* cst,
                      const BCol* bc) {
   hipLaunchKernelGGL(k_bpupdate, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
-                     batch_chunk(n1), r, dinv, p, cst, bc);
+                     batch_chunk(n1), r, dinv, p, part, cst, bc);
 }
 
 void launch_bpromote(hipStream_t s, int B, int32_t* cst, BCol* bc, int32_t* rlist,
@@ -1016,8 +1024,10 @@ void launch_bfinalize(hipStream_t s, int64_t nch, int B, double* part,
                       int32_t* cst, BCol* bc, int what) {
   const int G = batch_group(nch);
   const int64_t ng = (nch + G - 1) / G;
-  const int nk = what == BFIN_INNER ? 3 : what == BFIN_RESID ? 2 : 1;  // fin_kinds
-  double* part2 = part + nk * nch * B;
+  // group partials after the chunk partials: for the equation-chunk
+  // reductions after all three kinds (RESID runs before k_bpupdate writes
+  // kind 2), for ALPHA after the element shares
+  double* part2 = part + (what == BFIN_ALPHA ? 1 : BFIN_KINDS) * nch * B;
   hipLaunchKernelGGL(k_bgroup, dim3((unsigned)((ng * B + 255) / 256)), dim3(256), 0,
                      s, nch, B, G, part, part2, cst, what);
   hipLaunchKernelGGL(k_bfinalize, dim3(B), dim3(256), 0, s, nch, B, G, part2, cst, bc,

@@ -1463,6 +1463,7 @@ struct sgeo_batch {
   // own iterations
   int resid_period = 1;
   int64_t gstep = 0;
+  bool defer_x = true;  // odd iterations defer their x update (even periods)
   int32_t* h_stage = nullptr;  // pinned staging of the small uploads
   unsigned stage_busy = 0;
   double step_us = 0.0;  // running mean of the wall time of one step
@@ -1621,6 +1622,12 @@ extern "C" int sgeo_batch_set_residual_period(sgeo_batch* b, int32_t period) {
       return fail(SGEO_ESTATE, "sgeo_batch_set_residual_period: trials are running");
   if (period != b->resid_period) batch_drop_graphs(b);
   b->resid_period = period;
+  return 0;
+}
+
+extern "C" int sgeo_batch_set_deferred_x(sgeo_batch* b, int32_t enable) {
+  if (!b) return fail(SGEO_EARG, "null batch");
+  b->defer_x = enable != 0;  // read when a PCG call starts
   return 0;
 }
 
@@ -1914,7 +1921,7 @@ static int batch_start(sgeo_batch* b, const std::vector<int32_t>& cols, double c
     bc.maxiter = cg_maxiter;
     // residual steps fall on even iterations only with an even period:
     // then odd iterations may defer their x update to the next one
-    bc.xdefer = b->resid_period % 2 == 0 ? 1 : 0;
+    bc.xdefer = b->defer_x && b->resid_period % 2 == 0 ? 1 : 0;
     bc.it = skip ? b->seed_cg[j] : 0;
     b->need_start[j] = 0;
     b->last_it[j] = bc.it;
@@ -1948,7 +1955,7 @@ static void batch_step_launches(sgeo_batch* b, bool prof, bool resid) {
                   b->d_x, b->d_r, b->d_p, b->d_part, b->d_cst, b->d_rslot);
     launch_bfinalize(s, nch, B, b->d_part, b->d_cst, b->d_bc, BFIN_RESID);
   }
-  launch_bpupdate(s, n1, B, b->d_r, b->d_dinv32, b->d_p, b->d_cst, b->d_bc);
+  launch_bpupdate(s, n1, B, b->d_r, b->d_dinv32, b->d_p, b->d_part, b->d_cst, b->d_bc);
   if (prof) prof_mark(c);
   // Y = K P and the elements' p.Kp shares; alpha = r.z / p.Kp
   launch_bmv_mfma_f32(s, m->nelmt, B, c->n_cu, b->geom, m->d_KP32, m->d_gdof, b->d_p,

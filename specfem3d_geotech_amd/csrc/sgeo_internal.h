@@ -196,7 +196,7 @@ void launch_bupdate(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
                     const float* dinv, double* x, float* r, double* part,
                     const int32_t* cst, const BCol* bc);
 void launch_bpupdate(hipStream_t s, int64_t n1, int B, const float* r,
-                     const float* dinv, float* p, const int32_t* cst,
+                     const float* dinv, float* p, double* part, const int32_t* cst,
                      const BCol* bc);
 void launch_bmaterialize(hipStream_t s, int64_t n1, int B, double* x, const float* p,
                          const int32_t* cst, BCol* bc);

@@ -89,6 +89,41 @@ def test_batch_trial_identical_alone_and_in_mix(ctx, period):
         dm.close()
 
 
+def test_deferred_x_update_keeps_the_bits(ctx):
+    """With FP64 residuals every 8th step, odd iterations defer their x
+    update and the next iteration applies both in the same order; solves
+    that end on an odd iteration are completed before the body pass.  The
+    rows and x of a 4-column sweep are the same bits with the deferral on
+    and off (the stop test on a deferred iteration could only differ inside
+    a window of relative width cg_tol^2 around its threshold)."""
+    z = golden("slope_8x4x6_layered")
+    mesh = mesh_from_golden(z)
+    spec = hex2spec(mesh)
+    ctl = control_from_golden(z)
+    _, ym, nu, *_ = mesh.material_arrays()
+    kw = dict(ym=ym, nu=nu, cg_tol=ctl["cg_tol"], cg_maxiter=ctl["cg_maxiter"],
+              nl_tol=ctl["nl_tol"], nl_maxiter=40, keep_state=True, srf_parallel=2)
+    out = []
+    dm, ext = device_mesh(ctx, mesh, spec)
+    try:
+        for defer in (True, False):
+            an = _analyses(mesh, 4, [1.0, 1.3, 1.6], seed=5)
+            eng = BatchEngine(dm, ext, 4, resid_period=8)
+            try:
+                eng.set_deferred_x(defer)
+                run_analyses(eng, an, **kw)
+                st = eng.stats()
+            finally:
+                eng.close()
+            print("deferred" if defer else "every iteration", st,
+                  [a.bracket for a in an], flush=True)
+            out.append([(a.rows, a.bracket, a.x_last.tobytes(), a.flags_last.tobytes())
+                        for a in an])
+    finally:
+        dm.close()
+    assert out[0] == out[1]
+
+
 @pytest.mark.parametrize("name", ["slope_4x2x3", "slope_8x4x6_layered"])
 def test_mixed_solve_within_1e8_of_reference_solve(ctx, name):
     """One PCG solve, the reference's own first call (K x = extload from
