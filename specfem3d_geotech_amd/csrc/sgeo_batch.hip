@@ -37,8 +37,12 @@ __device__ __forceinline__ bool wanted(const int32_t* cst, int j, int mask) {
 // tile n = w % NT and row tiles m = w / NT (+ 4 / NT ...); the wave of each
 // column tile with the fewest row tiles also takes row 80.
 // The packed upper triangle of K_e is unpacked (with its mirror) into LDS
-// as Kf[c * BMV_LDK + r], zero padded in k; BMV_LDK = 112 puts the four
-// 16-lane groups of an A-operand read (c, c+1, c+2, c+3) on disjoint banks.
+// row by row, Kf[r * BMV_RLD + slot(c)] with slot(c) = (c % 4) * 24 + c / 4:
+// lane (i, kq) of k-steps 4m .. 4m+3 needs K(r, 4s + kq) for s = 4m .. 4m+3,
+// which sit at four consecutive slots, so one 16-byte read serves four
+// MFMAs; zero padded for k = 81..83.  BMV_RLD = 100 floats puts the 16 rows
+// of a read on disjoint groups of four banks and keeps every read 16-byte
+// aligned.
 // A column's result depends only on its own P column, so trials are
 // independent of the batch; columns not in `want_mask` are not written.
 //
@@ -58,8 +62,9 @@ __device__ __forceinline__ bool wanted(const int32_t* cst, int j, int mask) {
 // them in that order.  Per-element shares keep the reduction independent of
 // which block computed an element.
 // --------------------------------------------------------------------------
-#define BMV_LDK 112
+#define BMV_RLD 100
 #define BMV_KC 84
+__device__ __forceinline__ constexpr int bmv_slot(int c) { return (c & 3) * 24 + (c >> 2); }
 
 // (lane group, register) transpose of a 4x4 block of 16-lane groups:
 // on return a[i] in lane group g holds what a[g] held in lane group i
@@ -112,7 +117,7 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
   constexpr int NMT = 5;                 // MFMA row tiles: rows 0..79
   constexpr int NM = (NMT + NS - 1) / NS;  // row tiles per wave
   constexpr int KPT = (3321 + 255) / 256;  // packed entries per thread
-  __shared__ float Kf[BMV_KC * BMV_LDK];
+  __shared__ __align__(16) float Kf[81 * BMV_RLD];
   __shared__ int32_t sg[2][BMV_KC];
   // p.Kp shares of the row tiles (and row 80) of each column tile, when they
   // come from several waves
@@ -139,18 +144,19 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
   const int kq = l >> 4;
   const bool row80 = mpart == NS - 1;  // wave-uniform
   // zero padding, once: k columns 81..83 (rows 0..80)
-  for (int q = t; q < 3 * 81; q += 256) Kf[(81 + q / 81) * BMV_LDK + q % 81] = 0.0f;
+  for (int q = t; q < 3 * 81; q += 256) Kf[(q % 81) * BMV_RLD + bmv_slot(81 + q / 81)] = 0.0f;
   if (t >= 81 && t < BMV_KC) { sg[0][t] = 0; sg[1][t] = 0; }
   // LDS addresses of this thread's packed entries q = t + 256 i (fixed for
-  // the kernel): K(r, c) at c*LDK + r and its mirror at r*LDK + c, both
-  // below 2^16, packed in one word
+  // the kernel): K(r, c) at r*RLD + slot(c) and its mirror at c*RLD +
+  // slot(r), both below 2^16, packed in one word
   uint32_t kaddr[KPT];
 #pragma unroll
   for (int i = 0; i < KPT; i++) {
     const int q = t + 256 * i;
     const int c = q < 3321 ? tri_col(q) : 0;
     const int r = q < 3321 ? q - c * (c + 1) / 2 : 0;
-    kaddr[i] = (uint32_t)(c * BMV_LDK + r) | ((uint32_t)(r * BMV_LDK + c) << 16);
+    kaddr[i] = (uint32_t)(r * BMV_RLD + bmv_slot(c)) |
+               ((uint32_t)(c * BMV_RLD + bmv_slot(r)) << 16);
   }
   // prologue: K and gdof of the first element
   float kreg[KPT];
@@ -197,18 +203,41 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
 #pragma unroll
       for (int i = 0; i < NM; i++) acc[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
       float r80 = 0.0f;
+      // k-steps s = 0..19 four at a time (one 16-byte read per row tile),
+      // then s = 20; every accumulator sums its k-steps in order 0..20
+      const float* kr = Kf + (l & 15) * BMV_RLD + kq * 24;  // row (l & 15), group kq
 #pragma unroll
-      for (int s = 0; s < 21; s++) {
-        const float* kc = Kf + (4 * s + kq) * BMV_LDK + (l & 15);
+      for (int m = 0; m < 5; m++) {
+        f32x4 a[NM];
+        f32x4 a80;
 #pragma unroll
         for (int i = 0; i < NM; i++) {
           const int mt = mpart + i * NS;  // wave-uniform
           if (NM * NS > NMT && mt >= NMT) break;
-          acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(kc[mt * 16], bv[s], acc[i], 0, 0,
-                                                        0);
+          a[i] = *reinterpret_cast<const f32x4*>(kr + mt * 16 * BMV_RLD + 4 * m);
         }
-        if (row80) r80 = __builtin_fmaf(Kf[(4 * s + kq) * BMV_LDK + 80], bv[s], r80);
+        if (row80)
+          a80 = *reinterpret_cast<const f32x4*>(Kf + 80 * BMV_RLD + kq * 24 + 4 * m);
+#pragma unroll
+        for (int ss = 0; ss < 4; ss++) {
+          const int s = 4 * m + ss;
+#pragma unroll
+          for (int i = 0; i < NM; i++) {
+            const int mt = mpart + i * NS;
+            if (NM * NS > NMT && mt >= NMT) break;
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][ss], bv[s], acc[i], 0, 0, 0);
+          }
+          if (row80) r80 = __builtin_fmaf(a80[ss], bv[s], r80);
+        }
       }
+#pragma unroll
+      for (int i = 0; i < NM; i++) {
+        const int mt = mpart + i * NS;
+        if (NM * NS > NMT && mt >= NMT) break;
+        acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(kr[mt * 16 * BMV_RLD + 20], bv[20],
+                                                      acc[i], 0, 0, 0);
+      }
+      if (row80) r80 = __builtin_fmaf(Kf[80 * BMV_RLD + kq * 24 + 20], bv[20], r80);
       if (row80) {  // the four k-groups of column j: lanes j, j+16, j+32, j+48
         r80 = r80 + __shfl_xor(r80, 16);
         r80 = r80 + __shfl_xor(r80, 32);
