@@ -5,10 +5,11 @@
 //
 // Vectors are interleaved [n1][B] (column fastest): an element gathers 81
 // contiguous B-vectors and every update is unit-stride.  The inner CG
-// vectors (r, p, kp) and the element product are FP32 (the product on
-// MFMA); dot products, the solution x, the FP64 true residual f - K x and
-// all plastic state are FP64.  Every reduction is two-phase (per-chunk
-// partials, then one block per column) and combines in a fixed order, so a
+// vectors (r, p) and the element product are FP32 (the product on MFMA; K p
+// is summed from the element products inside the update, never stored);
+// dot products, the solution x, the FP64 true residual f - K x and all
+// plastic state are FP64.  Every reduction combines per-chunk (or
+// per-element) partials in groups, then per column, in a fixed order, so a
 // column's arithmetic does not depend on B or on the other columns: a trial
 // gives the same bits alone and in any batch.
 #pragma clang fp contract(off)

@@ -117,7 +117,7 @@ void launch_nodal_acc_out(hipStream_t s, int32_t nnode, const int32_t* rowptr,
 
 // ---- multi-trial mixed-precision engine (sgeo_batch.hip) -----------------
 // Per-column states of the mixed-precision PCG.  The inner iteration runs in
-// FP32 (operator, r, p, kp) with FP64 dot products and an FP64 solution x;
+// FP32 (operator, r, p, K p) with FP64 dot products and an FP64 solution x;
 // the FP64 true residual f - K x (FP64 operator) starts each solve and
 // replaces the recursive FP32 residual, keeping the search direction, once
 // r.z has dropped by inner_tol since the last replacement.  FP64 residuals
