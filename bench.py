@@ -88,6 +88,10 @@ def parse():
                     help="analysis mode: Poisson arrival rate (1/s) of requests")
     ap.add_argument("--srf-parallel", type=int, default=1,
                     help="analysis mode, batched: SRFs of one analysis in flight")
+    ap.add_argument("--defer-nl", type=int, default=None,
+                    help="analysis mode, batched: release an analysis's trials above a "
+                    "trial that has run this many plastic iterations without converging "
+                    "(latency over throughput; results unchanged)")
     ap.add_argument("--own-strengths", action="store_true",
                     help="analysis mode: the mesh's own strengths for every analysis")
     ap.add_argument("--requests", type=int, default=32)
@@ -411,7 +415,7 @@ def mode_analysis(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
                                  cg_maxiter=ctl["cg_maxiter"], nl_tol=ctl["nl_tol"],
                                  nl_maxiter=ctl["nl_maxiter"], arrivals=arrivals,
                                  on_done=done, srf_parallel=a.srf_parallel,
-                                 progress=prog, progress_steps=4000)
+                                 defer_nl=a.defer_nl, progress=prog, progress_steps=4000)
         finally:
             eng.close()
             dm.close()
@@ -456,7 +460,7 @@ def mode_analysis(a, ctx, wl, mesh, spec, prep_s, copy_gbs):
         dtype="f64" if form != "batched" else "f32 inner CG (MFMA) + f64", data=DATA,
         config=dict(workload=wl.description, nelmt=E, neq=N, form=form, mode="analysis",
                     analyses=nan, trials=a.trials if form == "batched" else 1,
-                    arrival_rate=a.rate, srf_parallel=a.srf_parallel,
+                    arrival_rate=a.rate, srf_parallel=a.srf_parallel, defer_nl=a.defer_nl,
                     strengths="own" if (a.own_strengths or not a.analyses and
                                         form != "batched") else "form A"),
         latency_s=dict(p50=percentile(lat, 50), p99=percentile(lat, 99), max=max(lat)),
