@@ -89,6 +89,36 @@ def test_batch_trial_identical_alone_and_in_mix(ctx, period):
         dm.close()
 
 
+def test_residual_period_arguments(ctx):
+    """The residual period is 1, 2, 4 or 8, and cannot change while trials
+    run (their residual steps fall at fixed counts of their own
+    iterations); setting the period in force again is accepted."""
+    from specfem3d_geotech_amd._lib import SgeoError
+    from specfem3d_geotech_amd.plastic import dt_viscoplas
+    z = golden("slope_4x2x3")
+    mesh = mesh_from_golden(z)
+    spec = hex2spec(mesh)
+    ctl = control_from_golden(z)
+    _, ym, nu, phi, coh, psi = mesh.material_arrays()
+    dm, ext = device_mesh(ctx, mesh, spec)
+    try:
+        eng = BatchEngine(dm, ext, 2, resid_period=8)
+        try:
+            for bad in (0, 3, 16):
+                with pytest.raises(SgeoError):
+                    eng.set_residual_period(bad)
+            eng.begin(0, ym, nu, phi, coh, psi, dt_viscoplas(nu, phi, ym))
+            eng.advance(ctl["cg_tol"], ctl["cg_maxiter"], DEFAULT_INNER_TOL, ctl["nl_tol"],
+                        ctl["nl_maxiter"], max_steps=16)
+            with pytest.raises(SgeoError):
+                eng.set_residual_period(4)
+            eng.set_residual_period(8)
+        finally:
+            eng.close()
+    finally:
+        dm.close()
+
+
 def test_deferred_x_update_keeps_the_bits(ctx):
     """With FP64 residuals every 8th step, odd iterations defer their x
     update and the next iteration applies both in the same order; solves
