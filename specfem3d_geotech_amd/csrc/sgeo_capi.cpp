@@ -1516,6 +1516,11 @@ extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extlo
   b->B = ncol;
   b->n1 = (int64_t)m->neq + 1;
   const size_t nv = (size_t)b->n1 * ncol;
+  // the element products address both search-direction buffers with 32-bit
+  // offsets (2 n1 B entries)
+  if (2 * (uint64_t)nv >= (1ull << 32))
+    return fail(SGEO_EARG, "sgeo_batch_create: %lld equations x %d columns too large",
+                (long long)b->n1, ncol);
   const size_t ny = (size_t)m->nelmt * 81 * ncol;
   const size_t ns = (size_t)m->nelmt * 27 * 6 * ncol;
   size_t* acc = &b->bytes;
