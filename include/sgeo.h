@@ -406,11 +406,12 @@ int sgeo_batch_set_geometry(sgeo_batch *batch, int32_t blocks_per_cu, int32_t xc
  * a residual is replaced), never on the batch or on the other columns.
  * Set it before starting trials.  No counterpart in the reference. */
 int sgeo_batch_set_residual_period(sgeo_batch *batch, int32_t period);
-/* With an even residual period, odd iterations defer their FP64 x update to
- * the next iteration, which applies both in the same order (x keeps its
- * bits; the HBM traffic of x halves).  enable = 0 writes x on every
- * iteration (default 1; taken at each PCG call's start).  No counterpart in
- * the reference. */
+/* With a residual period P > 1, x is written on every L-th iteration only
+ * (L = P, at most 8, or 2 when 8 (neq + 1) ncol reaches 2^32): that
+ * iteration applies the FP64 updates of the L - 1 before it in their order,
+ * so x keeps its bits, and the other iterations read neither x nor p.
+ * enable = 0 writes x on every iteration (default 1).  SGEO_ESTATE when it
+ * would change while trials run.  No counterpart in the reference. */
 int sgeo_batch_set_deferred_x(sgeo_batch *batch, int32_t enable);
 /* Totals since creation, as of the last host synchronisation: CG
  * iterations of all trials (seeded elastic solutions not counted), FP64

@@ -151,8 +151,9 @@ class BatchEngine:
               "sgeo_batch_set_residual_period")
 
     def set_deferred_x(self, enable: bool):
-        """Odd iterations defer their x update to the next one (with an even
-        residual period); x has the same bits either way."""
+        """x is written on every period-th iteration only (up to the 8th),
+        which applies the deferred updates in order; x has the same bits
+        either way.  Cannot change while trials run."""
         check(L.lib().sgeo_batch_set_deferred_x(self.h, int(bool(enable))),
               "sgeo_batch_set_deferred_x")
 

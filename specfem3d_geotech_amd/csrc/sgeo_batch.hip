@@ -141,7 +141,8 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
   const bool wave_on = __any(jv);
   const int jl = j < B ? j : B - 1;  // lanes past B read column B-1, store nothing
   // the column's search direction of the iteration being computed
-  const uint32_t poff = ((bc[jl].it + 1) & 1) ? (uint32_t)nv : 0u;
+  // (pmask + 1) n1 B < 2^32, checked when the engine is made
+  const uint32_t poff = (uint32_t)((bc[jl].it + 1) & bc[jl].pmask) * (uint32_t)nv;
   const int kq = l >> 4;
   const bool row80 = mpart == NS - 1;  // wave-uniform
   // zero padding, once: k columns 81..83 (rows 0..80)
@@ -452,9 +453,50 @@ __device__ __forceinline__ int64_t nchunks(int64_t n1, int ck) {
   return (n1 + ck - 1) / ck;
 }
 
+// Small systems (`grouped`: fewer than 16,384 chunks) combine the partials
+// of four consecutive chunks inside the block, in chunk order, and write
+// one value per group, part[(kind * ng4 + chunk / 4) * B + j] with
+// ng4 = ceil(nch / 4), which the final stage reads directly: one launch per
+// reduction.  The groups depend on the chunk index only (a block holds
+// 256 / B chunks, a multiple of 4), so a column's sums still do not depend
+// on B.  Every thread of the block takes part (there are barriers inside);
+// `put` says whether the thread's column writes.  Large systems write one
+// partial per chunk, part[(kind * nch + chunk) * B + j].
+template <bool GROUPED, int N>
+__device__ __forceinline__ void put_partials(double* __restrict__ part, double* lds,
+                                             int64_t nch, int B, const ChunkIdx& c,
+                                             const int (&kind)[N], const double (&v)[N],
+                                             const bool (&is_max)[N],
+                                             const bool (&put)[N]) {
+  if constexpr (!GROUPED) {
+#pragma unroll
+    for (int n = 0; n < N; n++)
+      if (put[n]) part[(kind[n] * nch + c.chunk) * B + c.j] = v[n];
+    return;
+  }
+  // lds holds N x 256 values; one barrier, the kernel's last use of lds
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int n = 0; n < N; n++) lds[n * 256 + t] = v[n];
+  __syncthreads();
+  if (((t / B) & 3) != 0) return;
+  const int64_t ng4 = (nch + 3) / 4;
+#pragma unroll
+  for (int n = 0; n < N; n++) {
+    if (!put[n]) continue;
+    double a = v[n];
+    for (int k = 1; k < 4 && c.chunk + k < nch; k++) {
+      const double w = lds[n * 256 + t + k * B];
+      a = is_max[n] ? fmax(a, w) : a + w;
+    }
+    part[(kind[n] * ng4 + c.chunk / 4) * B + c.j] = a;
+  }
+}
+
 // p_i = D r + beta p_{i-1} for INNER columns with a pending beta (i the
-// iteration about to run; p_i in buffer i & 1 of the two n1 x B buffers),
+// iteration about to run; p_i in buffer i & pmask of the n1 x B buffers),
 // and the chunk partial of max|p_i| for the iteration's stop test
+template <bool GROUPED>
 __global__ __launch_bounds__(256) void k_bpupdate(int64_t n1, int B, int ck,
                                                   const float* __restrict__ r,
                                                   const float* __restrict__ dinv,
@@ -462,26 +504,32 @@ __global__ __launch_bounds__(256) void k_bpupdate(int64_t n1, int B, int ck,
                                                   double* __restrict__ part,
                                                   const int32_t* __restrict__ cst,
                                                   const BCol* __restrict__ bc) {
+  __shared__ double lds[GROUPED ? 256 : 1];
   const ChunkIdx c = chunk_of(n1, B, ck);
-  if (!c.valid || cst[c.j] != BCOL_INNER || !bc[c.j].pending) return;
-  const float beta = (float)bc[c.j].beta;
-  const int i = bc[c.j].it + 1;
-  const int64_t nv = n1 * B;
-  const float* __restrict__ pold = p + ((i - 1) & 1) * nv;
-  float* __restrict__ pnew = p + (i & 1) * nv;
+  const bool act = c.valid && cst[c.j] == BCOL_INNER && bc[c.j].pending;
   double mp = 0.0;
-  for (int64_t g = c.g0; g < c.g1; g++) {
-    const int64_t q = g * B + c.j;
-    const float pg = dinv[g] * r[q] + beta * pold[q];
-    pnew[q] = pg;
-    mp = fmax(mp, fabs((double)pg));
+  if (act) {
+    const float beta = (float)bc[c.j].beta;
+    const int i = bc[c.j].it + 1;
+    const int64_t nv = n1 * B;
+    const int pm = bc[c.j].pmask;
+    const float* __restrict__ pold = p + ((i - 1) & pm) * nv;
+    float* __restrict__ pnew = p + (i & pm) * nv;
+    for (int64_t g = c.g0; g < c.g1; g++) {
+      const int64_t q = g * B + c.j;
+      const float pg = dinv[g] * r[q] + beta * pold[q];
+      pnew[q] = pg;
+      mp = fmax(mp, fabs((double)pg));
+    }
   }
-  part[(2 * nchunks(n1, ck) + c.chunk) * B + c.j] = mp;  // max|p_i| (INNER kind 2)
+  // max|p_i| (INNER kind 2)
+  put_partials<GROUPED, 1>(part, lds, nchunks(n1, ck), B, c, {2}, {mp}, {true}, {act});
 }
 
-// The deferred x update of a column whose solve ended on an odd iteration
-// (DONE or FAIL with xlag): x += alpha p_it, then xlag is cleared by
-// k_bxlag_clear once every chunk is done.
+// The deferred x updates of a column whose solve ended between two x
+// writes (DONE or FAIL with npend > 0): x += alpha_k p_k for the npend
+// iterations k after the last write, in order; k_bxlag_clear then clears
+// npend once every chunk is done.
 __global__ __launch_bounds__(256) void k_bmaterialize(int64_t n1, int B, int ck,
                                                       double* __restrict__ x,
                                                       const float* __restrict__ p,
@@ -490,12 +538,28 @@ __global__ __launch_bounds__(256) void k_bmaterialize(int64_t n1, int B, int ck,
   const ChunkIdx c = chunk_of(n1, B, ck);
   if (!c.valid) return;
   const int st = cst[c.j];
-  if ((st != BCOL_DONE && st != BCOL_FAIL) || !bc[c.j].xlag) return;
-  const double alpha = bc[c.j].alpha;
-  const float* __restrict__ pi = p + (bc[c.j].it & 1) * (n1 * B);
+  const BCol& bj = bc[c.j];
+  const int np = bj.npend;  // at most 7
+  if ((st != BCOL_DONE && st != BCOL_FAIL) || np == 0) return;
+  const int64_t nv = n1 * B;
+  double al[8];
+  const float* pk[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int it = bj.it - np + 1 + (k < np ? k : 0);
+    al[k] = bj.alpha_hist[it & 7];
+    pk[k] = p + (it & bj.pmask) * nv;
+  }
   for (int64_t g = c.g0; g < c.g1; g++) {
     const int64_t q = g * B + c.j;
-    x[q] = (g == 0) ? 0.0 : x[q] + alpha * (double)pi[q];
+    double xg = 0.0;
+    if (g != 0) {
+      xg = x[q];
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (k < np) xg = xg + al[k] * (double)pk[k][q];
+    }
+    x[q] = xg;
   }
 }
 
@@ -503,16 +567,16 @@ __global__ void k_bxlag_clear(int B, const int32_t* __restrict__ cst,
                               BCol* __restrict__ bc) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= B) return;
-  if (cst[j] == BCOL_DONE || cst[j] == BCOL_FAIL) bc[j].xlag = 0;
+  if (cst[j] == BCOL_DONE || cst[j] == BCOL_FAIL) bc[j].npend = 0;
 }
 
 // The update of an iteration, with kp assembled on the fly:
 //   kp[g][j] = sum over the slots of Y (element order, FP32),
-//   x += alpha p (FP64; see BCol.xdefer), r -= alpha kp, z = D r,
-// and the partials max|x| (after the update, x(0) = 0; when x is written)
-// and r.z (FP64) for INNER columns.  The max|p| partial of the iteration
-// comes from where p was formed (k_bpupdate, or k_bresid for p_1), so a
-// deferred iteration does not read p at all.  The (at most 8 on hexahedral meshes) element
+//   r -= alpha kp, z = D r,
+// and the r.z partial (FP64) for INNER columns.  x is written by k_bxfold
+// on the steps that write it; the max|p| partial of the iteration comes from
+// where p was formed (k_bpupdate, or k_bresid for p_1), so the update reads
+// neither x nor p.  The (at most 8 on hexahedral meshes) element
 // contributions of an equation are loaded together, then summed in slot
 // order; UPD_EQ equations per trip keep that many gathers in flight.
 // WAVE_CHUNK: B = 64, so a wave's 64 lanes are the 64 columns of one chunk
@@ -520,34 +584,14 @@ __global__ void k_bxlag_clear(int B, const int32_t* __restrict__ cst,
 #ifndef UPD_EQ
 #define UPD_EQ 4
 #endif
+// the body of k_bupdate for one active (chunk, column)
 template <bool WAVE_CHUNK>
-__global__ __launch_bounds__(256) void k_bupdate(
-    int64_t n1, int B, int ck, const int32_t* __restrict__ rowptr,
+__device__ __forceinline__ void update_chunk(
+    int64_t n1, int B, int ck, const ChunkIdx& c, const int32_t* __restrict__ rowptr,
     const int32_t* __restrict__ slots, const float* __restrict__ Y,
-    const float* __restrict__ p, const float* __restrict__ dinv,
-    double* __restrict__ x, float* __restrict__ r, double* __restrict__ part,
-    const int32_t* __restrict__ cst, const BCol* __restrict__ bc) {
-  ChunkIdx c = chunk_of(n1, B, ck);
-  if (WAVE_CHUNK) {
-    const int ch = __builtin_amdgcn_readfirstlane((int)c.chunk);
-    c.chunk = ch;
-    c.g0 = (int64_t)ch * ck;
-    c.g1 = c.g0 + ck < n1 ? c.g0 + ck : n1;
-    c.valid = c.g0 < n1;
-  }
-  if (!c.valid || cst[c.j] != BCOL_INNER) return;
-  const BCol& bj = bc[c.j];
-  const int i = bj.it + 1;  // the iteration this step computes
-  // x is written on every iteration, or with xdefer on even ones only,
-  // then with the deferred update of iteration i - 1 first
-  const bool xw = !bj.xdefer || !(i & 1);
-  const bool two = bj.xdefer && !(i & 1);
-  const double alpha = bj.alpha, alpha_p = bj.alpha_prev;
-  const float alpha32 = (float)alpha;
-  const int64_t nv = n1 * B;
-  const float* __restrict__ pi = p + (i & 1) * nv;
-  const float* __restrict__ pp = p + ((i - 1) & 1) * nv;
-  double mx = 0.0, s = 0.0;
+    const float* __restrict__ dinv, float* __restrict__ r, const BCol* __restrict__ bc,
+    double& s) {
+  const float alpha32 = (float)bc[c.j].alpha;
   auto gather = [&](int64_t g, float* v, int32_t& q0, int32_t& q1) {
     q0 = g > 0 ? rowptr[g] : 0;
     q1 = g > 0 ? rowptr[g + 1] : 0;
@@ -562,16 +606,6 @@ __global__ __launch_bounds__(256) void k_bupdate(
       if (q0 + k < q1) kp = kp + v[k];
     for (int32_t q = q0 + 8; q < q1; q++) kp = kp + Y[(int64_t)slots[q] * B + c.j];
     const int64_t q = g * B + c.j;
-    if (xw) {
-      double xg = 0.0;
-      if (g != 0) {
-        xg = x[q];
-        if (two) xg = xg + alpha_p * (double)pp[q];
-        xg = xg + alpha * (double)pi[q];
-      }
-      x[q] = xg;
-      mx = fmax(mx, fabs(xg));
-    }
     const float rg = r[q] - alpha32 * kp;
     r[q] = rg;
     const float zg = dinv[g] * rg;
@@ -592,9 +626,76 @@ __global__ __launch_bounds__(256) void k_bupdate(
     gather(g, va, a0, a1);
     finish(g, va, a0, a1);
   }
+}
+
+// one thread per (chunk, column); the r.z partial (kind 1)
+template <bool WAVE_CHUNK, bool GROUPED>
+__global__ __launch_bounds__(256) void k_bupdate(
+    int64_t n1, int B, int ck, const int32_t* __restrict__ rowptr,
+    const int32_t* __restrict__ slots, const float* __restrict__ Y,
+    const float* __restrict__ dinv, float* __restrict__ r, double* __restrict__ part,
+    const int32_t* __restrict__ cst, const BCol* __restrict__ bc) {
+  __shared__ double lds[GROUPED ? 256 : 1];
+  ChunkIdx c = chunk_of(n1, B, ck);
+  if (WAVE_CHUNK) {
+    const int ch = __builtin_amdgcn_readfirstlane((int)c.chunk);
+    c.chunk = ch;
+    c.g0 = (int64_t)ch * ck;
+    c.g1 = c.g0 + ck < n1 ? c.g0 + ck : n1;
+    c.valid = c.g0 < n1;
+  }
+  const bool act = c.valid && cst[c.j] == BCOL_INNER;
   const int64_t nch = nchunks(n1, ck);
-  part[(0 * nch + c.chunk) * B + c.j] = mx;
-  part[(1 * nch + c.chunk) * B + c.j] = s;  // kind 2, max|p_i|: k_bpupdate / k_bresid
+  double s = 0.0;
+  if (act) update_chunk<WAVE_CHUNK>(n1, B, ck, c, rowptr, slots, Y, dinv, r, bc, s);
+  put_partials<GROUPED, 1>(part, lds, nch, B, c, {1}, {s}, {false}, {act});
+}
+
+// The x write of an iteration i with i % xlen == 0 (INNER columns), after
+// the update of i: the npend deferred updates before i and i's own, in
+// order, x += alpha_k p_k for k = i - np + 1 .. i (np = npend + 1 <= 8),
+// and the max|x| partial (kind 0; x(0) = 0) for i's stop test.
+template <bool GROUPED>
+__global__ __launch_bounds__(256) void k_bxfold(int64_t n1, int B, int ck,
+                                                double* __restrict__ x,
+                                                const float* __restrict__ p,
+                                                double* __restrict__ part,
+                                                const int32_t* __restrict__ cst,
+                                                const BCol* __restrict__ bc) {
+  __shared__ double lds[GROUPED ? 256 : 1];
+  const ChunkIdx c = chunk_of(n1, B, ck);
+  bool act = c.valid && cst[c.j] == BCOL_INNER;
+  double mx = 0.0;
+  if (act) {
+    const BCol& bj = bc[c.j];
+    const int i = bj.it + 1;
+    // a column off the x-writing phase writes nothing here; the INNER
+    // finalize reports it
+    act = (i & (bj.xlen - 1)) == 0;
+    const int np = bj.npend + 1;
+    const int64_t nv = n1 * B;
+    double al[8];
+    uint32_t pk[8];  // buffer offsets, pmask n1 B < 2^32
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int it = i - np + 1 + (k < np ? k : 0);
+      al[k] = bj.alpha_hist[it & 7];
+      pk[k] = (uint32_t)(it & bj.pmask) * (uint32_t)nv;
+    }
+    for (int64_t g = act ? c.g0 : c.g1; g < c.g1; g++) {
+      const int64_t q = g * B + c.j;
+      double xg = 0.0;
+      if (g != 0) {
+        xg = x[q];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+          if (k < np) xg = xg + al[k] * (double)p[pk[k] + q];
+      }
+      x[q] = xg;
+      mx = fmax(mx, fabs(xg));
+    }
+  }
+  put_partials<GROUPED, 1>(part, lds, nchunks(n1, ck), B, c, {0}, {mx}, {true}, {act});
 }
 
 // At a residual step: INNER columns due for a replacement become REPLACE
@@ -631,6 +732,7 @@ __global__ __launch_bounds__(64) void k_bpromote(int B, int32_t* __restrict__ cs
 // order, FP64; Y64 holds the residual list's columns at their list places);
 // r = (float) r64; START: p_1 = D r; partials r.(D r) and (START) max|x_0|,
 // and for START the first iteration's max|p_1|
+template <bool GROUPED>
 __global__ __launch_bounds__(256) void k_bresid(
     int64_t n1, int B, int ck, const int32_t* __restrict__ rowptr,
     const int32_t* __restrict__ slots, const double* __restrict__ Y64,
@@ -638,14 +740,14 @@ __global__ __launch_bounds__(256) void k_bresid(
     const double* __restrict__ x, float* __restrict__ r, float* __restrict__ p,
     double* __restrict__ part, const int32_t* __restrict__ cst,
     const int32_t* __restrict__ rslot) {
+  __shared__ double lds[GROUPED ? 3 * 256 : 1];
   const ChunkIdx c = chunk_of(n1, B, ck);
-  if (!c.valid) return;
-  const int st = cst[c.j];
-  if (st != BCOL_START && st != BCOL_REPLACE) return;
-  const int qs = rslot[c.j];
+  const int st = c.valid ? cst[c.j] : BCOL_OFF;
+  const bool act = st == BCOL_START || st == BCOL_REPLACE;
+  const int qs = act ? rslot[c.j] : 0;
   const int64_t nv = n1 * B;
   double s = 0.0, mx = 0.0, mp = 0.0;
-  for (int64_t g = c.g0; g < c.g1; g++) {
+  for (int64_t g = act ? c.g0 : c.g1; g < c.g1; g++) {
     const int64_t q = g * B + c.j;
     double kx = 0.0;
     if (g > 0)
@@ -663,18 +765,20 @@ __global__ __launch_bounds__(256) void k_bresid(
     s = s + (double)rg * (double)zg;
   }
   const int64_t nch = nchunks(n1, ck);
-  part[(0 * nch + c.chunk) * B + c.j] = s;
-  part[(1 * nch + c.chunk) * B + c.j] = mx;
-  if (st == BCOL_START) part[(2 * nch + c.chunk) * B + c.j] = mp;  // INNER kind 2
+  // kind 2 (max|p_1|, START columns) is the INNER reduction's third kind
+  put_partials<GROUPED, 3>(part, lds, nch, B, c, {0, 1, 2}, {s, mx, mp},
+                           {false, true, true}, {act, act, act && st == BCOL_START});
 }
 
-// Reductions of the chunk partials, in a fixed order independent of B:
-// stage 1 combines groups of G consecutive chunks per column (thread per
-// (group, column), column fastest: coalesced), stage 2 (one block per
-// column) combines the groups and advances the column's state machine.  G
-// depends only on the chunk count (batch_group): 64 on large systems, 16
-// when there are few chunks, so a small system's stage 1 is one round of
-// loads and its stage 2 stays short.
+// Reductions of the chunk partials, in a fixed order independent of B.  On
+// large systems stage 1 (k_bgroup, thread per (group, column), column
+// fastest: coalesced) combines groups of G consecutive chunks per column
+// (batch_group: 64, or 16 for ALPHA below 16,384 elements) and stage 2
+// (k_bfinalize, one block per column) combines the groups and advances the
+// column's state machine.  Small systems skip
+// stage 1: their vector kernels already wrote one partial per four chunks
+// (put_partials), and the ALPHA reduction of up to BFIN_DIRECT_ELEMENTS
+// elements reads the per-element shares directly.
 //   RESID: chunks of equations; r.z of the FP64 residual, max|x| (START).
 //   ALPHA: the elements' p.Kp shares from the MFMA product (a "chunk" is an
 //          element).
@@ -730,29 +834,49 @@ __global__ __launch_bounds__(256) void k_bgroup(int64_t nch, int B, int G,
   }
 }
 
-__global__ __launch_bounds__(256) void k_bfinalize(int64_t nch, int B, int G,
-                                                   const double* __restrict__ part2,
+// `ng` values per column and kind in gp[(kind * ng + b) * B + j]: the
+// group partials of the two-stage form, or the partials written directly
+// by the kernels (grouped small systems; the element shares of ALPHA on
+// small meshes)
+__global__ __launch_bounds__(256) void k_bfinalize(int64_t ng, int B,
+                                                   const double* __restrict__ gp,
                                                    int32_t* __restrict__ cst,
                                                    BCol* __restrict__ bc,
-                                                   int what) {
-  __shared__ double red[4];
+                                                   int what, int xw) {
+  __shared__ double red[BFIN_KINDS * 4];
   const int j = blockIdx.x;
   const int st = cst[j];
   if (!wanted(cst, j, fin_states(what))) return;  // uniform per block
-  const int64_t ng = (nch + G - 1) / G;
   const int nk = fin_kinds(what);
+  // every kind in one pass (their loads overlap), each thread's values in
+  // group order, then the waves in order: the order block_sum / block_max
+  // take kind by kind
+  double a[BFIN_KINDS] = {0.0, 0.0, 0.0};
+  for (int64_t b = threadIdx.x; b < ng; b += 256) {
+#pragma unroll
+    for (int k = 0; k < BFIN_KINDS; k++) {
+      if (k < nk) {
+        const double y = gp[(k * ng + b) * B + j];
+        a[k] = fin_max(what, k) ? fmax(a[k], y) : a[k] + y;
+      }
+    }
+  }
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < BFIN_KINDS; k++) {
+    if (k < nk) {
+      const double t = fin_max(what, k) ? wave_max(a[k]) : wave_sum(a[k]);
+      if ((threadIdx.x & 63) == 0) red[k * 4 + w] = t;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   double v[BFIN_KINDS] = {0.0, 0.0, 0.0};
   for (int k = 0; k < nk; k++) {
     const bool mx = fin_max(what, k);
-    double a = 0.0;
-#pragma unroll 4
-    for (int64_t b = threadIdx.x; b < ng; b += 256) {
-      const double y = part2[(k * ng + b) * B + j];
-      a = mx ? fmax(a, y) : a + y;
-    }
-    v[k] = mx ? block_max<4>(a, red) : block_sum<4>(a, red);
+#pragma unroll
+    for (int i = 0; i < 4; i++) v[k] = mx ? fmax(v[k], red[k * 4 + i]) : v[k] + red[k * 4 + i];
   }
-  if (threadIdx.x != 0) return;
   BCol& c = bc[j];
   switch (what) {
     case BFIN_RESID:  // v0 = r.z of the FP64 residual, v1 = max|x| (START)
@@ -763,7 +887,8 @@ __global__ __launch_bounds__(256) void k_bfinalize(int64_t nch, int B, int G,
         c.it = 0;
         c.pending = 0;
         c.maxx = v[1];
-        c.xlag = 0;
+        c.xsum = 0.0;
+        c.npend = 0;
         cst[j] = (v[0] == 0.0) ? BCOL_DONE : BCOL_INNER;  // r = 0: x is exact
       } else {  // replaces r_{k+1}: beta = r.z (true) / r_k.z_k
         c.beta = v[0] / c.rz_prev;
@@ -774,29 +899,42 @@ __global__ __launch_bounds__(256) void k_bfinalize(int64_t nch, int B, int G,
       }
       break;
     case BFIN_ALPHA:  // v0 = p.Kp
-      c.alpha_prev = c.alpha;
       c.alpha = c.rz / v[0];
+      c.alpha_hist[(c.it + 1) & 7] = c.alpha;
       c.pending = 0;
       break;
     case BFIN_INNER: {  // v0 = max|x| (x written), v1 = r.z, v2 = max|p|
       c.it += 1;
       c.maxp = v[2];
-      const bool lag = c.xdefer && (c.it & 1);
-      bool stop;
-      if (!lag) {
-        // relative update test of src/solver.f90:65 (after the x update)
-        stop = fabs(c.alpha) * v[2] / v[0] <= c.tol;
-        c.maxx = v[0];
-      } else {
-        // x_it = x_{it-1} + alpha p_it is not written yet, but
-        // max|x_it| >= max|x_{it-1}| - a: a <= tol (max|x_{it-1}| - a)
-        // passes the same test for certain.  Only inside the window of
-        // relative width ~tol^2 where the bound cannot decide does the solve
-        // iterate once more than the test would.
-        const double a = fabs(c.alpha) * v[2];
-        stop = a <= c.tol * (c.maxx - a);
+      const double a = fabs(c.alpha) * v[2];
+      const bool xit = (c.it & (c.xlen - 1)) == 0;
+      if (xit != (xw != 0)) {
+        // the step's update and the column disagree on whether x was
+        // written (the engine's step phase and the column's iteration
+        // count went apart): the column ends, and the host reports it
+        c.xerr = 1;
+        cst[j] = BCOL_FAIL;
+        break;
       }
-      c.xlag = lag;
+      bool stop;
+      if (xit) {
+        // relative update test of src/solver.f90:65 (after the x update)
+        stop = a / v[0] <= c.tol;
+        c.maxx = v[0];
+        c.xsum = 0.0;
+        c.npend = 0;
+      } else {
+        // x_it = x_w + sum of alpha_k p_k over the iterations after the last
+        // write w is not written yet, but max|x_it| >= max|x_w| - xsum with
+        // xsum the sum of |alpha_k| max|p_k|: a <= tol (max|x_w| - xsum)
+        // passes the same test for certain.  Only inside the window of
+        // relative width ~tol xsum / max|x| (~xlen tol^2 near convergence)
+        // where the bound cannot decide does the solve iterate once more
+        // than the test would.
+        c.xsum += a;
+        stop = a <= c.tol * (c.maxx - c.xsum);
+        c.npend += 1;
+      }
       if (stop) {
         cst[j] = BCOL_DONE;
       } else if (c.it >= c.maxiter) {
@@ -945,6 +1083,8 @@ static inline int64_t nchunks_host(int64_t n1) {
 
 // stage-1 group size of the reductions (a function of the chunk count only)
 static inline int batch_group(int64_t nch) { return nch < 16384 ? 16 : 64; }
+// small systems: the vector kernels combine groups of four chunks themselves
+static inline bool batch_grouped(int64_t nch) { return batch_group(nch) == BGROUP_MIN; }
 
 int64_t batch_partials(int64_t n1) {
   const int64_t nch = nchunks_host(n1);
@@ -1008,23 +1148,40 @@ void launch_bmv_f64(hipStream_t s, int32_t nelmt, int B, int n_cu, const double*
 }
 
 void launch_bupdate(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
-                    const int32_t* slots, const float* Y, const float* p,
-                    const float* dinv, double* x, float* r, double* part,
-                    const int32_t* cst, const BCol* bc) {
+                    const int32_t* slots, const float* Y, const float* dinv, float* r,
+                    double* part, const int32_t* cst, const BCol* bc) {
   const int ck = batch_chunk(n1);
-  if (B == 64)
-    hipLaunchKernelGGL(k_bupdate<true>, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1,
-                       B, ck, rowptr, slots, Y, p, dinv, x, r, part, cst, bc);
+  const bool gr = batch_grouped(nchunks_host(n1));
+#define SGEO_UPD(W, G)                                                                  \
+  hipLaunchKernelGGL((k_bupdate<W, G>), dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, \
+                     B, ck, rowptr, slots, Y, dinv, r, part, cst, bc)
+  if (B == 64) {
+    if (gr) SGEO_UPD(true, true); else SGEO_UPD(true, false);
+  } else {
+    if (gr) SGEO_UPD(false, true); else SGEO_UPD(false, false);
+  }
+#undef SGEO_UPD
+}
+
+void launch_bxfold(hipStream_t s, int64_t n1, int B, double* x, const float* p,
+                   double* part, const int32_t* cst, const BCol* bc) {
+  if (batch_grouped(nchunks_host(n1)))
+    hipLaunchKernelGGL(k_bxfold<true>, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
+                       batch_chunk(n1), x, p, part, cst, bc);
   else
-    hipLaunchKernelGGL(k_bupdate<false>, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1,
-                       B, ck, rowptr, slots, Y, p, dinv, x, r, part, cst, bc);
+    hipLaunchKernelGGL(k_bxfold<false>, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
+                       batch_chunk(n1), x, p, part, cst, bc);
 }
 
 void launch_bpupdate(hipStream_t s, int64_t n1, int B, const float* r,
                      const float* dinv, float* p, double* part, const int32_t* cst,
                      const BCol* bc) {
-  hipLaunchKernelGGL(k_bpupdate, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
-                     batch_chunk(n1), r, dinv, p, part, cst, bc);
+  if (batch_grouped(nchunks_host(n1)))
+    hipLaunchKernelGGL(k_bpupdate<true>, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
+                       batch_chunk(n1), r, dinv, p, part, cst, bc);
+  else
+    hipLaunchKernelGGL(k_bpupdate<false>, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1,
+                       B, batch_chunk(n1), r, dinv, p, part, cst, bc);
 }
 
 void launch_bpromote(hipStream_t s, int B, int32_t* cst, BCol* bc, int32_t* rlist,
@@ -1036,9 +1193,14 @@ void launch_bresid(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
                    const int32_t* slots, const double* Y64, const double* load,
                    const float* dinv, const double* x, float* r, float* p,
                    double* part, const int32_t* cst, const int32_t* rslot) {
-  hipLaunchKernelGGL(k_bresid, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
-                     batch_chunk(n1), rowptr, slots, Y64, load, dinv, x, r, p, part,
-                     cst, rslot);
+  if (batch_grouped(nchunks_host(n1)))
+    hipLaunchKernelGGL(k_bresid<true>, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
+                       batch_chunk(n1), rowptr, slots, Y64, load, dinv, x, r, p, part, cst,
+                       rslot);
+  else
+    hipLaunchKernelGGL(k_bresid<false>, dim3(chunk_blocks(n1, B)), dim3(256), 0, s, n1, B,
+                       batch_chunk(n1), rowptr, slots, Y64, load, dinv, x, r, p, part, cst,
+                       rslot);
 }
 
 void launch_bmaterialize(hipStream_t s, int64_t n1, int B, double* x, const float* p,
@@ -1048,10 +1210,22 @@ void launch_bmaterialize(hipStream_t s, int64_t n1, int B, double* x, const floa
   hipLaunchKernelGGL(k_bxlag_clear, dim3((B + 63) / 64), dim3(64), 0, s, B, cst, bc);
 }
 
+#define BFIN_DIRECT_ELEMENTS 4096
+
 // `nch` partials per column and kind in `part` (equation chunks, or elements
-// for ALPHA); the group partials follow them
+// for ALPHA); the group partials follow them.  Small systems: one launch,
+// the final stage reading what the kernels wrote (groups of four chunks, or
+// for ALPHA the element shares of up to BFIN_DIRECT_ELEMENTS elements).
+// Large ones: the coalesced group stage, then the final stage.
 void launch_bfinalize(hipStream_t s, int64_t nch, int B, double* part,
-                      int32_t* cst, BCol* bc, int what) {
+                      int32_t* cst, BCol* bc, int what, int xw) {
+  const bool direct = what == BFIN_ALPHA ? nch <= BFIN_DIRECT_ELEMENTS : batch_grouped(nch);
+  if (direct) {
+    const int64_t ng = what == BFIN_ALPHA ? nch : (nch + 3) / 4;
+    hipLaunchKernelGGL(k_bfinalize, dim3(B), dim3(256), 0, s, ng, B, part, cst, bc, what,
+                       xw);
+    return;
+  }
   const int G = batch_group(nch);
   const int64_t ng = (nch + G - 1) / G;
   // group partials after the chunk partials: for the equation-chunk
@@ -1060,8 +1234,8 @@ void launch_bfinalize(hipStream_t s, int64_t nch, int B, double* part,
   double* part2 = part + (what == BFIN_ALPHA ? 1 : BFIN_KINDS) * nch * B;
   hipLaunchKernelGGL(k_bgroup, dim3((unsigned)((ng * B + 255) / 256)), dim3(256), 0,
                      s, nch, B, G, part, part2, cst, what);
-  hipLaunchKernelGGL(k_bfinalize, dim3(B), dim3(256), 0, s, nch, B, G, part2, cst, bc,
-                     what);
+  hipLaunchKernelGGL(k_bfinalize, dim3(B), dim3(256), 0, s, ng, B, part2, cst, bc, what,
+                     xw);
 }
 
 void launch_bload(hipStream_t s, int64_t n1, int B, const double* ext,

@@ -1463,7 +1463,14 @@ struct sgeo_batch {
   // own iterations
   int resid_period = 1;
   int64_t gstep = 0;
-  bool defer_x = true;  // odd iterations defer their x update (even periods)
+  // Deferred x: a column starts on a residual step and iterates once on
+  // every step until it ends, so its iteration i runs on a step with
+  // gstep + 1 = i (mod resid_period), the same phase for every column.  x
+  // is then written on the steps with (gstep + 1) % xlen == 0 only (xlen =
+  // batch_xlen, a divisor of the period), and is current on every residual
+  // step.  pring search-direction buffers hold the iterations not yet in x.
+  bool defer_x = true;
+  int pring = 2;  // p buffers: 8, or 2 when 8 n1 B reaches 2^32
   int32_t* h_stage = nullptr;  // pinned staging of the small uploads
   unsigned stage_busy = 0;
   double step_us = 0.0;  // running mean of the wall time of one step
@@ -1516,11 +1523,12 @@ extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extlo
   b->B = ncol;
   b->n1 = (int64_t)m->neq + 1;
   const size_t nv = (size_t)b->n1 * ncol;
-  // the element products address both search-direction buffers with 32-bit
-  // offsets (2 n1 B entries)
+  // the element products address the search-direction buffers with 32-bit
+  // offsets (pring n1 B entries, pring 8 or at least 2)
   if (2 * (uint64_t)nv >= (1ull << 32))
     return fail(SGEO_EARG, "sgeo_batch_create: %lld equations x %d columns too large",
                 (long long)b->n1, ncol);
+  b->pring = 8 * (uint64_t)nv < (1ull << 32) ? 8 : 2;
   const size_t ny = (size_t)m->nelmt * 81 * ncol;
   const size_t ns = (size_t)m->nelmt * 27 * 6 * ncol;
   size_t* acc = &b->bytes;
@@ -1532,7 +1540,7 @@ extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extlo
   TRY(dalloc(&b->d_ext, (size_t)b->n1, acc));
   TRY(dalloc(&b->d_xel, (size_t)b->n1, acc));
   TRY(dalloc(&b->d_r, nv, acc));
-  TRY(dalloc(&b->d_p, 2 * nv, acc));  // search directions of even and odd iterations
+  TRY(dalloc(&b->d_p, (size_t)b->pring * nv, acc));  // search directions, i & (pring - 1)
   TRY(dalloc(&b->d_Y32, ny, acc));
   TRY(dalloc(&b->d_dinv32, (size_t)b->n1, acc));
   TRY(dalloc(&b->d_evpt, ns, acc));
@@ -1581,6 +1589,11 @@ extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extlo
   HIPCHK(hipStreamSynchronize(s));
   *out = guard.release();
   return 0;
+}
+
+// steps between x writes (1: every step); a power of 2 dividing the period
+static int batch_xlen(const sgeo_batch* b) {
+  return b->defer_x ? std::min(b->resid_period, b->pring) : 1;
 }
 
 static void batch_drop_graphs(sgeo_batch* b) {
@@ -1632,7 +1645,12 @@ extern "C" int sgeo_batch_set_residual_period(sgeo_batch* b, int32_t period) {
 
 extern "C" int sgeo_batch_set_deferred_x(sgeo_batch* b, int32_t enable) {
   if (!b) return fail(SGEO_EARG, "null batch");
-  b->defer_x = enable != 0;  // read when a PCG call starts
+  const bool on = enable != 0;
+  for (int j = 0; j < b->B; j++)
+    if (b->phase[j] == BPH_RUN && on != b->defer_x)
+      return fail(SGEO_ESTATE, "sgeo_batch_set_deferred_x: trials are running");
+  if (on != b->defer_x) batch_drop_graphs(b);  // the graphs hold the x-writing steps
+  b->defer_x = on;
   return 0;
 }
 
@@ -1924,9 +1942,8 @@ static int batch_start(sgeo_batch* b, const std::vector<int32_t>& cols, double c
     bc.tol = cg_tol;
     bc.inner_tol = inner_tol;
     bc.maxiter = cg_maxiter;
-    // residual steps fall on even iterations only with an even period:
-    // then odd iterations may defer their x update to the next one
-    bc.xdefer = b->defer_x && b->resid_period % 2 == 0 ? 1 : 0;
+    bc.xlen = batch_xlen(b);
+    bc.pmask = b->pring - 1;
     bc.it = skip ? b->seed_cg[j] : 0;
     b->need_start[j] = 0;
     b->last_it[j] = bc.it;
@@ -1945,7 +1962,7 @@ static bool pcg_running(int st) {
 // With `resid` (a residual step), the step first forms the FP64 residual of
 // the columns that start a call or are due for a replacement (a no-op
 // launch when there are none).
-static void batch_step_launches(sgeo_batch* b, bool prof, bool resid) {
+static void batch_step_launches(sgeo_batch* b, bool prof, bool resid, bool xw) {
   sgeo_mesh* m = b->mesh;
   sgeo_ctx* c = m->ctx;
   hipStream_t s = c->stream;
@@ -1967,10 +1984,12 @@ static void batch_step_launches(sgeo_batch* b, bool prof, bool resid) {
                       n1 * B, b->d_Y32, b->d_edot, b->d_cst, b->d_bc, BCOL_BIT(BCOL_INNER));
   if (prof) prof_mark(c);
   launch_bfinalize(s, m->nelmt, B, b->d_edot, b->d_cst, b->d_bc, BFIN_ALPHA);
-  // kp assembled from Y; x, r, r.z, max|x|, max|p|; the stop test
-  launch_bupdate(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y32, b->d_p, b->d_dinv32,
-                 b->d_x, b->d_r, b->d_part, b->d_cst, b->d_bc);
-  launch_bfinalize(s, nch, B, b->d_part, b->d_cst, b->d_bc, BFIN_INNER);
+  // kp assembled from Y; r and r.z; on the x-writing steps x and max|x|;
+  // the stop test
+  launch_bupdate(s, n1, B, m->d_rowptr, m->d_slots, b->d_Y32, b->d_dinv32, b->d_r,
+                 b->d_part, b->d_cst, b->d_bc);
+  if (xw) launch_bxfold(s, n1, B, b->d_x, b->d_p, b->d_part, b->d_cst, b->d_bc);
+  launch_bfinalize(s, nch, B, b->d_part, b->d_cst, b->d_bc, BFIN_INNER, xw ? 1 : 0);
 }
 
 // `chunk` lockstep CG steps of every running column (FP64 residuals on the
@@ -1986,10 +2005,12 @@ static int batch_steps(sgeo_batch* b, int chunk, bool* any_running) {
   const auto t0 = std::chrono::steady_clock::now();
   const int gi = chunk == 8 ? 0 : chunk == 16 ? 1 : chunk == 32 ? 2 : -1;
   const int per = b->resid_period;
+  const int xlen = batch_xlen(b);  // divides per
   if (b->use_graphs && !c->prof && gi >= 0 && b->gstep % per == 0) {
     if (!b->gexec[gi]) {
       HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      for (int it = 0; it < chunk; it++) batch_step_launches(b, false, it % per == 0);
+      for (int it = 0; it < chunk; it++)
+        batch_step_launches(b, false, it % per == 0, (it + 1) % xlen == 0);
       hipGraph_t g = nullptr;
       const hipError_t ec = hipStreamEndCapture(s, &g);
       if (ec != hipSuccess || !g) return fail(SGEO_EHIP, "graph capture failed");
@@ -1999,7 +2020,8 @@ static int batch_steps(sgeo_batch* b, int chunk, bool* any_running) {
     HIPCHK(hipGraphLaunch(b->gexec[gi], s));
   } else {
     for (int it = 0; it < chunk; it++)
-      batch_step_launches(b, c->prof, (b->gstep + it) % per == 0);
+      batch_step_launches(b, c->prof, (b->gstep + it) % per == 0,
+                          (b->gstep + it + 1) % xlen == 0);
   }
   b->gstep += chunk;
   b->stat_steps += chunk;
@@ -2023,6 +2045,9 @@ static int batch_steps(sgeo_batch* b, int chunk, bool* any_running) {
     b->last_it[j] = b->h_bc[j].it;
     b->last_rep[j] = b->h_bc[j].nrep;
     b->cst[j] = b->h_cst[j];
+    if (b->h_bc[j].xerr)
+      return fail(SGEO_ESTATE, "batch column %d: x update out of phase with its iterations",
+                  j);
     running = running || pcg_running(b->cst[j]);
   }
   *any_running = running;

@@ -140,24 +140,29 @@ enum BColState {
 #define BCOL_BIT(s) (1 << (s))
 enum BFinWhat { BFIN_RESID = 0, BFIN_ALPHA = 1, BFIN_INNER = 2 };
 
-// With xdefer, the x update of every odd iteration is deferred and applied
-// with the next one, x_i = (x_{i-2} + alpha_{i-1} p_{i-1}) + alpha_i p_i (the
-// same FP64 operations in the same order as one update per iteration); the
-// search direction of iteration i lives in p buffer i & 1.
+// x is written on the iterations i with i % xlen == 0 only (xlen 1, 2, 4 or
+// 8); such an iteration applies the deferred updates in order,
+//   x_i = (...((x_{i-L} + alpha_{i-L+1} p_{i-L+1}) + ...) + alpha_i p_i,
+// the same FP64 operations in the same order as one update per iteration.
+// The search direction of iteration i lives in p buffer i & pmask (pmask + 1
+// buffers, at least xlen of them).
 struct BCol {  // per-column scalars, device resident
   double rz;        // r.z of the current residual
   double rz0;       // r.z at the last FP64 residual
   double alpha, beta, maxp;
   double tol, inner_tol;
-  double rz_prev;   // r.z of the previous iteration (beta after a replacement)
-  double alpha_prev;  // alpha of the previous iteration
-  double maxx;        // max|x| of the last x written (exact)
+  double rz_prev;        // r.z of the previous iteration (beta after a replacement)
+  double alpha_hist[8];  // alpha of iteration k at [k & 7]
+  double maxx;           // max|x| of the last x written (exact)
+  double xsum;           // sum of |alpha_k| max|p_k| over the iterations not in x
   int32_t it, maxiter;  // CG iterations of this solve, limit
   int32_t pending;      // 1: p = D r + beta p before the next product
   int32_t nrep;         // FP64 residuals of this solve
   int32_t want_rep;     // 1: replace r at the next residual step
-  int32_t xdefer;       // 1: odd iterations defer their x update
-  int32_t xlag;         // 1: x lacks alpha p of the last (odd) iteration
+  int32_t xlen;         // x written on iterations i % xlen == 0
+  int32_t npend;        // iterations not yet in x (0: x is current)
+  int32_t pmask;        // p buffer of iteration i: i & pmask
+  int32_t xerr;         // 1: a step's x write and the iteration count disagreed
 };
 
 // GLL numbering of a hex8 mesh, built on the device (sgeo_ingest.hip)
@@ -192,9 +197,10 @@ void launch_bmv_f64(hipStream_t s, int32_t nelmt, int B, int n_cu, const double*
                     const int32_t* gdof, const double* X, double* Y,
                     const int32_t* rlist);
 void launch_bupdate(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
-                    const int32_t* slots, const float* Y, const float* p,
-                    const float* dinv, double* x, float* r, double* part,
-                    const int32_t* cst, const BCol* bc);
+                    const int32_t* slots, const float* Y, const float* dinv, float* r,
+                    double* part, const int32_t* cst, const BCol* bc);
+void launch_bxfold(hipStream_t s, int64_t n1, int B, double* x, const float* p,
+                   double* part, const int32_t* cst, const BCol* bc);
 void launch_bpupdate(hipStream_t s, int64_t n1, int B, const float* r,
                      const float* dinv, float* p, double* part, const int32_t* cst,
                      const BCol* bc);
@@ -206,8 +212,9 @@ void launch_bresid(hipStream_t s, int64_t n1, int B, const int32_t* rowptr,
                    const int32_t* slots, const double* Y64, const double* load,
                    const float* dinv, const double* x, float* r, float* p,
                    double* part, const int32_t* cst, const int32_t* rslot);
+// xw (INNER only): this step's update wrote x (checked against each column)
 void launch_bfinalize(hipStream_t s, int64_t nch, int B, double* part,
-                      int32_t* cst, BCol* bc, int what);
+                      int32_t* cst, BCol* bc, int what, int xw = 0);
 void launch_bload(hipStream_t s, int64_t n1, int B, const double* ext,
                   const double* body, double* load, const int32_t* active);
 void launch_buerr(hipStream_t s, int64_t n1, int B, double* x, double* oldx,

@@ -59,7 +59,8 @@ def footprint_bytes(kind: str, nelmt: int, ncol: int = 8) -> int:
     - Stiffness: FP64 full + packed FP64 + packed FP32, and the element
       product buffers.
     - Slope, mixed multi-trial form with `ncol` columns: interleaved vectors
-      (about 3.7 equations per element, 4 FP64 + 3 FP32 per column), element
+      (about 3.7 equations per element, 4 FP64 + 9 FP32 per column: x, oldx,
+      load and body; r and a ring of 8 search directions), element
       products (FP32 and FP64), per-point state (evpt and stress, FP64),
       body loads.
     - Excavation: one FP64 trial (x, oldx, load, body, ext, PCG vectors) and
@@ -70,7 +71,7 @@ def footprint_bytes(kind: str, nelmt: int, ncol: int = 8) -> int:
     neq = int(3.7 * nelmt) + 1
     k = nelmt * (6561 * 8 + 3328 * 12 + 81 * (8 + 4 + 4)) + neq * 8 * 6
     if kind == "slope":
-        st = ncol * (neq * (4 * 8 + 3 * 4) + nelmt * 81 * (4 + 8 + 8)
+        st = ncol * (neq * (4 * 8 + 9 * 4) + nelmt * 81 * (4 + 8 + 8)
                      + nelmt * 27 * (6 * 8 * 2 + 1))
     else:
         st = neq * 8 * 10 + nelmt * (27 * (6 * 8 * 2 + 1 + 8) + 81 * 8)

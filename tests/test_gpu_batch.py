@@ -90,9 +90,10 @@ def test_batch_trial_identical_alone_and_in_mix(ctx, period):
 
 
 def test_residual_period_arguments(ctx):
-    """The residual period is 1, 2, 4 or 8, and cannot change while trials
-    run (their residual steps fall at fixed counts of their own
-    iterations); setting the period in force again is accepted."""
+    """The residual period is 1, 2, 4 or 8, and neither it nor the x
+    deferral can change while trials run (their residual steps and x writes
+    fall at fixed counts of their own iterations); setting the value in
+    force again is accepted."""
     from specfem3d_geotech_amd._lib import SgeoError
     from specfem3d_geotech_amd.plastic import dt_viscoplas
     z = golden("slope_4x2x3")
@@ -113,19 +114,24 @@ def test_residual_period_arguments(ctx):
             with pytest.raises(SgeoError):
                 eng.set_residual_period(4)
             eng.set_residual_period(8)
+            with pytest.raises(SgeoError):
+                eng.set_deferred_x(False)
+            eng.set_deferred_x(True)
         finally:
             eng.close()
     finally:
         dm.close()
 
 
-def test_deferred_x_update_keeps_the_bits(ctx):
-    """With FP64 residuals every 8th step, odd iterations defer their x
-    update and the next iteration applies both in the same order; solves
-    that end on an odd iteration are completed before the body pass.  The
-    rows and x of a 4-column sweep are the same bits with the deferral on
-    and off (the stop test on a deferred iteration could only differ inside
-    a window of relative width cg_tol^2 around its threshold)."""
+@pytest.mark.parametrize("period", [2, 8])
+def test_deferred_x_update_keeps_the_bits(ctx, period):
+    """With FP64 residuals every `period`-th step, x is written on every
+    period-th iteration only, which applies the updates of the iterations
+    before it in their order; solves that end between two writes are
+    completed before the body pass.  The rows and x of a 4-column sweep are
+    the same bits with the deferral on and off (the stop test on a deferred
+    iteration could only differ inside a window of relative width about
+    period cg_tol^2 around its threshold)."""
     z = golden("slope_8x4x6_layered")
     mesh = mesh_from_golden(z)
     spec = hex2spec(mesh)
@@ -138,7 +144,7 @@ def test_deferred_x_update_keeps_the_bits(ctx):
     try:
         for defer in (True, False):
             an = _analyses(mesh, 4, [1.0, 1.3, 1.6], seed=5)
-            eng = BatchEngine(dm, ext, 4, resid_period=8)
+            eng = BatchEngine(dm, ext, 4, resid_period=period)
             try:
                 eng.set_deferred_x(defer)
                 run_analyses(eng, an, **kw)
