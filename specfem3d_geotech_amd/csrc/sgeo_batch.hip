@@ -848,9 +848,8 @@ __global__ __launch_bounds__(256) void k_bfinalize(int64_t ng, int B,
   const int st = cst[j];
   if (!wanted(cst, j, fin_states(what))) return;  // uniform per block
   const int nk = fin_kinds(what);
-  // every kind in one pass (their loads overlap), each thread's values in
-  // group order, then the waves in order: the order block_sum / block_max
-  // take kind by kind
+  // every kind in one pass (their loads overlap); per kind, each thread's
+  // values in group order, then a wave tree, then the four waves in order
   double a[BFIN_KINDS] = {0.0, 0.0, 0.0};
   for (int64_t b = threadIdx.x; b < ng; b += 256) {
 #pragma unroll

@@ -2063,7 +2063,7 @@ static int batch_body(sgeo_batch* b, const std::vector<int32_t>& cols, double nl
   sgeo_mesh* m = b->mesh;
   hipStream_t s = m->ctx->stream;
   const int B = b->B;
-  // x of the solves that ended on a deferred (odd) iteration
+  // x of the solves that ended between two x writes
   launch_bmaterialize(s, b->n1, B, b->d_x, b->d_p, b->d_cst, b->d_bc);
   for (int j : cols) {  // first PCG call of a trial from x = 0: the elastic memo
     if (!b->memo_valid && b->nl_iter[j] == 0 && !b->seeded[j] && b->cst[j] == BCOL_DONE) {
