@@ -387,14 +387,14 @@ int sgeo_batch_set_graphs(sgeo_batch *batch, int enable);
 /* Launch-geometry autotuner of the MFMA element product (no counterpart in
  * the reference, which has no GPU product).  Times `reps` launches of each
  * candidate on this engine's mesh and iterating columns -- blocks per CU
- * 3, 2, 1 with the XCD-aware element split, then 3, 2, 1 with a plain
- * grid-stride walk (ms_table[6], in that order, may be NULL) -- and keeps the
+ * 4, 3, 2, 1 with the XCD-aware element split, then 4, 3, 2, 1 with a plain
+ * grid-stride walk (ms_table[8], in that order, may be NULL) -- and keeps the
  * fastest for later steps.  Needs at least one column iterating
  * (SGEO_ESTATE otherwise).  The geometry never changes a result bit, and
  * tuning between steps does not change the trials' results. */
 int sgeo_batch_autotune(sgeo_batch *batch, int32_t reps, int32_t *blocks_per_cu,
                         int32_t *xcd_split, double *ms_best, double *ms_table);
-/* Sets that geometry directly (blocks per CU 1..3, XCD split 0/1); the
+/* Sets that geometry directly (blocks per CU 1..4, XCD split 0/1); the
  * default is 3 blocks per CU with the XCD split. */
 int sgeo_batch_set_geometry(sgeo_batch *batch, int32_t blocks_per_cu, int32_t xcd_split);
 /* FP64 residuals (the product on the packed FP64 operator that starts a

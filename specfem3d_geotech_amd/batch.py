@@ -99,11 +99,11 @@ class BatchEngine:
         columns iterating now (at least one must be); keeps the fastest."""
         bpc, xs = C.c_int32(), C.c_int32()
         best = C.c_double()
-        table = np.zeros(6)
+        table = np.zeros(8)
         check(L.lib().sgeo_batch_autotune(self.h, int(reps), C.byref(bpc), C.byref(xs),
                                           C.byref(best), ptr(table)), "sgeo_batch_autotune")
         cands = [dict(blocks_per_cu=b, xcd_split=x, ms=float(t))
-                 for (x, b), t in zip([(1, 3), (1, 2), (1, 1), (0, 3), (0, 2), (0, 1)], table)]
+                 for (x, b), t in zip([(x, b) for x in (1, 0) for b in (4, 3, 2, 1)], table)]
         return dict(blocks_per_cu=bpc.value, xcd_split=xs.value, ms=best.value,
                     candidates=cands)
 

@@ -92,10 +92,11 @@ __device__ __forceinline__ int tri_col(int q) {  // c with c(c+1)/2 <= q < (c+1)
 }
 
 // Persistent and software-pipelined: the grid holds a few blocks per CU.
-// With nxcd = 8, each XCD (blocks are dispatched to the 8 XCDs round-robin)
-// walks its own contiguous eighth of the elements, so elements in flight
-// together share nodes and their gathered P rows hit in that XCD's L2;
-// nxcd = 1 is a plain grid-stride walk.  While element e is multiplied, K
+// The elements are taken in the order eorder[] (spatially compact runs,
+// built with the engine).  With nxcd = 8, each XCD (blocks are dispatched
+// to the 8 XCDs round-robin) walks its own contiguous eighth of that order,
+// so elements in flight together share nodes and their gathered P rows hit
+// in that XCD's L2; nxcd = 1 is a plain grid-stride walk.  While element e is multiplied, K
 // and gdof of the block's next element are already in registers, and the
 // next element's P rows are gathered before its K is unpacked.  The grid
 // size and nxcd change which block computes an element, never how: the
@@ -107,6 +108,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
                                                   const float* __restrict__ KP,
                                                   const int32_t* __restrict__ gdof,
+                                                  const int32_t* __restrict__ eorder,
                                                   const float* __restrict__ P,
                                                   int64_t nv,
                                                   float* __restrict__ Y,
@@ -124,12 +126,14 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
   // come from several waves
   __shared__ double xs[NS > 1 ? NT : 1][NS > 1 ? NMT + 1 : 1][64];
   const int t = threadIdx.x;
-  // element range of this block's XCD, and the block's stride within it
+  // range of the element walk eorder[] of this block's XCD, and the block's
+  // stride within it; wi is the walk position, e = eorder[wi] the element
   const int xcd = blockIdx.x % nxcd;
   const int nbx = gridDim.x / nxcd;  // grid is a multiple of nxcd
   const int64_t xe0 = (int64_t)nelmt * xcd / nxcd;
   const int64_t xe1 = (int64_t)nelmt * (xcd + 1) / nxcd;
-  int64_t e = xe0 + blockIdx.x / nxcd;
+  int64_t wi = xe0 + blockIdx.x / nxcd;
+  int64_t e = wi < xe1 ? eorder[wi] : 0;
   // column tile of this wave; columns outside the mask are neither
   // gathered nor multiplied nor stored, but the wave keeps to the block's
   // barriers
@@ -163,7 +167,7 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
   // prologue: K and gdof of the first element
   float kreg[KPT];
   int32_t greg = 0;
-  if (e < xe1) {
+  if (wi < xe1) {
     const float* src = KP + e * SGEO_KPACK;
 #pragma unroll
     for (int i = 0; i < KPT; i++) {
@@ -175,12 +179,12 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
   if (t < 81) sg[0][t] = greg;
   __syncthreads();
   float bv[21];
-  if (wave_on && e < xe1) {
+  if (wave_on && wi < xe1) {
 #pragma unroll
     for (int s = 0; s < 21; s++) bv[s] = P[(uint32_t)(sg[0][4 * s + kq] * B + jl) + poff];
   }
   int cur = 0;
-  for (; e < xe1; e += nbx) {
+  for (; wi < xe1; wi += nbx) {
     // unpack the packed upper triangle (and its mirror) of element e
 #pragma unroll
     for (int i = 0; i < KPT; i++) {
@@ -190,8 +194,9 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
       }
     }
     __syncthreads();
-    const int64_t en = e + nbx;
-    if (en < xe1) {  // next element's K and gdof, in flight during the MFMAs
+    const int64_t wn = wi + nbx;
+    const int64_t en = wn < xe1 ? eorder[wn] : 0;
+    if (wn < xe1) {  // next element's K and gdof, in flight during the MFMAs
       const float* src = KP + en * SGEO_KPACK;
 #pragma unroll
       for (int i = 0; i < KPT; i++) {
@@ -293,11 +298,12 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
       dsum = dsum + __shfl_xor(dsum, 32);
       if (jv && kq == 0) edot[e * B + j] = dsum;
     }
-    if (wave_on && en < xe1) {
+    if (wave_on && wn < xe1) {
 #pragma unroll
       for (int s = 0; s < 21; s++)
         bv[s] = P[(uint32_t)(sg[cur][4 * s + kq] * B + jl) + poff];
     }
+    e = en;
   }
 }
 
@@ -1113,25 +1119,25 @@ static inline int bmv_threads(int B) {
 // the kernel is built for 3 waves per SIMD, at most 168 VGPRs) and the
 // XCD-aware element split (nxcd 8) or a plain grid-stride walk (nxcd 1).
 void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu, BmvGeom geom,
-                         const float* KP, const int32_t* gdof, const float* P,
-                         int64_t nv, float* Y, double* edot, const int32_t* cst,
-                         const BCol* bc, int want_mask) {
+                         const float* KP, const int32_t* gdof, const int32_t* eorder,
+                         const float* P, int64_t nv, float* Y, double* edot,
+                         const int32_t* cst, const BCol* bc, int want_mask) {
   const int nx = geom.xcd_split ? BMV_XCD : 1;
-  const int bpc = geom.blocks_per_cu < 1 ? 1 : geom.blocks_per_cu > 3 ? 3 : geom.blocks_per_cu;
+  const int bpc = geom.blocks_per_cu < 1 ? 1 : geom.blocks_per_cu > 4 ? 4 : geom.blocks_per_cu;
   int64_t g = (int64_t)n_cu * bpc;
   const int64_t need = ((int64_t)nelmt + nx - 1) / nx * nx;
   if (g > need) g = need;
   g = (g + nx - 1) / nx * nx;  // a multiple of the XCD count
   const dim3 grid((unsigned)g);
   if (B >= 64)
-    hipLaunchKernelGGL(k_bmv_mfma<4>, grid, dim3(256), 0, s, nelmt, B, nx, KP, gdof, P,
-                       nv, Y, edot, cst, bc, want_mask);
+    hipLaunchKernelGGL(k_bmv_mfma<4>, grid, dim3(256), 0, s, nelmt, B, nx, KP, gdof,
+                       eorder, P, nv, Y, edot, cst, bc, want_mask);
   else if (B == 32)
-    hipLaunchKernelGGL(k_bmv_mfma<2>, grid, dim3(256), 0, s, nelmt, B, nx, KP, gdof, P,
-                       nv, Y, edot, cst, bc, want_mask);
+    hipLaunchKernelGGL(k_bmv_mfma<2>, grid, dim3(256), 0, s, nelmt, B, nx, KP, gdof,
+                       eorder, P, nv, Y, edot, cst, bc, want_mask);
   else
-    hipLaunchKernelGGL(k_bmv_mfma<1>, grid, dim3(256), 0, s, nelmt, B, nx, KP, gdof, P,
-                       nv, Y, edot, cst, bc, want_mask);
+    hipLaunchKernelGGL(k_bmv_mfma<1>, grid, dim3(256), 0, s, nelmt, B, nx, KP, gdof,
+                       eorder, P, nv, Y, edot, cst, bc, want_mask);
 }
 
 // FP64 residual product: persistent, 2 blocks per CU (60 KB of LDS each);

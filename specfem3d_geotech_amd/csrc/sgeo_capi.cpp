@@ -1436,6 +1436,7 @@ struct sgeo_batch {
   // residual list of a residual step: columns [0..64), count at [64]; and
   // each listed column's place in it
   int32_t *d_rlist = nullptr, *d_rslot = nullptr;
+  int32_t* d_eorder = nullptr;  // the MFMA product's walk over the elements
   double* d_part = nullptr;  // equation-chunk partials and their groups
   double* d_edot = nullptr;  // the elements' p.Kp shares and their groups
   // per column
@@ -1508,6 +1509,53 @@ static int batch_sync_operator(sgeo_batch* b) {
   return 0;
 }
 
+// The order in which the MFMA product walks the elements: by the Morton
+// code of the element centroid on a grid of the same spacing on every axis,
+// so the elements a product has in flight together (and each XCD's
+// contiguous share of the walk) form compact regions whose gathered P rows
+// meet in L2 and the MALL.  Y and the p.Kp shares stay indexed by element,
+// so the order changes no result bit.
+static int batch_walk_order(sgeo_mesh* m, sgeo_batch* b) {
+  const int32_t ne = m->nelmt;
+  std::vector<double> cor((size_t)ne * 24);
+  HIPCHK(hipMemcpy(cor.data(), m->d_corners, cor.size() * 8, hipMemcpyDeviceToHost));
+  std::vector<double> cen((size_t)ne * 3, 0.0);
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int32_t e = 0; e < ne; e++)
+    for (int d = 0; d < 3; d++) {
+      double c = 0.0;
+      for (int k = 0; k < 8; k++) c += cor[(size_t)e * 24 + k * 3 + d];
+      c *= 0.125;
+      cen[(size_t)e * 3 + d] = c;
+      lo[d] = std::min(lo[d], c);
+      hi[d] = std::max(hi[d], c);
+    }
+  double ext = 0.0;
+  for (int d = 0; d < 3; d++) ext = std::max(ext, hi[d] - lo[d]);
+  const double scale = ext > 0.0 ? 2097151.0 / ext : 0.0;  // 21 bits per axis
+  auto spread = [](uint64_t v) {  // bit i -> bit 3 i
+    uint64_t r = 0;
+    for (int i = 0; i < 21; i++) r |= ((v >> i) & 1ull) << (3 * i);
+    return r;
+  };
+  std::vector<std::pair<uint64_t, int32_t>> key((size_t)ne);
+  for (int32_t e = 0; e < ne; e++) {
+    uint64_t code = 0;
+    for (int d = 0; d < 3; d++) {
+      const double q = std::floor((cen[(size_t)e * 3 + d] - lo[d]) * scale);
+      const uint64_t u = (uint64_t)std::min(2097151.0, std::max(0.0, q));
+      code |= spread(u) << d;
+    }
+    key[(size_t)e] = {code, e};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int32_t> order((size_t)ne);
+  for (int32_t i = 0; i < ne; i++) order[(size_t)i] = key[(size_t)i].second;
+  TRY(dalloc(&b->d_eorder, (size_t)ne, &b->bytes));
+  HIPCHK(hipMemcpy(b->d_eorder, order.data(), (size_t)ne * 4, hipMemcpyHostToDevice));
+  return 0;
+}
+
 extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extload,
                                  sgeo_batch** out) {
   if (!m || !extload || !out || m->kprec < 0 || !m->d_corners || !m->d_mat)
@@ -1558,6 +1606,7 @@ extern "C" int sgeo_batch_create(sgeo_mesh* m, int32_t ncol, const double* extlo
   TRY(dalloc(&b->d_rslot, (size_t)64, acc));
   TRY(dalloc(&b->d_part, (size_t)batch_partials(b->n1) * ncol, acc));
   TRY(dalloc(&b->d_edot, (size_t)batch_element_partials(m->nelmt) * ncol, acc));
+  TRY(batch_walk_order(m, b));
   HIPCHK(hipHostMalloc((void**)&b->h_ps, sizeof(PlasticScalars) * ncol));
   HIPCHK(hipHostMalloc((void**)&b->h_bc, sizeof(BCol) * ncol));
   HIPCHK(hipHostMalloc((void**)&b->h_cst, sizeof(int32_t) * ncol));
@@ -1614,7 +1663,7 @@ extern "C" int sgeo_batch_destroy(sgeo_batch* b) {
                   b->d_Y32,   b->d_dinv32, b->d_evpt,   b->d_stress, b->d_BL,
                   b->d_flags, b->d_mp,     b->d_ps,     b->d_bc,    b->d_cst,
                   b->d_mask,  b->d_islast, b->d_cols,   b->d_part,  b->d_vm,
-                  b->d_sf,    b->d_nacc,   b->d_rlist,  b->d_rslot};
+                  b->d_sf,    b->d_nacc,   b->d_rlist,  b->d_rslot, b->d_eorder};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (b->h_ps) hipHostFree(b->h_ps);
@@ -1656,8 +1705,8 @@ extern "C" int sgeo_batch_set_deferred_x(sgeo_batch* b, int32_t enable) {
 
 extern "C" int sgeo_batch_set_geometry(sgeo_batch* b, int32_t blocks_per_cu,
                                        int32_t xcd_split) {
-  if (!b || blocks_per_cu < 1 || blocks_per_cu > 3 || xcd_split < 0 || xcd_split > 1)
-    return fail(SGEO_EARG, "sgeo_batch_set_geometry: blocks_per_cu 1..3, xcd_split 0/1");
+  if (!b || blocks_per_cu < 1 || blocks_per_cu > 4 || xcd_split < 0 || xcd_split > 1)
+    return fail(SGEO_EARG, "sgeo_batch_set_geometry: blocks_per_cu 1..4, xcd_split 0/1");
   if (blocks_per_cu != b->geom.blocks_per_cu || xcd_split != b->geom.xcd_split)
     batch_drop_graphs(b);
   b->geom.blocks_per_cu = blocks_per_cu;
@@ -1689,14 +1738,14 @@ extern "C" int sgeo_batch_autotune(sgeo_batch* b, int32_t reps, int32_t* blocks_
   BmvGeom pick = b->geom;
   int k = 0;
   for (int xs = 1; xs >= 0; xs--) {
-    for (int bpc = 3; bpc >= 1; bpc--, k++) {
+    for (int bpc = 4; bpc >= 1; bpc--, k++) {
       BmvGeom g;
       g.blocks_per_cu = bpc;
       g.xcd_split = xs;
       auto launch = [&]() {
-        launch_bmv_mfma_f32(s, m->nelmt, b->B, c->n_cu, g, m->d_KP32, m->d_gdof, b->d_p,
-                            b->n1 * b->B, b->d_Y32, b->d_edot, b->d_cst, b->d_bc,
-                            BCOL_BIT(BCOL_INNER));
+        launch_bmv_mfma_f32(s, m->nelmt, b->B, c->n_cu, g, m->d_KP32, m->d_gdof,
+                            b->d_eorder, b->d_p, b->n1 * b->B, b->d_Y32, b->d_edot,
+                            b->d_cst, b->d_bc, BCOL_BIT(BCOL_INNER));
       };
       launch();  // warm-up
       HIPCHK(hipEventRecord(e0, s));
@@ -1980,8 +2029,9 @@ static void batch_step_launches(sgeo_batch* b, bool prof, bool resid, bool xw) {
   launch_bpupdate(s, n1, B, b->d_r, b->d_dinv32, b->d_p, b->d_part, b->d_cst, b->d_bc);
   if (prof) prof_mark(c);
   // Y = K P and the elements' p.Kp shares; alpha = r.z / p.Kp
-  launch_bmv_mfma_f32(s, m->nelmt, B, c->n_cu, b->geom, m->d_KP32, m->d_gdof, b->d_p,
-                      n1 * B, b->d_Y32, b->d_edot, b->d_cst, b->d_bc, BCOL_BIT(BCOL_INNER));
+  launch_bmv_mfma_f32(s, m->nelmt, B, c->n_cu, b->geom, m->d_KP32, m->d_gdof, b->d_eorder,
+                      b->d_p, n1 * B, b->d_Y32, b->d_edot, b->d_cst, b->d_bc,
+                      BCOL_BIT(BCOL_INNER));
   if (prof) prof_mark(c);
   launch_bfinalize(s, m->nelmt, B, b->d_edot, b->d_cst, b->d_bc, BFIN_ALPHA);
   // kp assembled from Y; r and r.z; on the x-writing steps x and max|x|;

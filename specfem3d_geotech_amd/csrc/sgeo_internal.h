@@ -189,8 +189,10 @@ struct BmvGeom {
 int64_t batch_partials(int64_t n1);          // per column, equation-chunk partials
 int64_t batch_element_partials(int32_t nelmt);  // per column, element p.Kp shares
 int64_t batch_equation_chunks(int64_t n1);
+// eorder: the walk over the elements (a permutation of 0 .. nelmt - 1)
 void launch_bmv_mfma_f32(hipStream_t s, int32_t nelmt, int B, int n_cu, BmvGeom geom,
-                         const float* KP, const int32_t* gdof, const float* P,
+                         const float* KP, const int32_t* gdof, const int32_t* eorder,
+                         const float* P,
                          int64_t nv, float* Y, double* edot, const int32_t* cst,
                          const BCol* bc, int want_mask);
 void launch_bmv_f64(hipStream_t s, int32_t nelmt, int B, int n_cu, const double* KP,

@@ -344,8 +344,9 @@ def test_graph_steps_match_eager_launches(ctx):
 
 def test_launch_geometry_does_not_change_bits(ctx):
     """The MFMA product's launch geometry (blocks per CU, the XCD-aware
-    element split) changes which block computes an element, never how: a
-    64-column engine gives the same rows and displacement bits under each."""
+    split of its element walk) changes which block computes an element,
+    never how: a 64-column engine gives the same rows and displacement bits
+    under each."""
     z = golden("slope_8x4x6_layered")
     mesh = mesh_from_golden(z)
     spec = hex2spec(mesh)
@@ -356,7 +357,7 @@ def test_launch_geometry_does_not_change_bits(ctx):
     out = []
     dm, ext = device_mesh(ctx, mesh, spec)
     try:
-        for geom in [(3, True), (1, False), (2, True)]:
+        for geom in [(3, True), (1, False), (2, True), (4, False)]:
             an = _analyses(mesh, 20, [1.0, 1.5], seed=11)
             eng = BatchEngine(dm, ext, 64)
             try:
