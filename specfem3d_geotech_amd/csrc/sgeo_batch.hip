@@ -585,20 +585,60 @@ __global__ void k_bxlag_clear(int B, const int32_t* __restrict__ cst,
 // neither x nor p.  The (at most 8 on hexahedral meshes) element
 // contributions of an equation are loaded together, then summed in slot
 // order; UPD_EQ equations per trip keep that many gathers in flight.
-// WAVE_CHUNK: B = 64, so a wave's 64 lanes are the 64 columns of one chunk
-// and the index loads (rowptr, slots) are wave-uniform scalar loads.
+// WAVE_CHUNK: B = 64, so a wave's 64 lanes are the 64 columns of one chunk;
+// the chunk's row pointers and slots are first staged in the wave's LDS rows
+// (stage_indices), so each gather waits on the Y load only, not on a chain
+// of row-pointer and slot loads before it.
 #ifndef UPD_EQ
 #define UPD_EQ 4
 #endif
-// the body of k_bupdate for one active (chunk, column)
-template <bool WAVE_CHUNK>
+#define UPD_SL 512  // staged slots per wave (64 equations, 8 slots each)
+// WAVE_CHUNK: the wave's chunk's row pointers rowptr[ga .. g1] (ga = max(g0,
+// 1): equation 0, the sink, has no update) and its slots, one coalesced load
+// each, into the wave's LDS rows rp[65], sl[UPD_SL], so the update's gathers
+// wait on one memory level instead of three.  All 64 lanes call it; false
+// when the slots do not fit (the update then reads the indices from memory).
+__device__ __forceinline__ bool stage_indices(const ChunkIdx& c,
+                                              const int32_t* __restrict__ rowptr,
+                                              const int32_t* __restrict__ slots,
+                                              int32_t* rp, int32_t* sl) {
+  const int ln = threadIdx.x & 63;
+  const int64_t ga = c.g0 > 0 ? c.g0 : 1;
+  const int nr = (int)(c.g1 - ga) + 1;  // <= 65
+  if (ln < nr) rp[ln] = rowptr[ga + ln];
+  if (ln == 0 && nr > 64) rp[64] = rowptr[ga + 64];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int32_t sb = rp[0], nsl = rp[nr - 1] - sb;
+  if (nsl > UPD_SL) return false;
+  for (int k = ln; k < nsl; k += 64) sl[k] = slots[sb + k];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return true;
+}
+
+// the body of k_bupdate for one active (chunk, column); rp / sl: the staged
+// indices (null: read them from memory)
+template <bool WAVE_CHUNK, bool STAGED>
 __device__ __forceinline__ void update_chunk(
     int64_t n1, int B, int ck, const ChunkIdx& c, const int32_t* __restrict__ rowptr,
     const int32_t* __restrict__ slots, const float* __restrict__ Y,
     const float* __restrict__ dinv, float* __restrict__ r, const BCol* __restrict__ bc,
-    double& s) {
+    double& s, const int32_t* rp, const int32_t* sl) {
   const float alpha32 = (float)bc[c.j].alpha;
+  const int64_t ga = c.g0 > 0 ? c.g0 : 1;
+  const int32_t sb = STAGED ? rp[0] : 0;
   auto gather = [&](int64_t g, float* v, int32_t& q0, int32_t& q1) {
+    if constexpr (STAGED) {
+      q0 = g > 0 ? rp[g - ga] : 0;
+      q1 = g > 0 ? rp[g - ga + 1] : 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        v[k] = (q0 + k < q1) ? Y[(int64_t)sl[q0 + k - sb] * B + c.j] : 0.0f;
+      return;
+    }
     q0 = g > 0 ? rowptr[g] : 0;
     q1 = g > 0 ? rowptr[g + 1] : 0;
 #pragma unroll
@@ -610,7 +650,8 @@ __device__ __forceinline__ void update_chunk(
 #pragma unroll
     for (int k = 0; k < 8; k++)
       if (q0 + k < q1) kp = kp + v[k];
-    for (int32_t q = q0 + 8; q < q1; q++) kp = kp + Y[(int64_t)slots[q] * B + c.j];
+    for (int32_t q = q0 + 8; q < q1; q++)
+      kp = kp + Y[(int64_t)(STAGED ? sl[q - sb] : slots[q]) * B + c.j];
     const int64_t q = g * B + c.j;
     const float rg = r[q] - alpha32 * kp;
     r[q] = rg;
@@ -642,6 +683,7 @@ __global__ __launch_bounds__(256) void k_bupdate(
     const float* __restrict__ dinv, float* __restrict__ r, double* __restrict__ part,
     const int32_t* __restrict__ cst, const BCol* __restrict__ bc) {
   __shared__ double lds[GROUPED ? 256 : 1];
+  __shared__ int32_t idx[WAVE_CHUNK ? 4 * (65 + UPD_SL) : 1];
   ChunkIdx c = chunk_of(n1, B, ck);
   if (WAVE_CHUNK) {
     const int ch = __builtin_amdgcn_readfirstlane((int)c.chunk);
@@ -652,8 +694,24 @@ __global__ __launch_bounds__(256) void k_bupdate(
   }
   const bool act = c.valid && cst[c.j] == BCOL_INNER;
   const int64_t nch = nchunks(n1, ck);
+  const int32_t* rp = nullptr;
+  const int32_t* sl = nullptr;
+  if constexpr (WAVE_CHUNK) {
+    int32_t* wr = idx + (threadIdx.x >> 6) * (65 + UPD_SL);
+    // c.valid is uniform over the wave (one chunk per wave); skip when no
+    // lane's column iterates
+    if (c.valid && __any(act) && stage_indices(c, rowptr, slots, wr, wr + 65)) {
+      rp = wr;
+      sl = wr + 65;
+    }
+  }
   double s = 0.0;
-  if (act) update_chunk<WAVE_CHUNK>(n1, B, ck, c, rowptr, slots, Y, dinv, r, bc, s);
+  if (act) {
+    if (rp)
+      update_chunk<WAVE_CHUNK, true>(n1, B, ck, c, rowptr, slots, Y, dinv, r, bc, s, rp, sl);
+    else
+      update_chunk<WAVE_CHUNK, false>(n1, B, ck, c, rowptr, slots, Y, dinv, r, bc, s, rp, sl);
+  }
   put_partials<GROUPED, 1>(part, lds, nch, B, c, {1}, {s}, {false}, {act});
 }
 
