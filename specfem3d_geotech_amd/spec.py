@@ -101,27 +101,6 @@ class SpecMesh:
         rowptr[1:] = np.cumsum(counts)
         return rowptr.astype(np.int32), order.astype(np.int32)
 
-    def colours(self) -> np.ndarray:
-        """Greedy element colouring: no two elements of one colour share a
-        free equation.  Returns (nelmt,) int32 colour ids."""
-        nel = self.nelmt
-        col = np.full(nel, -1, dtype=np.int32)
-        node_elems = [[] for _ in range(self.nnode)]
-        for e in range(nel):
-            for n in self.g_num[e]:
-                node_elems[n].append(e)
-        for e in range(nel):
-            used = set()
-            for n in self.g_num[e]:
-                for f in node_elems[n]:
-                    if col[f] >= 0:
-                        used.add(int(col[f]))
-            c = 0
-            while c in used:
-                c += 1
-            col[e] = c
-        return col
-
 
 def hex2spec(mesh: HexMesh) -> SpecMesh:
     """Build the 27-node SEM mesh and equation numbering of `mesh`."""
@@ -162,14 +141,6 @@ def hex2spec(mesh: HexMesh) -> SpecMesh:
                     g_coord=g_coord, gdof=gdof, gdof_elmt=gdof_elmt,
                     corners=np.ascontiguousarray(corners),
                     mat_id=np.asarray(mesh.mat_id, dtype=np.int32))
-
-
-def structured_colours(mesh: HexMesh) -> np.ndarray:
-    """8-colour (i%2, j%2, k%2) colouring for meshes from `slope_mesh`."""
-    nx, ny, nz = (mesh.meta[k] for k in ("nx", "ny", "nz"))
-    e = np.arange(nx * ny * nz)
-    i, j, k = e % nx, (e // nx) % ny, e // (nx * ny)
-    return ((i % 2) + 2 * (j % 2) + 4 * (k % 2)).astype(np.int32)
 
 
 def hex2spec_device(ctx, mesh: HexMesh) -> SpecMesh:

@@ -1,5 +1,5 @@
-"""Host-side pieces (no GPU): numbering, inverse map, colouring, input
-format round trip, per-trial scalars, and the C-ABI library's exports."""
+"""Host-side pieces (no GPU): numbering, inverse map, input format round
+trip, per-trial scalars, and the C-ABI library's exports."""
 import os
 import re
 import subprocess
@@ -12,7 +12,7 @@ from conftest import ROOT, golden, mesh_from_golden
 from specfem3d_geotech_amd import plastic
 from specfem3d_geotech_amd.mesh import (Material, read_reference_input, slope_mesh,
                                         write_reference_input)
-from specfem3d_geotech_amd.spec import hex2spec, structured_colours
+from specfem3d_geotech_amd.spec import hex2spec
 
 
 def test_inverse_map_reproduces_element_order():
@@ -25,16 +25,6 @@ def test_inverse_map_reproduces_element_order():
         assert (flat[s] == g).all()
         assert (np.diff(s // 81) > 0).all()  # ascending element order
     assert rowptr[-1] == flat.size
-
-
-def test_colourings_are_conflict_free():
-    m = slope_mesh(5, 3, 4)
-    spec = hex2spec(m)
-    for col in (spec.colours(), structured_colours(m)):
-        for c in np.unique(col):
-            nodes = spec.g_num[col == c].ravel()
-            assert np.unique(nodes).size == nodes.size
-    assert structured_colours(m).max() == 7
 
 
 def test_reference_input_round_trip(tmp_path):
