@@ -143,7 +143,11 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
   const int j = n * 16 + (l & 15);
   const bool jv = j < B && wanted(cst, j, want_mask);
   const bool wave_on = __any(jv);
-  const int jl = j < B ? j : B - 1;  // lanes past B read column B-1, store nothing
+  const int jl = j < B ? j : B - 1;
+  // lanes of columns not in the mask (and past B) gather nothing: their B
+  // operand is 0 and their results are not stored.  Their own search
+  // direction can sit in any of the ring's buffers, so a gather there would
+  // fetch lines no wanted column uses.
   // the column's search direction of the iteration being computed
   // (pmask + 1) n1 B < 2^32, checked when the engine is made
   const uint32_t poff = (uint32_t)((bc[jl].it + 1) & bc[jl].pmask) * (uint32_t)nv;
@@ -181,7 +185,8 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
   float bv[21];
   if (wave_on && wi < xe1) {
 #pragma unroll
-    for (int s = 0; s < 21; s++) bv[s] = P[(uint32_t)(sg[0][4 * s + kq] * B + jl) + poff];
+    for (int s = 0; s < 21; s++)
+      bv[s] = jv ? P[(uint32_t)(sg[0][4 * s + kq] * B + jl) + poff] : 0.0f;
   }
   int cur = 0;
   for (; wi < xe1; wi += nbx) {
@@ -301,7 +306,7 @@ void k_bmv_mfma(int32_t nelmt, int B, int nxcd,
     if (wave_on && wn < xe1) {
 #pragma unroll
       for (int s = 0; s < 21; s++)
-        bv[s] = P[(uint32_t)(sg[cur][4 * s + kq] * B + jl) + poff];
+        bv[s] = jv ? P[(uint32_t)(sg[cur][4 * s + kq] * B + jl) + poff] : 0.0f;
     }
     e = en;
   }
